@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Benchmark: RK4 steps/s of 3-D forced HIT at 256^3 on one MI355X.
+
+Workload (one "step"): one RK4 step of the pseudo-spectral Navier-Stokes
+solver at 256^3 in FP32 (fixed-power forcing P=0.1 on 0<|k|<=2, nu=1e-3,
+dt=5e-4, Rogallo init E(k) ~ k^4 exp(-2 (k/4)^2) rescaled to E=0.5), followed
+by the type-2 NUFFT interpolation of the 32^3-filtered velocity at 32^3
+particles and their FP64 Euler shift -- the combined-mode loop body
+(combined.py:97,127,130,199).  The same workload runs again in FP64 for the
+FP32-vs-FP64 speedup and kinetic-energy error at the same horizon.
+
+Prints ONE JSON line.  --gpus above 1 is refused: this build runs on one GPU.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "ql-les-data_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "RK4 steps/s at 256^3 HIT (FP32 vs FP64) @ stated KE error; % HBM roofline"
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_gpu(args, precision, steps, warmup, profile):
+    import numpy as np
+    import torch
+
+    from l3es_amd import _lib, set_up_solver
+    from l3es_amd.integrator import pos_init_cartesian, spectral_interpolator_wrapper
+    from l3es_amd.spectral_solver import diagnostics
+
+    N, n = args.N, args.ckp_N
+    t_setup = time.perf_counter()
+    u, uh, _, _, step, _, _, e0, _ = set_up_solver(
+        N, args.nu, 3, "HIT", args.dt, args.seed, n, args.kf, args.e_kin, "power", precision=precision,
+        spectrum="k4exp", power=args.power, warmup=False)
+    del u
+    torch.cuda.synchronize()
+    log(f"[bench] {precision}: setup {time.perf_counter() - t_setup:.1f}s, E0={e0:.6f}, "
+        f"workspace {step.plan.workspace_bytes / 2**30:.2f} GiB")
+    it = spectral_interpolator_wrapper(n, (1, 2, 3), backend="nufft", precision=precision, tol=args.tol)
+    r = torch.from_numpy(pos_init_cartesian(n, 3)).cuda()
+    ur = torch.empty((r.shape[0], 3), dtype=step.dtype, device=r.device)
+    lib = _lib.load()
+
+    def advance(k):
+        _lib.check(lib.qlles_step_advect(step.plan.handle, it.it.handle, None, _lib.ptr(uh), int(k), _lib.ptr(r),
+                                         r.shape[0], _lib.ptr(ur), None, _lib.stream_ptr()))
+
+    advance(warmup)
+    torch.cuda.synchronize()
+    if profile:
+        _lib.check(lib.qlles_profile(step.plan.handle, 1))
+    t0 = time.perf_counter()
+    advance(steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = None
+    if profile:
+        import ctypes
+
+        ms_a = (ctypes.c_double * 5)()
+        cnt_a = (ctypes.c_int * 5)()
+        by_a = (ctypes.c_double * 5)()
+        _lib.check(lib.qlles_profile_read(step.plan.handle, ms_a, cnt_a, by_a))
+        _lib.check(lib.qlles_profile(step.plan.handle, 0))
+        prof = {_lib.KINDS[k]: dict(ms=ms_a[k], count=cnt_a[k], bytes=by_a[k]) for k in range(5) if cnt_a[k]}
+    sps = steps / (t1 - t0)
+    u_fin = torch.empty((3, N, N, N), dtype=step.dtype, device=uh.device)
+    _lib.check(lib.qlles_irfftn(step.plan.handle, _lib.ptr(uh), _lib.ptr(u_fin), _lib.stream_ptr()))
+    e_fin = float(diagnostics(u_fin, step.plan)[0])
+    finite = bool(torch.isfinite(ur).all()) and np.isfinite(e_fin)
+    log(f"[bench] {precision}: {steps} steps in {t1 - t0:.3f}s -> {sps:.1f} steps/s, E={e_fin:.9f}")
+    del uh, u_fin, step, it
+    torch.cuda.empty_cache()
+    return sps, e_fin, prof, finite
+
+
+def cpu_baseline(args):
+    """The FP64 NumPy/SciPy restatement (oracle) timed on this host's cores, bounded sample."""
+    import numpy as np
+
+    import oracle as O
+
+    cores = len(os.sched_getaffinity(0))
+    fft = O.fft_backend(workers=cores)
+    N = args.cpu_N
+    t_s = time.perf_counter()
+    u, uh, step, _, _ = O.set_up_solver(N, args.nu, 3, "HIT", args.dt, seed=args.seed, kf=args.kf,
+                                        e_kin_target=args.e_kin, forcing_type="power", power=args.power,
+                                        spectrum="k4exp", fft=fft, warmup=False)
+    log(f"[bench] cpu setup {time.perf_counter() - t_s:.1f}s ({cores} cores)")
+    u, uh, _ = step(u, uh)  # warm-up
+    n, t0 = 0, time.perf_counter()
+    while n < args.cpu_max_steps and (n == 0 or time.perf_counter() - t0 < args.cpu_seconds):
+        u, uh, _ = step(u, uh)
+        n += 1
+    dt = time.perf_counter() - t0
+    log(f"[bench] cpu: {n} steps in {dt:.1f}s")
+    del u, uh, np
+    return dict(value=n / dt, unit="steps/s", cores=cores, kind="port",
+                sample=f"{n} RK4 step(s) of the FP64 oracle (NumPy + scipy.fft, workers={cores}) on the same "
+                       f"{N}^3 power-forced HIT field after 1 warm-up step; particles excluded")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--N", type=int, default=256)
+    ap.add_argument("--ckp-N", dest="ckp_N", type=int, default=32)
+    ap.add_argument("--nu", type=float, default=1e-3)
+    ap.add_argument("--dt", type=float, default=5e-4)
+    ap.add_argument("--kf", type=int, default=2)
+    ap.add_argument("--power", type=float, default=0.1)
+    ap.add_argument("--e-kin", dest="e_kin", type=float, default=0.5)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-N", dest="cpu_N", type=int, default=256)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-max-steps", type=int, default=4)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only (the solver grid fits one MI355X; "
+              "multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    torch.cuda.set_device(0)
+    sps32, e32, prof, ok32 = run_gpu(args, "float32", args.steps, args.warmup, profile=True)
+    fp64 = None
+    ke_err = None
+    if not args.no_fp64:
+        k64 = max(10, args.steps // 2)
+        # the FP64 run covers the same horizon (warmup + steps) for the energy comparison
+        sps64, e64, _, ok64 = run_gpu(args, "float64", k64, args.warmup + args.steps - k64, profile=False)
+        ke_err = abs(e32 - e64) / abs(e64)
+        fp64 = dict(steps_per_s=sps64, ms_per_step=1000.0 / sps64, speedup_fp32_over_fp64=sps32 / sps64,
+                    e_kin=e64, finite=ok64)
+    N = args.N
+    s = 4
+    C = 2 * s * N * N * (N // 2 + 1)
+    R = s * N ** 3
+    ledger = 211 * C + 3 * R  # algorithmic bytes per RK4 step, 3 axis passes per transform
+    kernels = {}
+    roof = None
+    if prof:
+        for k, v in prof.items():
+            gbs = v["bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 else 0.0
+            kernels[k] = dict(avg_us=1e3 * v["ms"] / v["count"], launches=v["count"],
+                              alg_bytes_per_launch=v["bytes"] / v["count"], achieved_GBs=gbs,
+                              frac_of_peak=gbs / HBM_PEAK_GBS, share_of_time=0.0)
+        tot = sum(v["ms"] for v in prof.values())
+        for k, v in prof.items():
+            kernels[k]["share_of_time"] = v["ms"] / tot
+        dom = max(prof, key=lambda k: prof[k]["ms"])
+        a = kernels[dom]["achieved_GBs"]
+        roof = dict(bound="hbm", kernel=dom, achieved=a, peak=HBM_PEAK_GBS, unit="GB/s", frac=a / HBM_PEAK_GBS,
+                    traffic=None,
+                    alg_bytes_per_launch=kernels[dom]["alg_bytes_per_launch"], avg_us=kernels[dom]["avg_us"])
+    out = {
+        "metric": METRIC,
+        "value": sps32,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 / sps32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"3-D forced HIT {N}^3 FP32 RK4 step + {args.ckp_N}^3-particle NUFFT advection",
+                   "N": N, "nu": args.nu, "dt": args.dt, "forcing": f"power {args.power} on 0<|k|<={args.kf}",
+                   "init": "Rogallo k^4 exp(-2(k/4)^2), E0=%g, seed %d" % (args.e_kin, args.seed),
+                   "particles": args.ckp_N ** 3, "nufft_tol": args.tol, "fft_precision": "f32",
+                   "reductions": "f64", "positions": "f64"},
+        "roofline": roof,
+        "step_hbm": {"ledger_bytes_per_step": ledger, "ledger_frac": ledger * sps32 / (HBM_PEAK_GBS * 1e9),
+                     "alg_bytes_per_step": (sum(v["bytes"] for v in prof.values()) / args.steps) if prof else None},
+        "kernels": kernels,
+        "fp64": fp64,
+        "ke_rel_err_fp32_vs_fp64": ke_err,
+        "ke_horizon_steps": args.warmup + args.steps,
+        "finite": ok32,
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(args),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

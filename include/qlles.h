@@ -1,0 +1,172 @@
+/*
+ * qlles.h -- C ABI of the MI355X-native spectral Navier-Stokes step and
+ * particle interpolation (gfx950, HIP).
+ *
+ * The reference (tumaer/ql-les-data, package l3es) exposes this path as
+ * closure factories over static parameters that return pure array functions;
+ * each entry point below replaces one of them.  Citations are file:line in
+ * that repository.
+ *
+ * Conventions
+ *  - All array arguments are DEVICE pointers owned by the caller, in the
+ *    reference's C-order, component-major layouts:
+ *        3-D: u (3,N,N,N) real, u_hat (3,N,N,N/2+1) complex (interleaved re,im)
+ *        2-D: u (3,N,N,1) real, u_hat (3,N,N/2+1)   complex, component 2 == 0
+ *    Element type is float or double as chosen by desc.dtype (complex =
+ *    2 consecutive elements).
+ *  - `stream` is a hipStream_t (NULL = default stream).  Calls are
+ *    stream-ordered and asynchronous; they return once work is enqueued.
+ *  - Return value: QLLES_OK or an error code; qlles_last_error() gives a
+ *    thread-local message.  No exceptions cross the ABI.
+ *  - A plan owns its device workspace.  A plan is not thread-safe; separate
+ *    plans on separate streams are.
+ */
+#ifndef QLLES_H
+#define QLLES_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QLLES_OK 0
+#define QLLES_ERR_ARG 1         /* bad argument / unsupported combination */
+#define QLLES_ERR_SHAPE 2       /* unsupported N / dim */
+#define QLLES_ERR_HIP 3         /* HIP runtime error */
+#define QLLES_ERR_NONFINITE 4   /* non-finite value detected by a check */
+
+#define QLLES_F32 0
+#define QLLES_F64 1
+
+#define QLLES_FORCING_NONE 0
+#define QLLES_FORCING_EKIN_TOT 1   /* spectral_solver.py:82-112 */
+#define QLLES_FORCING_POWER 2      /* fixed injection power on 0<|k|<=kf (no reference counterpart) */
+#define QLLES_FORCING_KOLMOGOROV 3 /* f_x = amp sin(k y), linear drag (no reference counterpart) */
+
+#define QLLES_VISC_EXPLICIT 0      /* -nu k^2 u_hat inside RK4, spectral_solver.py:59 */
+#define QLLES_VISC_IF 1            /* integrating factor exp(-nu k^2 t) (no reference counterpart) */
+
+typedef struct qlles_plan qlles_plan;
+
+/* Static parameters of rhs_wrapper(N, nu, fft_axes) and
+ * rk4_wrapper(dt, rhs_fn, forcing_mask, e_kin_init, forcing_type)
+ * (spectral_solver.py:27,65). */
+typedef struct qlles_desc {
+  int dim;            /* 2 or 3 (fft_axes (1,2) or (1,2,3)) */
+  int N;              /* grid points per axis: 8,16,32,64,96,128,256 (3-D); also 512 (2-D) */
+  int dtype;          /* QLLES_F32 or QLLES_F64 */
+  double nu;          /* kinematic viscosity */
+  double dt;          /* RK4 time step */
+  int forcing;        /* QLLES_FORCING_* */
+  int kf;             /* forcing shell radius (ekin_tot / power), 0 < kf < 16 */
+  double e_kin_init;  /* ekin_tot target energy (spectral_solver.py:209) */
+  double power;       /* fixed injection power */
+  int kolm_k;         /* Kolmogorov forcing wavenumber */
+  double kolm_amp;    /* Kolmogorov forcing amplitude */
+  double drag;        /* linear drag coefficient */
+  int viscous;        /* QLLES_VISC_* */
+} qlles_desc;
+
+/* Plan = rhs_wrapper + rk4_wrapper + custom_jit_integrate
+ * (spectral_solver.py:27-48, :65-71, :130-153): validates the descriptor,
+ * builds twiddle tables and allocates the device workspace. */
+int qlles_plan_create(const qlles_desc* desc, qlles_plan** plan);
+int qlles_plan_destroy(qlles_plan* plan);
+/* bytes of device workspace held by the plan */
+size_t qlles_plan_workspace_bytes(const qlles_plan* plan);
+
+/* `nsteps` applications of step_fn(u, u_hat) -> (u, u_hat, e_inj)
+ * (spectral_solver.py:73-118).  u_hat is updated in place.
+ *   u_in   : optional physical field used by the first RK stage (the
+ *            reference's stage 0 uses the passed-in u, :51); NULL = irfftn(u_hat)
+ *   u_out  : optional; receives irfftn(u_hat) after the last step (:114-116)
+ *   e_inj  : optional device array of nsteps doubles (injection rate, :109) */
+int qlles_step(qlles_plan* plan, void* u_out, void* u_hat, const void* u_in, int nsteps,
+               double* e_inj, void* stream);
+
+/* The combined-mode hot loop (combined.py:97,127,130,199): `nsteps` times
+ *   u_hat <- step(u_hat); u_r <- interpolate(spectral_filtering(irfftn(u_hat), n), r);
+ *   r <- (r + dt u_r) mod 2 pi
+ * with the interpolator `it` (n = it's grid, n <= N, same dim and dtype as the
+ * plan).  r: (np, dim) FP64, updated in place; u_r: (np, dim) in the plan dtype
+ * (the last step's velocities); u_out / e_inj as in qlles_step. */
+struct qlles_interp;
+int qlles_step_advect(qlles_plan* plan, struct qlles_interp* it, void* u_out, void* u_hat, int nsteps, double* r,
+                      int np, void* u_r, double* e_inj, void* stream);
+
+/* rhs_fn(u, u_hat, rk) -> du_hat (spectral_solver.py:50-60).  For rk == 0
+ * the physical u is used when non-NULL; otherwise u = irfftn(u_hat). */
+int qlles_rhs(qlles_plan* plan, const void* u, const void* u_hat, int rk, void* du_hat, void* stream);
+
+/* rfftn(u, axes=fft_axes) (spectral_solver.py:206) and irfftn(u_hat)
+ * (:114); unnormalized forward, 1/N^dim inverse as numpy. */
+int qlles_rfftn(qlles_plan* plan, const void* u, void* u_hat, void* stream);
+int qlles_irfftn(qlles_plan* plan, const void* u_hat, void* u, void* stream);
+
+/* FP64 diagnostics of a physical field u (spectral_solver.py:318-320,
+ * :123-127): out[0] = 0.5 <|u|^2>, out[1] = max|u|, out[2..4] = max|u_i| per
+ * component.  `out` is a device array of 5 doubles. */
+int qlles_diagnostics(qlles_plan* plan, const void* u, double* out, void* stream);
+
+/* FP64 dissipation nu <|curl u|^2> through irfftn of i k x u_hat
+ * (utils.py:300-315); out is a device double. */
+int qlles_dissipation(qlles_plan* plan, const void* u_hat, double* out, void* stream);
+
+/* FP64 Parseval energy of u_hat: out[0] = 0.5 <|u|^2>, out[1] = same over
+ * the forcing shell 0 < |k|^2 <= kf^2 (spectral_solver.py:94,:98). */
+int qlles_energy_hat(qlles_plan* plan, const void* u_hat, double* out, void* stream);
+
+/* spectral_filtering(u, ckp_N) (utils.py:100-138) evaluated directly from
+ * u_hat: crop to [-n/2, n/2)^dim, scale (n/N)^dim, take the real part.
+ * u_lres: (3,n,n,n) / (2,n,n) real, n = ckp_N <= N, n even. */
+int qlles_filter(qlles_plan* plan, const void* u_hat, int ckp_N, void* u_lres, void* stream);
+
+/* ---- measurement ----------------------------------------------------------
+ * No reference counterpart (the reference times whole steps with wall-clock
+ * around block_until_ready, spectral_solver.py:310-313).  When enabled, every
+ * kernel launch of the step is bracketed by HIP events on its stream and
+ * charged with the algorithmic bytes it must move.  qlles_profile_read
+ * synchronizes the recorded events, returns per-kind totals and resets them.
+ * Kinds: 0 x-pass (fused RK stage), 1 inverse y-pass, 2 z-pass (c2r / cross
+ * product / r2c), 3 forward y-pass, 4 other. */
+#define QLLES_NKIND 5
+int qlles_profile(qlles_plan* plan, int enable);
+int qlles_profile_read(qlles_plan* plan, double* ms, int* count, double* bytes);
+
+/* ---- particle interpolation (integrator.py:26-93) ------------------------
+ * An interpolator = spectral_interpolator_wrapper(N, fft_axes, splits,
+ * backend) with N = the interpolation grid (ckp_N).  backend:
+ *   QLLES_INTERP_NUFFT : type-2 NUFFT, exponential-of-semicircle kernel
+ *                        (the reference calls finufft nufft{2,3}d2, isign=+1,
+ *                        eps 1e-6, integrator.py:53,55)
+ *   QLLES_INTERP_DFT   : exact direct sum (interpolate_dft, integrator.py:62-91)
+ * Positions r are (np, dim) FP64; velocities u_r are (np, dim) in the plan
+ * dtype. */
+#define QLLES_INTERP_NUFFT 0
+#define QLLES_INTERP_DFT 1
+
+typedef struct qlles_interp qlles_interp;
+
+int qlles_interp_create(int dim, int n, int dtype, int backend, double tol, qlles_interp** out);
+int qlles_interp_destroy(qlles_interp* it);
+
+/* interpolate(u, r) -> u_r from the physical (dim, n,...) field u on the
+ * n-grid (integrator.py:36-60). */
+int qlles_interp_field(qlles_interp* it, const void* u, const double* r, int np, void* u_r, void* stream);
+
+/* Fused filter + interpolate + shift (combined.py:127,130,199):
+ * u_r = interpolate(spectral_filtering(irfftn(u_hat), n), r) straight from the
+ * N-grid spectrum, then, when dt != 0, r <- (r + dt u_r) mod 2 pi in FP64
+ * (shift_fn, integrator.py:21-23).  N is the solver grid. */
+int qlles_interp_hat(qlles_interp* it, int N, const void* u_hat, double* r, int np, void* u_r, double dt,
+                     void* stream);
+
+/* message of the last failed call on this thread */
+const char* qlles_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QLLES_H */

@@ -1,0 +1,75 @@
+// Engine: one solver configuration (dim, N, precision, nu, dt, forcing) with
+// its device workspace; the C ABI plan wraps one of these.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include <functional>
+#include <vector>
+
+namespace ql {
+
+struct Desc {
+  int dim, N, dtype;
+  double nu, dt;
+  int forcing, kf;
+  double e_kin_init, power;
+  int kolm_k;
+  double kolm_amp, drag;
+  int viscous;
+};
+
+class Engine {
+ public:
+  virtual ~Engine() {}
+  virtual size_t workspace_bytes() const = 0;
+  virtual void step(void* u_out, void* uhat, const void* u_in, int nsteps, double* e_inj, hipStream_t s) = 0;
+  virtual void rhs(const void* u, const void* uhat, int rk, void* du, hipStream_t s) = 0;
+  virtual void rfftn(const void* u, void* uhat, hipStream_t s) = 0;
+  virtual void irfftn(const void* uhat, void* u, hipStream_t s) = 0;
+  virtual void diagnostics(const void* u, double* out, hipStream_t s) = 0;
+  virtual void dissipation(const void* uhat, double* out, hipStream_t s) = 0;
+  virtual void energy_hat(const void* uhat, double* out, hipStream_t s) = 0;
+  virtual void filter(const void* uhat, int n, void* ulres, hipStream_t s) = 0;
+
+  // per-kernel-kind timing with HIP events on the launch stream, plus the
+  // algorithmic bytes each launch must move (reads + writes of its arrays)
+  enum Kind { K_XPASS = 0, K_YINV = 1, K_ZPASS = 2, K_YFWD = 3, K_OTHER = 4, NKIND = 5 };
+  void profile_enable(bool on);
+  // called on the step's stream after every completed step (particle advection)
+  std::function<void(hipStream_t)> after_step;
+  void profile_read(double* ms, int* count, double* bytes);
+
+ protected:
+  struct Rec {
+    int kind;
+    double bytes;
+    hipEvent_t a, b;
+  };
+  bool prof_ = false;
+  std::vector<Rec> recs_;
+  std::vector<hipEvent_t> pool_;
+  double acc_ms_[NKIND] = {0, 0, 0, 0, 0};
+  double acc_bytes_[NKIND] = {0, 0, 0, 0, 0};
+  int acc_count_[NKIND] = {0, 0, 0, 0, 0};
+  hipEvent_t take_event();
+  void prof_begin(hipStream_t s, hipEvent_t* a);
+  void prof_end(hipStream_t s, int kind, double bytes, hipEvent_t a);
+  void prof_drain();
+};
+
+Engine* make_engine(const Desc& d);
+template <typename T, int DIM, int N> Engine* make_engine_t(const Desc& d);
+
+// dealias bookkeeping shared with the host: kz kept iff kz < 2/3 (N/2+1)
+// evaluated in double exactly as spectral_solver.py:42 does
+inline int dealias_count(int N) {
+  const double kmax = 2.0 / 3.0 * (double)(N / 2 + 1);
+  int c = 0;
+  for (int k = 0; k <= N / 2; ++k)
+    if ((double)k < kmax) ++c;
+  return c;
+}
+inline int round_up(int a, int m) { return (a + m - 1) / m * m; }
+
+}  // namespace ql

@@ -1,0 +1,433 @@
+// EngineT<T, DIM, N>: launch orchestration for one solver configuration.
+#pragma once
+#include <algorithm>
+#include <vector>
+
+#include "../../include/qlles.h"
+#include "aux_kernels.hpp"
+#include "engine.hpp"
+#include "fftlib.hpp"
+#include "ns2d.hpp"
+
+namespace ql {
+using namespace qk;
+
+template <typename T, int DIM, int N>
+class EngineT final : public Engine {
+  static constexpr int Nh = N / 2 + 1;
+  Desc d_;
+  int KZD, KZP, KYD, LD;
+  size_t rows_;  // N*N (3-D) or N (2-D)
+  size_t bytes_ = 0;
+  std::vector<void*> allocs_;
+  cx<T>*S1a = nullptr, *S1b = nullptr, *S2 = nullptr, *X5 = nullptr, *Y6 = nullptr, *NZ = nullptr, *NY = nullptr;
+  double* partials_ = nullptr;
+  double* alpha_ = nullptr;
+  double* dscr_ = nullptr;
+  const cx<T>* twN_ = nullptr;
+  int nxblk_ = 0;
+
+  template <class P>
+  P* alloc(size_t n) {
+    void* p = nullptr;
+    QL_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(P)));
+    allocs_.push_back(p);
+    bytes_ += n * sizeof(P);
+    return static_cast<P*>(p);
+  }
+
+  StageCtl<T> ctl(int stage) const {
+    static const double a_rk4[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};  // spectral_solver.py:70
+    static const double b_rk4[3] = {0.5, 0.5, 1.0};                              // spectral_solver.py:71
+    StageCtl<T> c{};
+    c.stage = stage;
+    c.visc_if = d_.viscous == QLLES_VISC_IF;
+    c.nu = (T)d_.nu;
+    c.dt = (T)d_.dt;
+    if (stage >= 0 && stage < 4) {
+      c.ca = (T)(a_rk4[stage] * d_.dt);
+      c.cb = stage < 3 ? (T)(b_rk4[stage] * d_.dt) : (T)0;
+    }
+    c.force_power = d_.forcing == QLLES_FORCING_POWER;
+    c.kf2 = d_.kf * d_.kf;
+    c.power = d_.power;
+    if (d_.forcing == QLLES_FORCING_KOLMOGOROV) {
+      // rfft2 of amp sin(k y) on the node grid: -i amp N^2 / 2 at (kx=0, ky=k)
+      c.kolm_k = d_.kolm_k;
+      c.kolm_re = (T)0;
+      c.kolm_im = (T)(-0.5 * d_.kolm_amp * (double)N * (double)N);
+    }
+    c.drag = (T)d_.drag;
+    return c;
+  }
+
+  // ---------------- 3-D launches ----------------
+  double C() const { return (double)rows_ * Nh * 2.0 * sizeof(T); }  // one spectral component
+  double xpass_bytes(const StageCtl<T>& c, bool inv, int ncomp, int nx) const {
+    const double cc = C();
+    double b = 0;
+    if (c.stage < 0) {
+      b += ncomp * cc;  // state
+    } else {
+      const double nyv = (double)ncomp * N * (DIM == 3 ? KYD : 1) * KZD * 2.0 * sizeof(T);
+      b += nyv;                                               // N^ (dealiased modes only)
+      b += ncomp * cc;                                        // u_s
+      if (c.stage == 1 || c.stage == 2) b += 2 * ncomp * cc;  // u_n, acc read
+      if (c.stage == 3) b += ncomp * cc;                      // acc read
+      if (c.stage < 3) b += ncomp * cc;                       // acc write
+      if (c.stage == 4) b -= 0;
+      b += ncomp * cc;                                        // u_{s+1} / du write
+    }
+    if (inv) b += nx * cc;
+    return b;
+  }
+  void x3(NS3X<T>& a, hipStream_t s) {
+    dim3 g((Nh + TI - 1) / TI, N);
+    hipEvent_t e0 = nullptr;
+    prof_begin(s, &e0);
+    hipLaunchKernelGGL((k_ns3_x<T, N>), g, dim3(XCfg<N>::THREADS), 0, s, a);
+    QL_LAUNCHED();
+    prof_end(s, K_XPASS, xpass_bytes(a.c, a.do_inverse, 3, 5), e0);
+  }
+  void yinv3(int nf, hipStream_t s) {
+    NS3Y<T> a{X5, Y6, LD, nf, twN_};
+    dim3 g((Nh + TI - 1) / TI, N);
+    hipEvent_t e0 = nullptr;
+    prof_begin(s, &e0);
+    hipLaunchKernelGGL((k_ns3_yinv<T, N>), g, dim3(XCfg<N>::THREADS), 0, s, a);
+    QL_LAUNCHED();
+    prof_end(s, K_YINV, (nf == 6 ? 11.0 : 6.0) * C(), e0);
+  }
+  void z3(const T* u_in, T* u_out, hipStream_t s) {
+    NS3Z<T> a{Y6, NZ, u_out, u_in, LD, KZD, KZP, twN_};
+    hipEvent_t e0 = nullptr;
+    prof_begin(s, &e0);
+    hipLaunchKernelGGL((k_ns3_z<T, N>), dim3(N * N / ZCfg<N>::P), dim3(ZCfg<N>::THREADS), 0, s, a);
+    QL_LAUNCHED();
+    const double R = (double)N * N * N * sizeof(T);
+    prof_end(s, K_ZPASS, (u_in ? 3.0 * C() + 3.0 * R : 6.0 * C()) + (u_out ? 3.0 * R : 0.0) +
+                             3.0 * N * N * KZD * 2.0 * sizeof(T), e0);
+  }
+  void yfwd3(hipStream_t s) {
+    NS3YF<T> a{NZ, NY, KZD, KZP, KYD, twN_};
+    dim3 g((KZD + TI - 1) / TI, N);
+    hipEvent_t e0 = nullptr;
+    prof_begin(s, &e0);
+    hipLaunchKernelGGL((k_ns3_yfwd<T, N>), g, dim3(XCfg<N>::THREADS), 0, s, a);
+    QL_LAUNCHED();
+    prof_end(s, K_YFWD, 3.0 * N * (N + KYD) * KZD * 2.0 * sizeof(T), e0);
+  }
+  NS3X<T> xargs3(cx<T>* S0) const {
+    NS3X<T> a{};
+    a.ny = NY;
+    a.un = S0;
+    a.ld_un = Nh;
+    a.acc = S2;
+    a.x5 = X5;
+    a.ld = LD;
+    a.KZD = KZD;
+    a.KZP = KZP;
+    a.KYD = KYD;
+    a.tw = twN_;
+    return a;
+  }
+
+  // ---------------- 2-D launches ----------------
+  void x2(NS2X<T>& a, hipStream_t s) {
+    hipEvent_t e0 = nullptr;
+    prof_begin(s, &e0);
+    hipLaunchKernelGGL((k_ns2_x<T, N>), dim3((Nh + TI - 1) / TI), dim3(XCfg<N>::THREADS), 0, s, a);
+    QL_LAUNCHED();
+    prof_end(s, K_XPASS, xpass_bytes(a.c, a.do_inverse, 2, 3), e0);
+  }
+  void y2(const T* u_in, T* u_out, hipStream_t s) {
+    NS2Y<T> a{X5, NY, u_out, u_in, LD, KZD, KZP, twN_};
+    hipEvent_t e0 = nullptr;
+    prof_begin(s, &e0);
+    hipLaunchKernelGGL((k_ns2_y<T, N>), dim3((N + ZCfg<N>::P - 1) / ZCfg<N>::P), dim3(ZCfg<N>::THREADS), 0, s,
+                       a);
+    QL_LAUNCHED();
+    const double R = (double)N * N * sizeof(T);
+    prof_end(s, K_ZPASS, (u_in ? C() + 2.0 * R : 3.0 * C()) + (u_out ? 2.0 * R : 0.0) + 2.0 * N * KZD * 2.0 * sizeof(T),
+             e0);
+  }
+  NS2X<T> xargs2(cx<T>* S0) const {
+    NS2X<T> a{};
+    a.ny = NY;
+    a.un = S0;
+    a.ld_un = Nh;
+    a.acc = S2;
+    a.x3 = X5;
+    a.ld = LD;
+    a.KZD = KZD;
+    a.KZP = KZP;
+    a.tw = twN_;
+    return a;
+  }
+
+  void reduce_phys(const T* u, int ncomp, double factor, double* out, int nout, hipStream_t s) {
+    const size_t npts = DIM == 3 ? (size_t)N * N * N : (size_t)N * N;
+    hipLaunchKernelGGL((k_reduce_phys<T>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, u, npts, ncomp, partials_);
+    QL_LAUNCHED();
+    hipLaunchKernelGGL(k_reduce_phys_final, dim3(1), dim3(RED_THREADS), 0, s, partials_, RED_BLOCKS, (double)npts,
+                       dscr_);
+    QL_LAUNCHED();
+    // dscr_[0] = 0.5 sum/npts
+    hipLaunchKernelGGL(k_scale_copy, dim3(1), dim3(32), 0, s, dscr_, out, nout, 2.0 * factor);
+    QL_LAUNCHED();
+  }
+
+  // out-of-place inverse of a [ncomp][rows][ld_in] rfft-layout spectrum on this grid
+  void irfftn_into(const cx<T>* src, int ld_in, cx<T>* tmp, T* dst, int ncomp, hipStream_t s) {
+    const T scale = (T)1 / (T)(DIM == 3 ? (double)N * N * N : (double)N * N);
+    if (DIM == 3) {
+      c2c_strided<T>(N, src, tmp, ncomp, N * ld_in, false, (T)1, twN_, s);
+      c2c_strided<T>(N, tmp, tmp, ncomp * N, ld_in, false, (T)1, twN_, s);
+      c2r_contig<T>(N, tmp, dst, ncomp * N * N, ld_in, scale, twN_, s);
+    } else {
+      c2c_strided<T>(N, src, tmp, ncomp, ld_in, false, (T)1, twN_, s);
+      c2r_contig<T>(N, tmp, dst, ncomp * N, ld_in, scale, twN_, s);
+    }
+  }
+
+ public:
+  explicit EngineT(const Desc& d) : d_(d) {
+    KZD = dealias_count(N);
+    KZP = round_up(KZD, 16);
+    KYD = 2 * KZD - 1;
+    LD = round_up(Nh, 16);
+    rows_ = DIM == 3 ? (size_t)N * N : (size_t)N;
+    const size_t fsz = rows_ * LD;
+    if (DIM == 3) {
+      S1a = alloc<cx<T>>(3 * fsz);
+      S1b = alloc<cx<T>>(3 * fsz);
+      S2 = alloc<cx<T>>(3 * fsz);
+      X5 = alloc<cx<T>>(5 * fsz);
+      Y6 = alloc<cx<T>>(6 * fsz);
+      NZ = alloc<cx<T>>((size_t)3 * N * N * KZP);
+      NY = alloc<cx<T>>((size_t)3 * N * KYD * KZP);
+      nxblk_ = (Nh + TI - 1) / TI * N;
+    } else {
+      S1a = alloc<cx<T>>(2 * fsz);
+      S1b = alloc<cx<T>>(2 * fsz);
+      S2 = alloc<cx<T>>(2 * fsz);
+      X5 = alloc<cx<T>>(3 * fsz);
+      Y6 = alloc<cx<T>>(3 * fsz);  // scratch for transforms / diagnostics
+      NY = alloc<cx<T>>((size_t)2 * N * KZP);
+      nxblk_ = (Nh + TI - 1) / TI;
+    }
+    partials_ = alloc<double>(std::max(RED_BLOCKS * 4, nxblk_ * 2));
+    alpha_ = alloc<double>(8);
+    dscr_ = alloc<double>(8);
+    twN_ = twiddles<T>(N);
+  }
+  ~EngineT() override {
+    prof_drain();
+    for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+    for (void* p : allocs_) (void)hipFree(p);
+  }
+  size_t workspace_bytes() const override { return bytes_; }
+
+  void step(void* u_out_v, void* uhat, const void* u_in_v, int nsteps, double* e_inj, hipStream_t s) override {
+    cx<T>* S0 = static_cast<cx<T>*>(uhat);
+    T* u_out = static_cast<T*>(u_out_v);
+    const T* u_in = static_cast<const T*>(u_in_v);
+    const bool ekin = d_.forcing == QLLES_FORCING_EKIN_TOT;
+    if (DIM == 3) {
+      NS3X<T> a = xargs3(S0);
+      a.c = ctl(-1);
+      a.us = S0;
+      a.ld_us = Nh;
+      a.do_inverse = 1;
+      x3(a, s);
+      for (int n = 0; n < nsteps; ++n) {
+        const bool more = (n < nsteps - 1) || (u_out != nullptr);
+        for (int st = 0; st < 4; ++st) {
+          yinv3(6, s);
+          z3((n == 0 && st == 0) ? u_in : nullptr, nullptr, s);
+          yfwd3(s);
+          a = xargs3(S0);
+          a.c = ctl(st);
+          a.us = st == 0 ? S0 : (st == 2 ? S1b : S1a);
+          a.ld_us = st == 0 ? Nh : LD;
+          a.uout = st == 0 ? S1a : (st == 1 ? S1b : (st == 2 ? S1a : S0));
+          a.ld_uout = st == 3 ? Nh : LD;
+          a.do_inverse = (st < 3) || (!ekin && more);
+          if (st == 3 && ekin) a.c.partials = partials_;
+          x3(a, s);
+        }
+        if (ekin) {
+          const double n6 = (double)N * N * N * (double)N * N * N;
+          hipLaunchKernelGGL(k_ekin_alpha, dim3(1), dim3(RED_THREADS), 0, s, partials_, nxblk_, 1.0 / n6,
+                             d_.e_kin_init, d_.dt, alpha_, e_inj ? e_inj + n : nullptr);
+          QL_LAUNCHED();
+          a = xargs3(S0);
+          a.c = ctl(-1);
+          a.c.alpha = alpha_;
+          a.rescale_dst = S0;
+          a.us = S0;
+          a.ld_us = Nh;
+          a.do_inverse = more ? 1 : 0;
+          x3(a, s);
+        } else if (e_inj) {
+          hipLaunchKernelGGL(k_fill_double, dim3(1), dim3(32), 0, s, e_inj + n, 1, 0.0);
+          QL_LAUNCHED();
+        }
+        if (after_step) after_step(s);
+      }
+      if (u_out) {
+        yinv3(3, s);
+        c2r_contig<T>(N, Y6, u_out, 3 * N * N, LD, (T)1 / ((T)N * (T)N * (T)N), twN_, s);
+      }
+    } else {
+      if (ekin) throw std::invalid_argument("ekin_tot forcing is defined for 3-D HIT only");
+      NS2X<T> a = xargs2(S0);
+      a.c = ctl(-1);
+      a.us = S0;
+      a.ld_us = Nh;
+      a.do_inverse = 1;
+      x2(a, s);
+      for (int n = 0; n < nsteps; ++n) {
+        const bool more = (n < nsteps - 1) || (u_out != nullptr);
+        for (int st = 0; st < 4; ++st) {
+          y2((n == 0 && st == 0) ? u_in : nullptr, nullptr, s);
+          a = xargs2(S0);
+          a.c = ctl(st);
+          a.us = st == 0 ? S0 : (st == 2 ? S1b : S1a);
+          a.ld_us = st == 0 ? Nh : LD;
+          a.uout = st == 0 ? S1a : (st == 1 ? S1b : (st == 2 ? S1a : S0));
+          a.ld_uout = st == 3 ? Nh : LD;
+          a.do_inverse = (st < 3) || more;
+          x2(a, s);
+        }
+        if (e_inj) {
+          hipLaunchKernelGGL(k_fill_double, dim3(1), dim3(32), 0, s, e_inj + n, 1, 0.0);
+          QL_LAUNCHED();
+        }
+        if (after_step) after_step(s);
+      }
+      if (u_out) {
+        c2r_contig<T>(N, X5, u_out, 2 * N, LD, (T)1 / ((T)N * (T)N), twN_, s);
+        hipLaunchKernelGGL((k_fill<T>), dim3(64), dim3(256), 0, s, u_out + (size_t)2 * N * N, (size_t)N * N, (T)0);
+        QL_LAUNCHED();
+      }
+    }
+  }
+
+  void rhs(const void* u_v, const void* uhat_v, int rk, void* du_v, hipStream_t s) override {
+    cx<T>* uh = const_cast<cx<T>*>(static_cast<const cx<T>*>(uhat_v));
+    cx<T>* du = static_cast<cx<T>*>(du_v);
+    const T* u = (rk == 0) ? static_cast<const T*>(u_v) : nullptr;
+    if (DIM == 3) {
+      NS3X<T> a = xargs3(uh);
+      a.c = ctl(-1);
+      a.us = uh;
+      a.ld_us = Nh;
+      a.do_inverse = 1;
+      x3(a, s);
+      yinv3(6, s);
+      z3(u, nullptr, s);
+      yfwd3(s);
+      a = xargs3(uh);
+      a.c = ctl(4);
+      a.us = uh;
+      a.ld_us = Nh;
+      a.uout = du;
+      a.ld_uout = Nh;
+      a.do_inverse = 0;
+      x3(a, s);
+    } else {
+      NS2X<T> a = xargs2(uh);
+      a.c = ctl(-1);
+      a.us = uh;
+      a.ld_us = Nh;
+      a.do_inverse = 1;
+      x2(a, s);
+      y2(u, nullptr, s);
+      a = xargs2(uh);
+      a.c = ctl(4);
+      a.us = uh;
+      a.ld_us = Nh;
+      a.uout = du;
+      a.ld_uout = Nh;
+      a.do_inverse = 0;
+      x2(a, s);
+      hipLaunchKernelGGL((k_fill<T>), dim3(16), dim3(256), 0, s, reinterpret_cast<T*>(du + (size_t)2 * N * Nh),
+                         (size_t)2 * N * Nh, (T)0);
+      QL_LAUNCHED();
+    }
+  }
+
+  void rfftn(const void* u_v, void* uhat_v, hipStream_t s) override {
+    const T* u = static_cast<const T*>(u_v);
+    cx<T>* uh = static_cast<cx<T>*>(uhat_v);
+    if (DIM == 3) {
+      r2c_contig<T>(N, u, uh, 3 * N * N, Nh, (T)1, twN_, s);
+      c2c_strided<T>(N, uh, uh, 3 * N, Nh, true, (T)1, twN_, s);
+      c2c_strided<T>(N, uh, uh, 3, N * Nh, true, (T)1, twN_, s);
+    } else {
+      r2c_contig<T>(N, u, uh, 3 * N, Nh, (T)1, twN_, s);
+      c2c_strided<T>(N, uh, uh, 3, Nh, true, (T)1, twN_, s);
+    }
+  }
+
+  void irfftn(const void* uhat_v, void* u_v, hipStream_t s) override {
+    irfftn_into(static_cast<const cx<T>*>(uhat_v), Nh, Y6, static_cast<T*>(u_v), 3, s);
+  }
+
+  void diagnostics(const void* u_v, double* out, hipStream_t s) override {
+    reduce_phys(static_cast<const T*>(u_v), 3, 0.5, out, 5, s);
+  }
+
+  void dissipation(const void* uhat_v, double* out, hipStream_t s) override {
+    const cx<T>* uh = static_cast<const cx<T>*>(uhat_v);
+    const size_t total = rows_ * Nh;
+    const int nb = (int)std::min<size_t>((total + 255) / 256, 4096);
+    hipLaunchKernelGGL((k_curl_hat<T>), dim3(nb), dim3(256), 0, s, uh, Y6, N, DIM, Nh, Nh);
+    QL_LAUNCHED();
+    T* om = reinterpret_cast<T*>(X5);
+    irfftn_into(Y6, Nh, Y6, om, 3, s);
+    reduce_phys(om, 3, d_.nu, out, 1, s);
+  }
+
+  void energy_hat(const void* uhat_v, double* out, hipStream_t s) override {
+    const double nd = DIM == 3 ? (double)N * N * N : (double)N * N;
+    hipLaunchKernelGGL((k_energy_hat<T>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s,
+                       static_cast<const cx<T>*>(uhat_v), N, DIM, Nh, d_.kf * d_.kf, partials_);
+    QL_LAUNCHED();
+    hipLaunchKernelGGL(k_energy_hat_final, dim3(1), dim3(RED_THREADS), 0, s, partials_, RED_BLOCKS, 1.0 / (nd * nd),
+                       out);
+    QL_LAUNCHED();
+  }
+
+  void filter(const void* uhat_v, int n, void* ulres_v, hipStream_t s) override {
+    if (n > N || (n % 2) != 0 || !supported_len(n)) throw std::invalid_argument("unsupported ckp_N");
+    const int nh = n / 2 + 1;
+    const int ncomp = DIM == 3 ? 3 : 2;
+    const size_t rows = DIM == 3 ? (size_t)n * n : (size_t)n;
+    const size_t total = rows * nh * ncomp;
+    const int nb = (int)std::min<size_t>((total + 255) / 256, 8192);
+    cx<T>* g = Y6;  // [ncomp][rows][nh]
+    hipLaunchKernelGGL((k_window<T>), dim3(nb), dim3(256), 0, s, static_cast<const cx<T>*>(uhat_v), N, DIM, Nh, g,
+                       n, nh);
+    QL_LAUNCHED();
+    const cx<T>* twn = twiddles<T>(n);
+    const T scale = (T)1 / (T)(DIM == 3 ? (double)N * N * N : (double)N * N);
+    T* dst = static_cast<T*>(ulres_v);
+    if (DIM == 3) {
+      c2c_strided<T>(n, g, g, 3, n * nh, false, (T)1, twn, s);
+      c2c_strided<T>(n, g, g, 3 * n, nh, false, (T)1, twn, s);
+      c2r_contig<T>(n, g, dst, 3 * n * n, nh, scale, twn, s);
+    } else {
+      c2c_strided<T>(n, g, g, 2, nh, false, (T)1, twn, s);
+      c2r_contig<T>(n, g, dst, 2 * n, nh, scale, twn, s);
+    }
+  }
+};
+
+template <typename T, int DIM, int N>
+Engine* make_engine_t(const Desc& d) {
+  return new EngineT<T, DIM, N>(d);
+}
+
+}  // namespace ql

@@ -1,0 +1,361 @@
+// Pencil FFT engine for gfx950 (CDNA4).
+//
+// A "pencil" is one 1-D line of an array that is transformed; a workgroup
+// transforms many pencils at once, each by TP cooperating lanes.  The engine
+// is a mixed-radix Stockham autosort FFT:
+//   * pass 0 takes its inputs from registers (the caller loads them straight
+//     from HBM in the pass-0 order, so no LDS staging on the way in),
+//   * every later pass is one LDS round trip (write, barrier, read, barrier),
+//   * the last pass leaves its outputs in registers in "final order", so the
+//     caller can fuse pointwise work and store straight to HBM.
+// The inverse plan is the forward plan's radix list reversed, which makes the
+// forward transform's output register order equal to the inverse transform's
+// input register order: a forward FFT, pointwise math, and an inverse FFT can
+// run back to back without touching LDS in between.
+//
+// Conventions: forward = exp(-2 pi i n k / M), inverse = exp(+2 pi i n k / M),
+// both unnormalized.  Twiddles come from a table tw[i] = exp(-2 pi i i / TN)
+// (computed in FP64 on the host), usually staged in LDS.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qf {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cx {
+  T re, im;
+};
+
+template <typename T> __device__ __forceinline__ cx<T> cadd(cx<T> a, cx<T> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename T> __device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename T> __device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+// a * conj(b)
+template <typename T> __device__ __forceinline__ cx<T> cmulc(cx<T> a, cx<T> b) {
+  return {a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im};
+}
+template <typename T> __device__ __forceinline__ cx<T> cconj(cx<T> a) { return {a.re, -a.im}; }
+template <typename T> __device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { return {a.re * s, a.im * s}; }
+template <typename T> __device__ __forceinline__ cx<T> cneg(cx<T> a) { return {-a.re, -a.im}; }
+// multiply by +i
+template <typename T> __device__ __forceinline__ cx<T> cmuli(cx<T> a) { return {-a.im, a.re}; }
+// multiply by -i (FWD) or +i (!FWD)
+template <typename T, bool FWD> __device__ __forceinline__ cx<T> crot(cx<T> a) {
+  if constexpr (FWD) return {a.im, -a.re};
+  else return {-a.im, a.re};
+}
+
+// ---------------------------------------------------------------------------
+// compile-time cos/sin of 2 pi k / R (Taylor series after reduction to
+// [-pi/4, pi/4]); used only for the constant in-register DFT twiddles.
+// ---------------------------------------------------------------------------
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr double taylor_sin(double x) {
+  double t = x, s = x;
+  for (int n = 1; n < 14; ++n) { t *= -x * x / ((2.0 * n) * (2.0 * n + 1.0)); s += t; }
+  return s;
+}
+constexpr double taylor_cos(double x) {
+  double t = 1.0, s = 1.0;
+  for (int n = 1; n < 14; ++n) { t *= -x * x / ((2.0 * n - 1.0) * (2.0 * n)); s += t; }
+  return s;
+}
+// cos / sin of 2 pi k / R using the octant symmetries on 8k/R
+constexpr double cos2pi(int k, int R) {
+  k = ((k % R) + R) % R;
+  // angle = 2 pi k / R; reflect to [0, pi/4]
+  const double f = (double)k / (double)R;  // in [0,1)
+  if (f <= 0.125) return taylor_cos(2 * kPi * f);
+  if (f <= 0.25) return taylor_sin(2 * kPi * (0.25 - f));
+  if (f <= 0.375) return -taylor_sin(2 * kPi * (f - 0.25));
+  if (f <= 0.5) return -taylor_cos(2 * kPi * (0.5 - f));
+  if (f <= 0.625) return -taylor_cos(2 * kPi * (f - 0.5));
+  if (f <= 0.75) return -taylor_sin(2 * kPi * (0.75 - f));
+  if (f <= 0.875) return taylor_sin(2 * kPi * (f - 0.75));
+  return taylor_cos(2 * kPi * (1.0 - f));
+}
+constexpr double sin2pi(int k, int R) { return cos2pi(4 * k - R, 4 * R); }
+
+// multiply by W_R^K = exp(-+2 pi i K/R) (sign by FWD); trivial factors folded
+template <typename T, bool FWD, int K, int R>
+__device__ __forceinline__ cx<T> twc(cx<T> a) {
+  constexpr int k = ((K % R) + R) % R;
+  if constexpr (k == 0) {
+    return a;
+  } else if constexpr (2 * k == R) {
+    return cneg(a);
+  } else if constexpr (4 * k == R) {
+    return crot<T, FWD>(a);
+  } else if constexpr (4 * k == 3 * R) {
+    return crot<T, !FWD>(a);
+  } else {
+    constexpr double c = cos2pi(k, R);
+    constexpr double s = sin2pi(k, R);
+    const T cc = (T)c;
+    const T ss = FWD ? (T)(-s) : (T)s;  // W = c + i*ss
+    return {a.re * cc - a.im * ss, a.re * ss + a.im * cc};
+  }
+}
+
+// ---------------------------------------------------------------------------
+// in-register small DFTs on v[0..R) (stride S in the register array)
+// ---------------------------------------------------------------------------
+template <typename T, bool FWD, int R> struct DFT;
+
+template <typename T, bool FWD> struct DFT<T, FWD, 1> {
+  template <int S = 1> __device__ __forceinline__ static void run(cx<T>*) {}
+};
+
+template <typename T, bool FWD> struct DFT<T, FWD, 2> {
+  template <int S = 1> __device__ __forceinline__ static void run(cx<T>* v) {
+    cx<T> a = v[0], b = v[S];
+    v[0] = cadd(a, b);
+    v[S] = csub(a, b);
+  }
+};
+
+template <typename T, bool FWD> struct DFT<T, FWD, 3> {
+  template <int S = 1> __device__ __forceinline__ static void run(cx<T>* v) {
+    constexpr double s3 = 0.86602540378443864676372317075293618;  // sin(2 pi/3)
+    cx<T> a = v[0], b = v[S], c = v[2 * S];
+    cx<T> t = cadd(b, c);
+    cx<T> m = {a.re - (T)0.5 * t.re, a.im - (T)0.5 * t.im};
+    cx<T> d = csub(b, c);
+    // s = -i*s3*d (FWD) or +i*s3*d
+    cx<T> s = crot<T, FWD>(cscale(d, (T)s3));
+    v[0] = cadd(a, t);
+    v[S] = cadd(m, s);
+    v[2 * S] = csub(m, s);
+  }
+};
+
+template <typename T, bool FWD> struct DFT<T, FWD, 4> {
+  template <int S = 1> __device__ __forceinline__ static void run(cx<T>* v) {
+    cx<T> a0 = cadd(v[0], v[2 * S]);
+    cx<T> a1 = csub(v[0], v[2 * S]);
+    cx<T> a2 = cadd(v[S], v[3 * S]);
+    cx<T> a3 = crot<T, FWD>(csub(v[S], v[3 * S]));
+    v[0] = cadd(a0, a2);
+    v[2 * S] = csub(a0, a2);
+    v[S] = cadd(a1, a3);
+    v[3 * S] = csub(a1, a3);
+  }
+};
+
+// R = 8: radix-2 DIT over two 4-point DFTs
+template <typename T, bool FWD> struct DFT<T, FWD, 8> {
+  template <int S = 1> __device__ __forceinline__ static void run(cx<T>* v) {
+    cx<T> e[4] = {v[0], v[2 * S], v[4 * S], v[6 * S]};
+    cx<T> o[4] = {v[S], v[3 * S], v[5 * S], v[7 * S]};
+    DFT<T, FWD, 4>::template run<1>(e);
+    DFT<T, FWD, 4>::template run<1>(o);
+    o[1] = twc<T, FWD, 1, 8>(o[1]);
+    o[2] = twc<T, FWD, 2, 8>(o[2]);
+    o[3] = twc<T, FWD, 3, 8>(o[3]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      v[k * S] = cadd(e[k], o[k]);
+      v[(k + 4) * S] = csub(e[k], o[k]);
+    }
+  }
+};
+
+// R = 16: 4 x 4 with internal twiddles W16^(n2 k1)
+template <typename T, bool FWD> struct DFT<T, FWD, 16> {
+  template <int S = 1> __device__ __forceinline__ static void run(cx<T>* v) {
+    cx<T> a[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+      cx<T> t[4] = {v[n2 * S], v[(n2 + 4) * S], v[(n2 + 8) * S], v[(n2 + 12) * S]};
+      DFT<T, FWD, 4>::template run<1>(t);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) a[n2 * 4 + k1] = t[k1];
+    }
+    a[1 * 4 + 1] = twc<T, FWD, 1, 16>(a[1 * 4 + 1]);
+    a[1 * 4 + 2] = twc<T, FWD, 2, 16>(a[1 * 4 + 2]);
+    a[1 * 4 + 3] = twc<T, FWD, 3, 16>(a[1 * 4 + 3]);
+    a[2 * 4 + 1] = twc<T, FWD, 2, 16>(a[2 * 4 + 1]);
+    a[2 * 4 + 2] = twc<T, FWD, 4, 16>(a[2 * 4 + 2]);
+    a[2 * 4 + 3] = twc<T, FWD, 6, 16>(a[2 * 4 + 3]);
+    a[3 * 4 + 1] = twc<T, FWD, 3, 16>(a[3 * 4 + 1]);
+    a[3 * 4 + 2] = twc<T, FWD, 6, 16>(a[3 * 4 + 2]);
+    a[3 * 4 + 3] = twc<T, FWD, 9, 16>(a[3 * 4 + 3]);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      cx<T> t[4] = {a[0 * 4 + k1], a[1 * 4 + k1], a[2 * 4 + k1], a[3 * 4 + k1]};
+      DFT<T, FWD, 4>::template run<1>(t);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[(k1 + 4 * k2) * S] = t[k2];
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// radix lists
+// ---------------------------------------------------------------------------
+template <int... Rs> struct Radices {
+  static constexpr int n = sizeof...(Rs);
+  static constexpr int at(int p) {
+    constexpr int a[] = {Rs...};
+    return a[p];
+  }
+  static constexpr int prod_before(int p) {
+    constexpr int a[] = {Rs...};
+    int s = 1;
+    for (int q = 0; q < p; ++q) s *= a[q];
+    return s;
+  }
+  static constexpr int total() { return prod_before(n); }
+};
+
+template <typename A, typename B> struct RevImpl;
+template <int... Acc> struct RevImpl<Radices<Acc...>, Radices<>> { using type = Radices<Acc...>; };
+template <int... Acc, int R, int... Rest> struct RevImpl<Radices<Acc...>, Radices<R, Rest...>> {
+  using type = typename RevImpl<Radices<R, Acc...>, Radices<Rest...>>::type;
+};
+template <typename L> using Reverse = typename RevImpl<Radices<>, L>::type;
+
+// default radix plan (forward order) and lanes per pencil, by length M
+template <int M> struct Plan;
+template <> struct Plan<4> { using R = Radices<4>; static constexpr int TP = 1; };
+template <> struct Plan<8> { using R = Radices<8>; static constexpr int TP = 1; };
+template <> struct Plan<16> { using R = Radices<4, 4>; static constexpr int TP = 4; };
+template <> struct Plan<32> { using R = Radices<4, 8>; static constexpr int TP = 4; };
+template <> struct Plan<48> { using R = Radices<3, 16>; static constexpr int TP = 4; };
+template <> struct Plan<64> { using R = Radices<8, 8>; static constexpr int TP = 8; };
+template <> struct Plan<96> { using R = Radices<3, 4, 8>; static constexpr int TP = 8; };
+template <> struct Plan<128> { using R = Radices<8, 16>; static constexpr int TP = 16; };
+template <> struct Plan<256> { using R = Radices<16, 16>; static constexpr int TP = 16; };
+template <> struct Plan<512> { using R = Radices<8, 8, 8>; static constexpr int TP = 32; };
+
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+// ---------------------------------------------------------------------------
+// PFFT: M-point FFT of one pencil by TP lanes (lane index j in [0,TP)).
+//   in : v[t*R0 + r] = x[(j + TP*t) + (M/R0)*r]          (pass-0 order)
+//   out: v[t*RL + r] = X[(j + TP*t) + (M/RL)*r]          (final order)
+// Slots whose butterfly index is >= M/R of that pass are ignored.
+// lds points at this pencil's element 0; element n lives at lds[n*ls].
+// tw is a table exp(-2 pi i q / TN) with TN = M * TWS.
+// Every thread of the workgroup must call run() (it contains barriers) unless
+// the plan has a single pass.
+// ---------------------------------------------------------------------------
+template <typename T, bool FWD, int M, int TP, typename RL>
+struct PFFT {
+  static constexpr int NP = RL::n;
+  static_assert(RL::total() == M, "radix plan does not multiply to M");
+  static constexpr int R(int p) { return RL::at(p); }
+  static constexpr int B(int p) { return M / RL::at(p); }
+  static constexpr int NB(int p) { return (B(p) + TP - 1) / TP; }
+  static constexpr int maxe_impl(int p) { return p >= NP ? 0 : cmax(NB(p) * R(p), maxe_impl(p + 1)); }
+  static constexpr int MAXE = maxe_impl(0);
+  static constexpr int NIN = NB(0) * R(0);
+  static constexpr int NOUT = NB(NP - 1) * R(NP - 1);
+  static constexpr int B0 = B(0);
+  static constexpr int BL = B(NP - 1);
+  static constexpr int RF = R(NP - 1);
+  static constexpr int NBL = NB(NP - 1);
+
+  // element index held in input slot s
+  __device__ __forceinline__ static int in_index(int j, int s) {
+    const int t = s / R(0), r = s % R(0);
+    return (j + TP * t) + B(0) * r;
+  }
+  __device__ __forceinline__ static bool in_valid(int j, int s) { return (j + TP * (s / R(0))) < B(0); }
+  __device__ __forceinline__ static int out_index(int j, int s) {
+    const int t = s / RF, r = s % RF;
+    return (j + TP * t) + BL * r;
+  }
+  __device__ __forceinline__ static bool out_valid(int j, int s) { return (j + TP * (s / RF)) < BL; }
+
+  template <int p>
+  __device__ __forceinline__ static void butterflies(cx<T>* v, int j, const cx<T>* tw, int twstride) {
+    constexpr int Rp = R(p);
+    constexpr int Ns = RL::prod_before(p);
+#pragma unroll
+    for (int t = 0; t < NB(p); ++t) {
+      const int b = j + TP * t;
+      if constexpr (Ns > 1) {
+        const int bm = b % Ns;
+        // W_{Ns*Rp}^{bm*r} = tw[bm*r*(M/(Ns*Rp)) * twstride]
+        const int step = bm * (M / (Ns * Rp)) * twstride;
+#pragma unroll
+        for (int r = 1; r < Rp; ++r) {
+          cx<T> w = tw[step * r];
+          v[t * Rp + r] = FWD ? cmul(v[t * Rp + r], w) : cmulc(v[t * Rp + r], w);
+        }
+      }
+      DFT<T, FWD, Rp>::template run<1>(v + t * Rp);
+    }
+  }
+
+  template <int p>
+  __device__ __forceinline__ static void write_lds(const cx<T>* v, cx<T>* lds, int ls, int j) {
+    constexpr int Rp = R(p);
+    constexpr int Ns = RL::prod_before(p);
+#pragma unroll
+    for (int t = 0; t < NB(p); ++t) {
+      const int b = j + TP * t;
+      if ((B(p) % TP == 0) || b < B(p)) {
+        const int base = (b / Ns) * Ns * Rp + (b % Ns);
+#pragma unroll
+        for (int r = 0; r < Rp; ++r) lds[(base + Ns * r) * ls] = v[t * Rp + r];
+      }
+    }
+  }
+
+  template <int p>
+  __device__ __forceinline__ static void read_lds(cx<T>* v, const cx<T>* lds, int ls, int j) {
+    constexpr int Rp = R(p);
+#pragma unroll
+    for (int t = 0; t < NB(p); ++t) {
+      const int b = j + TP * t;
+      if ((B(p) % TP == 0) || b < B(p)) {
+#pragma unroll
+        for (int r = 0; r < Rp; ++r) v[t * Rp + r] = lds[(b + B(p) * r) * ls];
+      }
+    }
+  }
+
+  template <int p>
+  __device__ __forceinline__ static void pass_from(cx<T>* v, cx<T>* lds, int ls, int j, const cx<T>* tw, int tws) {
+    if constexpr (p < NP) {
+      write_lds<p - 1>(v, lds, ls, j);
+      __syncthreads();
+      read_lds<p>(v, lds, ls, j);
+      __syncthreads();
+      butterflies<p>(v, j, tw, tws);
+      pass_from<p + 1>(v, lds, ls, j, tw, tws);
+    }
+  }
+
+  // v must have MAXE slots
+  __device__ __forceinline__ static void run(cx<T>* v, cx<T>* lds, int ls, int j, const cx<T>* tw, int tws) {
+    butterflies<0>(v, j, tw, tws);
+    pass_from<1>(v, lds, ls, j, tw, tws);
+  }
+};
+
+// forward / inverse plans sharing register order at the hand-off point
+template <typename T, int M, int TP>
+using FwdFFT = PFFT<T, true, M, TP, typename Plan<M>::R>;
+template <typename T, int M, int TP>
+using InvFFT = PFFT<T, false, M, TP, Reverse<typename Plan<M>::R>>;
+// the mirror pair: inverse first (plan order), then forward (reversed)
+template <typename T, int M, int TP>
+using InvFFTp = PFFT<T, false, M, TP, typename Plan<M>::R>;
+template <typename T, int M, int TP>
+using FwdFFTr = PFFT<T, true, M, TP, Reverse<typename Plan<M>::R>>;
+
+// fftfreq(N) value of index i (Nyquist index N/2 maps to -N/2)
+__device__ __forceinline__ int kfreq(int i, int N) { return i < (N + 1) / 2 ? i : i - N; }
+
+// copy a twiddle table of n entries into LDS (caller synchronizes)
+template <typename T>
+__device__ __forceinline__ void stage_table(cx<T>* dst, const cx<T>* src, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
+}  // namespace qf

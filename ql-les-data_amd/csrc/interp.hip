@@ -1,0 +1,451 @@
+// Grid -> particle interpolation u_r(x) = Re sum_{k in [-n/2,n/2)^d} G_k e^{i k.x}
+// (integrator.py:36-91; finufft nufft{2,3}d2 with isign=+1 at integrator.py:53,55).
+//
+// Two backends:
+//  * NUFFT (type 2): the coefficients G_k are divided by the kernel's Fourier
+//    transform, zero-padded onto a fine grid of nf = 2n points per axis,
+//    inverse FFT'd with the library's own pencil FFT, and gathered at each
+//    particle with the exponential-of-semicircle kernel
+//        phi(z) = exp(beta (sqrt(1 - z^2) - 1)),  |z| <= 1,
+//    of width w = ceil(log10(1/tol)) + 1 grid points and beta = 2.30 w
+//    (Barnett, Magland & af Klinteberg, SIAM J. Sci. Comput. 41 (2019)).
+//    One 64-lane wavefront gathers one particle: lanes own (dy, dz) stencil
+//    columns and loop over dx, then reduce across the wave.
+//  * DFT: the exact direct sum (interpolate_dft, integrator.py:62-91), one
+//    particle per lane with the coefficient planes staged through LDS and
+//    FP64 phases.
+// G_k comes either from a physical n-grid field (interpolate(u, r)) or, fused
+// with spectral_filtering (utils.py:100-138), straight from the solver's N-grid
+// rfft spectrum: G_k = (F_k + F_k*) / (2 N^d), F the Hermitian extension of
+// u_hat and k* = k with every -n/2 component replaced by +n/2 (the .real
+// projection of the asymmetric crop).  After interpolation an optional FP64
+// Euler shift r <- (r + dt u_r) mod 2 pi (shift_fn, integrator.py:21-23).
+#include <cmath>
+#include <vector>
+
+#include "aux_kernels.hpp"
+#include "fftlib.hpp"
+#include "interp.hpp"
+
+struct qlles_interp {
+  int dim, n, dtype, backend;
+  double tol;
+  int w;        // kernel width (grid points)
+  double beta;  // kernel shape
+  int nf;       // fine grid points per axis
+  void* corr;   // [n] deconvolution factors (T)
+  void* coef;   // NUFFT: [ncomp][nf^d] zero-padded coefficients; DFT: [ncomp][n^d] shifted coefficients
+  void* fine;   // NUFFT: [ncomp][nf^d] fine-grid values
+  void* tmp;    // [ncomp][n^(d-1)][n/2+1] rfft of a physical n-grid field
+};
+
+namespace ql {
+using namespace qk;
+
+namespace {
+
+// Gauss-Legendre nodes/weights on [-1, 1] (Newton on P_m)
+void gauss_legendre(int m, std::vector<double>& x, std::vector<double>& wts) {
+  x.resize(m);
+  wts.resize(m);
+  for (int i = 0; i < m; ++i) {
+    double z = std::cos(M_PI * (i + 0.75) / (m + 0.5));
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;
+      for (int k = 2; k <= m; ++k) {
+        const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      const double dp = m * (z * p1 - p0) / (z * z - 1.0);
+      const double dz = p1 / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-16) break;
+    }
+    double p0 = 1.0, p1 = z;
+    for (int k = 2; k <= m; ++k) {
+      const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+      p0 = p1;
+      p1 = p2;
+    }
+    const double dp = m * (z * p1 - p0) / (z * z - 1.0);
+    x[i] = z;
+    wts[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T es_kernel(T z, T beta) {
+  const T a = (T)1 - z * z;
+  return a > (T)0 ? exp(beta * (sqrt(a) - (T)1)) : (T)0;
+}
+
+__device__ __forceinline__ double wrap2pi(double v) {
+  const double L = 6.283185307179586476925286766559;
+  v = v - L * floor(v / L);
+  return v >= L ? v - L : v;
+}
+
+// coefficients G_k from a source rfft spectrum (grid Ns, row stride ld)
+template <typename T>
+__global__ void k_build_coef(const cx<T>* __restrict__ uh, int Ns, int ld, int dim, int n, double scale,
+                             const T* __restrict__ corr, cx<T>* __restrict__ out, int mode, int nf) {
+  const int ncomp = dim;
+  const size_t per = dim == 3 ? (size_t)n * n * n : (size_t)n * n;
+  const size_t cs_in = (dim == 3 ? (size_t)Ns * Ns : (size_t)Ns) * ld;
+  const int h = n / 2;
+  for (size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x; m < per * ncomp; m += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(m / per);
+    const size_t q = m % per;
+    const cx<T>* u = uh + c * cs_in;
+    cx<T> g;
+    int ix, iy, iz = 0;
+    if (dim == 3) {
+      ix = (int)(q / ((size_t)n * n));
+      iy = (int)((q / n) % n);
+      iz = (int)(q % n);
+      const int kx = ix - h, ky = iy - h, kz = iz - h;
+      const int sx = kx == -h ? h : kx, sy = ky == -h ? h : ky, sz = kz == -h ? h : kz;
+      g = cscale(cadd(full_spectrum3(u, Ns, ld, kx, ky, kz), full_spectrum3(u, Ns, ld, sx, sy, sz)), (T)(0.5 * scale));
+      if (mode == 0) {
+        g = cscale(g, corr[ix] * corr[iy] * corr[iz]);
+        const size_t o = (((size_t)c * nf + (kx + nf) % nf) * nf + (ky + nf) % nf) * nf + (kz + nf) % nf;
+        out[o] = g;
+      } else {
+        out[(size_t)c * per + q] = g;
+      }
+    } else {
+      ix = (int)(q / n);
+      iy = (int)(q % n);
+      const int kx = ix - h, ky = iy - h;
+      const int sx = kx == -h ? h : kx, sy = ky == -h ? h : ky;
+      g = cscale(cadd(full_spectrum2(u, Ns, ld, kx, ky), full_spectrum2(u, Ns, ld, sx, sy)), (T)(0.5 * scale));
+      if (mode == 0) {
+        g = cscale(g, corr[ix] * corr[iy]);
+        const size_t o = ((size_t)c * nf + (kx + nf) % nf) * nf + (ky + nf) % nf;
+        out[o] = g;
+      } else {
+        out[(size_t)c * per + q] = g;
+      }
+    }
+  }
+}
+
+// one wavefront per particle; lanes own (dy, dz) columns of the w^3 stencil
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+k_gather(const cx<T>* __restrict__ f, int nf, int w, T beta, const double* __restrict__ r, int np,
+         T* __restrict__ ur, double dt, double* __restrict__ r_out) {
+  const int lane = threadIdx.x & 63;
+  const int pid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (pid >= np) return;  // whole wavefront exits together
+  const double h = 6.283185307179586476925286766559 / nf;
+  const double halfw = 0.5 * w;
+  const T two_over_w = (T)(2.0 / w);
+  double t[DIM];
+  int i0[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    t[d] = r[(size_t)pid * DIM + d] / h;
+    i0[d] = (int)ceil(t[d] - halfw);
+  }
+  T acc[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) acc[d] = 0;
+  const size_t cs = DIM == 3 ? (size_t)nf * nf * nf : (size_t)nf * nf;
+  if (DIM == 3) {
+    const int dy = lane / w, dz = lane % w;
+    if (lane < w * w) {
+      const int iy = i0[1] + dy, iz = i0[2] + dz;
+      const T wy = es_kernel((T)(t[1] - iy) * two_over_w, beta);
+      const T wz = es_kernel((T)(t[2] - iz) * two_over_w, beta);
+      const T wyz = wy * wz;
+      const int jy = ((iy % nf) + nf) % nf, jz = ((iz % nf) + nf) % nf;
+      for (int dx = 0; dx < w; ++dx) {
+        const int ix = i0[0] + dx;
+        const T wx = es_kernel((T)(t[0] - ix) * two_over_w, beta) * wyz;
+        const int jx = ((ix % nf) + nf) % nf;
+        const size_t o = ((size_t)jx * nf + jy) * nf + jz;
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) acc[c] += wx * f[c * cs + o].re;
+      }
+    }
+  } else {
+    const int dy = lane;
+    if (lane < w) {
+      const int iy = i0[1] + dy;
+      const T wy = es_kernel((T)(t[1] - iy) * two_over_w, beta);
+      const int jy = ((iy % nf) + nf) % nf;
+      for (int dx = 0; dx < w; ++dx) {
+        const int ix = i0[0] + dx;
+        const T wx = es_kernel((T)(t[0] - ix) * two_over_w, beta) * wy;
+        const int jx = ((ix % nf) + nf) % nf;
+        const size_t o = (size_t)jx * nf + jy;
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) acc[c] += wx * f[c * cs + o].re;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      ur[(size_t)pid * DIM + c] = acc[c];
+      if (r_out) r_out[(size_t)pid * DIM + c] = wrap2pi(r[(size_t)pid * DIM + c] + dt * (double)acc[c]);
+    }
+  }
+}
+
+// exact direct sum; one particle per lane, coefficient planes through LDS
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+k_dft(const cx<T>* __restrict__ G, int n, const double* __restrict__ r, int np, T* __restrict__ ur, double dt,
+      double* __restrict__ r_out) {
+  extern __shared__ unsigned char s_raw[];
+  cx<T>* s_plane = reinterpret_cast<cx<T>*>(s_raw);  // [DIM][n^(DIM-1)]
+  const int pid = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = pid < np;
+  double x[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) x[d] = ok ? r[(size_t)pid * DIM + d] : 0.0;
+  const int h = n / 2;
+  const size_t plane = DIM == 3 ? (size_t)n * n : (size_t)n;
+  const size_t per = plane * n;
+  double acc[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) acc[d] = 0.0;
+  for (int ixk = 0; ixk < n; ++ixk) {
+    __syncthreads();
+    for (size_t e = threadIdx.x; e < plane * DIM; e += blockDim.x) {
+      const size_t c = e / plane, q = e % plane;
+      s_plane[e] = G[c * per + (size_t)ixk * plane + q];
+    }
+    __syncthreads();
+    double sx, cxv;
+    sincos((double)(ixk - h) * x[0], &sx, &cxv);
+    if (DIM == 3) {
+      double ez_s, ez_c, s0, c0;
+      sincos(x[2], &ez_s, &ez_c);
+      sincos(-(double)h * x[2], &s0, &c0);
+      for (int iyk = 0; iyk < n; ++iyk) {
+        double sy, cy;
+        sincos((double)(iyk - h) * x[1], &sy, &cy);
+        const double pr = cxv * cy - sx * sy, pi = cxv * sy + sx * cy;  // e^{i(kx x + ky y)}
+        double zr = pr * c0 - pi * s0, zi = pr * s0 + pi * c0;       // times e^{-i h z}
+        const cx<T>* row = s_plane + (size_t)iyk * n;
+        for (int izk = 0; izk < n; ++izk) {
+#pragma unroll
+          for (int c = 0; c < DIM; ++c) {
+            const cx<T> g = row[c * plane + izk];
+            acc[c] += (double)g.re * zr - (double)g.im * zi;
+          }
+          const double t = zr * ez_c - zi * ez_s;
+          zi = zr * ez_s + zi * ez_c;
+          zr = t;
+        }
+      }
+    } else {
+      double ey_s, ey_c, s0, c0;
+      sincos(x[1], &ey_s, &ey_c);
+      sincos(-(double)h * x[1], &s0, &c0);
+      double zr = cxv * c0 - sx * s0, zi = cxv * s0 + sx * c0;
+      for (int iyk = 0; iyk < n; ++iyk) {
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+          const cx<T> g = s_plane[c * plane + iyk];
+          acc[c] += (double)g.re * zr - (double)g.im * zi;
+        }
+        const double t = zr * ey_c - zi * ey_s;
+        zi = zr * ey_s + zi * ey_c;
+        zr = t;
+      }
+    }
+  }
+  if (ok) {
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      ur[(size_t)pid * DIM + c] = (T)acc[c];
+      if (r_out) r_out[(size_t)pid * DIM + c] = wrap2pi(x[c] + dt * (double)(T)acc[c]);
+    }
+  }
+}
+
+size_t tsize(int dtype) { return dtype == QLLES_F32 ? sizeof(float) : sizeof(double); }
+
+template <typename T>
+void setup_t(qlles_interp* it) {
+  const int n = it->n, d = it->dim;
+  const size_t per_f = d == 3 ? (size_t)it->nf * it->nf * it->nf : (size_t)it->nf * it->nf;
+  const size_t per_n = d == 3 ? (size_t)n * n * n : (size_t)n * n;
+  // deconvolution factors corr(k) = h / psi^(k), psi^(k) = (w h/2) int_{-1}^{1} phi(z) cos(k z w h/2) dz
+  std::vector<T> corr(n, (T)1);
+  if (it->backend == QLLES_INTERP_NUFFT) {
+    std::vector<double> gx, gw;
+    gauss_legendre(256, gx, gw);
+    const double h = 2.0 * M_PI / it->nf;
+    const double a = 0.5 * it->w * h;
+    for (int i = 0; i < n; ++i) {
+      const double k = i - n / 2;
+      double s = 0.0;
+      for (size_t q = 0; q < gx.size(); ++q) {
+        const double z = gx[q];
+        const double ph = std::exp(it->beta * (std::sqrt(std::max(0.0, 1.0 - z * z)) - 1.0));
+        s += gw[q] * ph * std::cos(k * z * a);
+      }
+      corr[i] = (T)(h / (a * s));
+    }
+  }
+  QL_HIP(hipMalloc(&it->corr, sizeof(T) * n));
+  QL_HIP(hipMemcpy(it->corr, corr.data(), sizeof(T) * n, hipMemcpyHostToDevice));
+  if (it->backend == QLLES_INTERP_NUFFT) {
+    QL_HIP(hipMalloc(&it->coef, sizeof(cx<T>) * per_f * d));
+    QL_HIP(hipMemset(it->coef, 0, sizeof(cx<T>) * per_f * d));
+    QL_HIP(hipMalloc(&it->fine, sizeof(cx<T>) * per_f * d));
+  } else {
+    QL_HIP(hipMalloc(&it->coef, sizeof(cx<T>) * per_n * d));
+  }
+  const size_t nrows = d == 3 ? (size_t)n * n : (size_t)n;
+  QL_HIP(hipMalloc(&it->tmp, sizeof(cx<T>) * nrows * (n / 2 + 1) * d));
+}
+
+template <typename T>
+void run_t(qlles_interp* it, const cx<T>* uh, int Ns, int ld, double scale, const double* r, int np, T* ur,
+           double dt, double* r_out, hipStream_t s) {
+  const int n = it->n, d = it->dim;
+  const size_t per_n = d == 3 ? (size_t)n * n * n : (size_t)n * n;
+  const int nb = (int)std::min<size_t>((per_n * d + 255) / 256, 8192);
+  const int mode = it->backend == QLLES_INTERP_NUFFT ? 0 : 1;
+  hipLaunchKernelGGL((k_build_coef<T>), dim3(nb), dim3(256), 0, s, uh, Ns, ld, d, n, scale,
+                     static_cast<const T*>(it->corr), static_cast<cx<T>*>(it->coef), mode, it->nf);
+  QL_LAUNCHED();
+  if (np == 0) return;
+  if (mode == 0) {
+    const int nf = it->nf;
+    const cx<T>* twf = twiddles<T>(nf);
+    cx<T>* co = static_cast<cx<T>*>(it->coef);
+    cx<T>* fi = static_cast<cx<T>*>(it->fine);
+    if (d == 3) {
+      c2c_strided<T>(nf, co, fi, 3, nf * nf, false, (T)1, twf, s);
+      c2c_strided<T>(nf, fi, fi, 3 * nf, nf, false, (T)1, twf, s);
+      c2c_contig<T>(nf, fi, fi, 3 * nf * nf, false, (T)1, twf, s);
+      hipLaunchKernelGGL((k_gather<T, 3>), dim3((np + 3) / 4), dim3(256), 0, s, fi, nf, it->w, (T)it->beta, r, np,
+                         ur, dt, r_out);
+    } else {
+      c2c_strided<T>(nf, co, fi, 2, nf, false, (T)1, twf, s);
+      c2c_contig<T>(nf, fi, fi, 2 * nf, false, (T)1, twf, s);
+      hipLaunchKernelGGL((k_gather<T, 2>), dim3((np + 3) / 4), dim3(256), 0, s, fi, nf, it->w, (T)it->beta, r, np,
+                         ur, dt, r_out);
+    }
+    QL_LAUNCHED();
+  } else {
+    const size_t shm = sizeof(cx<T>) * (d == 3 ? (size_t)n * n : (size_t)n) * d;
+    if (d == 3)
+      hipLaunchKernelGGL((k_dft<T, 3>), dim3((np + 255) / 256), dim3(256), shm, s,
+                         static_cast<const cx<T>*>(it->coef), n, r, np, ur, dt, r_out);
+    else
+      hipLaunchKernelGGL((k_dft<T, 2>), dim3((np + 255) / 256), dim3(256), shm, s,
+                         static_cast<const cx<T>*>(it->coef), n, r, np, ur, dt, r_out);
+    QL_LAUNCHED();
+  }
+}
+
+template <typename T>
+void field_t(qlles_interp* it, const T* u, const double* r, int np, T* ur, hipStream_t s) {
+  // n-grid rfft of the physical field, then the same coefficient builder with Ns = n
+  const int n = it->n, d = it->dim;
+  const int nh = n / 2 + 1;
+  cx<T>* uh = static_cast<cx<T>*>(it->tmp);
+  const cx<T>* twn = twiddles<T>(n);
+  if (d == 3) {
+    r2c_contig<T>(n, u, uh, 3 * n * n, nh, (T)1, twn, s);
+    c2c_strided<T>(n, uh, uh, 3 * n, nh, true, (T)1, twn, s);
+    c2c_strided<T>(n, uh, uh, 3, n * nh, true, (T)1, twn, s);
+    run_t<T>(it, uh, n, nh, 1.0 / ((double)n * n * n), r, np, ur, 0.0, nullptr, s);
+  } else {
+    r2c_contig<T>(n, u, uh, 2 * n, nh, (T)1, twn, s);
+    c2c_strided<T>(n, uh, uh, 2, nh, true, (T)1, twn, s);
+    run_t<T>(it, uh, n, nh, 1.0 / ((double)n * n), r, np, ur, 0.0, nullptr, s);
+  }
+}
+
+}  // namespace
+
+qlles_interp* make_interp(int dim, int n, int dtype, int backend, double tol) {
+  if (!supported_len(n)) throw std::invalid_argument("interpolation grid n must be one of 8,16,32,64,96,128,256");
+  auto* it = new qlles_interp{};
+  it->dim = dim;
+  it->n = n;
+  it->dtype = dtype;
+  it->backend = backend;
+  it->tol = tol;
+  if (backend == QLLES_INTERP_NUFFT) {
+    it->nf = 2 * n;
+    if (!supported_len(it->nf)) {
+      delete it;
+      throw std::invalid_argument("NUFFT fine grid 2n unsupported for this n");
+    }
+    if (dim == 3 && n > 64) {
+      delete it;
+      throw std::invalid_argument("3-D NUFFT supports n <= 64");
+    }
+    it->w = (int)std::ceil(std::log10(1.0 / tol)) + 1;
+    if (it->w < 2) it->w = 2;
+    if (it->w > 8) it->w = 8;  // 8*8 (dy,dz) columns fill one wavefront
+    const double bo = it->w == 2 ? 2.20 : it->w == 3 ? 2.26 : it->w == 4 ? 2.38 : 2.30;
+    it->beta = bo * it->w;
+  } else {
+    if (dim == 3 && n > (dtype == QLLES_F32 ? 64 : 32)) {
+      delete it;
+      throw std::invalid_argument("3-D DFT interpolation supports n <= 64 (FP32) / 32 (FP64)");
+    }
+    it->nf = n;
+    it->w = 0;
+    it->beta = 0;
+  }
+  try {
+    if (dtype == QLLES_F32) setup_t<float>(it);
+    else setup_t<double>(it);
+  } catch (...) {
+    destroy_interp(it);
+    throw;
+  }
+  return it;
+}
+
+int interp_dim(const qlles_interp* it) { return it->dim; }
+int interp_dtype(const qlles_interp* it) { return it->dtype; }
+int interp_n(const qlles_interp* it) { return it->n; }
+
+void destroy_interp(qlles_interp* it) {
+  if (!it) return;
+  if (it->corr) (void)hipFree(it->corr);
+  if (it->coef) (void)hipFree(it->coef);
+  if (it->fine) (void)hipFree(it->fine);
+  if (it->tmp) (void)hipFree(it->tmp);
+  delete it;
+}
+
+void interp_field(qlles_interp* it, const void* u, const double* r, int np, void* u_r, hipStream_t s) {
+  if (it->dtype == QLLES_F32)
+    field_t<float>(it, static_cast<const float*>(u), r, np, static_cast<float*>(u_r), s);
+  else
+    field_t<double>(it, static_cast<const double*>(u), r, np, static_cast<double*>(u_r), s);
+}
+
+void interp_hat(qlles_interp* it, int N, const void* u_hat, double* r, int np, void* u_r, double dt, hipStream_t s) {
+  if (N < it->n || (N % 2) != 0) throw std::invalid_argument("solver grid N must be even and >= n");
+  const int nh = N / 2 + 1;
+  const double scale = 1.0 / (it->dim == 3 ? (double)N * N * N : (double)N * N);
+  double* r_out = dt != 0.0 ? r : nullptr;
+  if (it->dtype == QLLES_F32)
+    run_t<float>(it, static_cast<const cx<float>*>(u_hat), N, nh, scale, r, np, static_cast<float*>(u_r), dt, r_out, s);
+  else
+    run_t<double>(it, static_cast<const cx<double>*>(u_hat), N, nh, scale, r, np, static_cast<double*>(u_r), dt,
+                  r_out, s);
+}
+
+}  // namespace ql

@@ -1,0 +1,15 @@
+// Particle interpolation of the spectral velocity (integrator.py:26-93).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/qlles.h"
+
+namespace ql {
+qlles_interp* make_interp(int dim, int n, int dtype, int backend, double tol);
+void destroy_interp(qlles_interp* it);
+void interp_field(qlles_interp* it, const void* u, const double* r, int np, void* u_r, hipStream_t s);
+int interp_dim(const qlles_interp* it);
+int interp_dtype(const qlles_interp* it);
+int interp_n(const qlles_interp* it);
+void interp_hat(qlles_interp* it, int N, const void* u_hat, double* r, int np, void* u_r, double dt, hipStream_t s);
+}  // namespace ql

@@ -1,0 +1,573 @@
+// Fused 3-D pseudo-spectral Navier-Stokes RK4 step (rotational form).
+//
+// Reference semantics: l3es/spectral_solver.py:27-120 (rhs_wrapper, rk4_wrapper).
+// One RHS evaluation is four kernels; each reads and writes HBM once:
+//
+//   X  (x-pass)   forward x-FFT of the dealiased nonlinear term N^ (3 comps),
+//                 dealias in kx, pressure projection, viscous term, forcing,
+//                 RK4 stage update, then the inverse x-FFT of the new stage
+//                 field and of the two curl components that need kx:
+//                   X5 = Fx^-1 [u^, v^, w^, i(kz u^ - kx w^), i(kx v^ - ky u^)]
+//   Yi (y-inv)    inverse y-FFT of the five fields plus i(ky w - kz v) -> 6
+//   Z  (z-pass)   6 x c2r along z, cross product u x omega in registers,
+//                 3 x r2c along z, keep kz < kmax (2/3 rule)
+//   Yf (y-fwd)    forward y-FFT of the 3 components, keep |ky| < kmax
+//
+// Array layouts (complex, component-major, z/kz fastest):
+//   state u^        [3][N][N][Nh]   (the caller's array, Nh = N/2+1)
+//   stage/acc/X5/Y6 [f][N][N][LD]   (LD = Nh rounded up to 16: tile rows
+//                                    start on 128-byte boundaries in FP32)
+//   NZ              [3][N][N][KZP]  (KZP = KZD rounded up to 16)
+//   NY              [3][N][KYD][KZP] (only the |ky| < kmax rows)
+// where KZD = #{kz >= 0 : kz < kmax}, KYD = 2*KZD - 1 and
+// kmax = 2/3 (N/2 + 1) exactly as spectral_solver.py:42 computes it.
+#pragma once
+#include "fft_kernels.hpp"
+
+namespace qk {
+
+// RK / forcing controls shared by the 3-D and 2-D x-pass kernels
+template <typename T>
+struct StageCtl {
+  int stage;      // -1: load state and inverse-transform only; 0..3: RK stage; 4: write du (rhs only)
+  int visc_if;    // 0: explicit viscous term (reference); 1: integrating factor
+  T nu, dt;
+  T ca, cb;       // explicit RK: a_s*dt, b_s*dt
+  // fixed-power forcing on 0 < |k|^2 <= kf2: du += P/(2 E_f + eps) * u_s
+  int force_power;
+  int kf2;
+  double power;
+  // Kolmogorov forcing (2-D): du_x(kx=0, ky=kolm_k) += -i amp N^2/2 ; drag: du -= drag*u_s
+  int kolm_k;
+  T kolm_re, kolm_im;
+  T drag;
+  // ekin_tot: stage 3 writes FP64 partial sums (E, E_low) per block
+  double* partials;
+  // stage -1 with rescale: masked modes multiplied by *alpha and written back
+  const double* alpha;
+};
+
+template <typename T>
+struct NS3X {
+  const cx<T>* ny;   // [3][N][KYD][KZP]
+  const cx<T>* us;   // stage field u_s (row stride ld_us)
+  const cx<T>* un;   // u_n (row stride ld_un)
+  cx<T>* uout;       // u_{s+1} / u_{n+1} / du (row stride ld_uout)
+  cx<T>* acc;        // RK accumulator (row stride ld)
+  cx<T>* x5;         // [5][N][N][ld]
+  cx<T>* rescale_dst;  // stage -1 + alpha: where masked modes are written back (row stride ld_un)
+  int ld_us, ld_un, ld_uout, ld;
+  int KZD, KZP, KYD;
+  int do_inverse;
+  const cx<T>* tw;
+  StageCtl<T> c;
+};
+
+template <int N>
+struct XCfg {
+  static constexpr int TP = Plan<N>::TP;
+  static constexpr int THREADS = TI * TP;
+  static constexpr int LS = TI + LPAD;
+};
+
+__device__ __forceinline__ bool keep_k(int idx, int N, int KZD) { return idx < KZD || idx > N - KZD; }
+
+// deterministic block sum of NV doubles per thread (fixed tree order)
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* s_red) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) s_red[q * nt + tid] = v[q];
+  __syncthreads();
+  for (int h = nt / 2; h > 0; h >>= 1) {
+    if (tid < h) {
+#pragma unroll
+      for (int q = 0; q < NV; ++q) s_red[q * nt + tid] += s_red[q * nt + tid + h];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = s_red[q * nt];
+  __syncthreads();
+}
+
+// E_f = 0.5 * sum_{0<|k|^2<=kf2} w |u^|^2 (unnormalized; the caller divides by N^6)
+template <typename T, int N>
+__device__ double forced_shell_energy3(const cx<T>* us, int ld, int kf2, double* s_red) {
+  constexpr int Nh = N / 2 + 1;
+  int kf = 0;
+  while ((kf + 1) * (kf + 1) <= kf2) ++kf;
+  const int nx = 2 * kf + 1, nz = kf + 1;
+  const int cnt = nx * nx * nz;
+  double acc[1] = {0.0};
+  for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+    const int kx = q / (nx * nz) - kf, ky = (q / nz) % nx - kf, kz = q % nz;
+    const int k2 = kx * kx + ky * ky + kz * kz;
+    if (k2 > 0 && k2 <= kf2) {
+      const int ix = (kx + N) % N, iy = (ky + N) % N;
+      const double w = (kz == 0 || kz == N / 2) ? 1.0 : 2.0;
+      double e = 0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const cx<T> a = us[((size_t)(c * N + ix) * N + iy) * ld + kz];
+        e += (double)a.re * a.re + (double)a.im * a.im;
+      }
+      acc[0] += w * e;
+    }
+  }
+  (void)Nh;
+  block_sum<1>(acc, s_red);
+  return 0.5 * acc[0];
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__(XCfg<N>::THREADS)
+k_ns3_x(NS3X<T> a) {
+  using C = XCfg<N>;
+  constexpr int TP = C::TP;
+  constexpr int Nh = N / 2 + 1;
+  using FF = FwdFFT<T, N, TP>;
+  using IF = InvFFT<T, N, TP>;
+  static_assert(FF::NOUT == IF::NIN, "fft hand-off mismatch");
+  constexpr int NS = FF::NOUT;
+  __shared__ cx<T> s_tw[N];
+  __shared__ cx<T> s_buf[N * C::LS];
+  __shared__ double s_red[2 * C::THREADS];
+
+  const int tid = threadIdx.x, p = tid % TI, j = tid / TI;
+  const int kz = blockIdx.x * TI + p;
+  const bool ok = kz < Nh;
+  const int kyi = blockIdx.y;
+  const int ky = kfreq(kyi, N);
+  const int kzv = kz;  // rfft axis: index N/2 carries +N/2 (spectral_solver.py:34-36)
+  stage_table(s_tw, a.tw, N);
+  __syncthreads();
+  const StageCtl<T>& c = a.c;
+
+  cx<T> w[3][NS] = {};  // new stage field (input of the inverse x-FFT)
+  if (c.stage >= 0) {
+    // ---- forward x-FFT of the dealiased nonlinear term ----
+    const int KZD = a.KZD, KYD = a.KYD;
+    const bool row_kept = keep_k(kyi, N, KZD);
+    const int kyd = kyi < KZD ? kyi : kyi - (N - KYD);
+    const bool fwd_ok = ok && kz < KZD && row_kept;
+    cx<T> nh[3][NS];
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+      cx<T> v[FF::MAXE];
+#pragma unroll
+      for (int s = 0; s < FF::NIN; ++s) {
+        v[s] = (fwd_ok && FF::in_valid(j, s))
+                   ? a.ny[((size_t)(cc * N + FF::in_index(j, s)) * KYD + kyd) * a.KZP + kz]
+                   : cx<T>{0, 0};
+      }
+      FF::run(v, s_buf + p, C::LS, j, s_tw, 1);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) nh[cc][s] = v[s];
+    }
+
+    // ---- forcing normalisation (FP64) ----
+    T fscale = 0;
+    if (c.force_power && kz < TI && blockIdx.x == 0) {
+      // only blocks that hold forced modes need E_f (uniform branch per block)
+      if (ky * ky <= c.kf2) {
+        const double ef = forced_shell_energy3<T, N>(a.us, a.ld_us, c.kf2, s_red) /
+                          ((double)N * N * N * (double)N * N * N);
+        fscale = (T)(c.power / (2.0 * ef + 2.220446049250313e-16));
+      }
+    }
+
+    // ---- pointwise: dealias kx, projection, viscous, forcing, RK update ----
+    double epart[2] = {0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int kxi = FF::out_index(j, s);
+      const int kx = kfreq(kxi, N);
+      const bool valid = ok && FF::out_valid(j, s);
+      const bool keep_x = keep_k(kxi, N, a.KZD);
+      cx<T> n0 = keep_x ? nh[0][s] : cx<T>{0, 0};
+      cx<T> n1 = keep_x ? nh[1][s] : cx<T>{0, 0};
+      cx<T> n2 = keep_x ? nh[2][s] : cx<T>{0, 0};
+      const int k2 = kx * kx + ky * ky + kzv * kzv;
+      const T inv = k2 == 0 ? (T)0 : (T)1 / (T)k2;
+      // p^ = sum_c N^_c k_c / k^2 (spectral_solver.py:57)
+      const cx<T> pr = {(n0.re * (T)kx + n1.re * (T)ky + n2.re * (T)kzv) * inv,
+                        (n0.im * (T)kx + n1.im * (T)ky + n2.im * (T)kzv) * inv};
+      cx<T> du[3];
+      du[0] = csub(n0, cscale(pr, (T)kx));
+      du[1] = csub(n1, cscale(pr, (T)ky));
+      du[2] = csub(n2, cscale(pr, (T)kzv));
+      cx<T> us3[3] = {{0, 0}, {0, 0}, {0, 0}};
+      if (valid) {
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) us3[cc] = a.us[((size_t)(cc * N + kxi) * N + kyi) * a.ld_us + kz];
+      }
+      const T nuk2 = c.nu * (T)k2;
+      const bool forced = fscale != (T)0 && k2 > 0 && k2 <= c.kf2;
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        if (!c.visc_if) du[cc] = csub(du[cc], cscale(us3[cc], nuk2));  // spectral_solver.py:59
+        if (forced) du[cc] = cadd(du[cc], cscale(us3[cc], fscale));
+      }
+      if (!valid) continue;
+      const size_t i_ld = ((size_t)kxi * N + kyi) * a.ld + kz;
+      const size_t i_un = ((size_t)kxi * N + kyi) * a.ld_un + kz;
+      const size_t i_out = ((size_t)kxi * N + kyi) * a.ld_uout + kz;
+      const size_t cs_ld = (size_t)N * N * a.ld, cs_un = (size_t)N * N * a.ld_un, cs_out = (size_t)N * N * a.ld_uout;
+      double e_all = 0, e_low = 0;
+      T eh = 1, e1 = 1;
+      if (c.visc_if) {  // integrating factor exp(-nu k^2 h), exact for the linear part
+        eh = (T)exp(-(double)c.nu * k2 * 0.5 * (double)c.dt);
+        e1 = (T)exp(-(double)c.nu * k2 * (double)c.dt);
+      }
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        cx<T> out;
+        if (c.stage == 4) {
+          out = du[cc];
+        } else if (!c.visc_if) {
+          // spectral_solver.py:74-80
+          if (c.stage == 0) {
+            a.acc[cc * cs_ld + i_ld] = cadd(us3[cc], cscale(du[cc], c.ca));
+            out = cadd(us3[cc], cscale(du[cc], c.cb));
+          } else if (c.stage < 3) {
+            const cx<T> un = a.un[cc * cs_un + i_un];
+            a.acc[cc * cs_ld + i_ld] = cadd(a.acc[cc * cs_ld + i_ld], cscale(du[cc], c.ca));
+            out = cadd(un, cscale(du[cc], c.cb));
+          } else {
+            out = cadd(a.acc[cc * cs_ld + i_ld], cscale(du[cc], c.ca));
+          }
+        } else {
+          const T dt = c.dt;
+          if (c.stage == 0) {
+            a.acc[cc * cs_ld + i_ld] = cadd(cscale(us3[cc], e1), cscale(du[cc], dt / (T)6 * e1));
+            out = cscale(cadd(us3[cc], cscale(du[cc], dt / (T)2)), eh);
+          } else if (c.stage == 1) {
+            const cx<T> un = a.un[cc * cs_un + i_un];
+            a.acc[cc * cs_ld + i_ld] = cadd(a.acc[cc * cs_ld + i_ld], cscale(du[cc], dt / (T)3 * eh));
+            out = cadd(cscale(un, eh), cscale(du[cc], dt / (T)2));
+          } else if (c.stage == 2) {
+            const cx<T> un = a.un[cc * cs_un + i_un];
+            a.acc[cc * cs_ld + i_ld] = cadd(a.acc[cc * cs_ld + i_ld], cscale(du[cc], dt / (T)3 * eh));
+            out = cadd(cscale(un, e1), cscale(du[cc], dt * eh));
+          } else {
+            out = cadd(a.acc[cc * cs_ld + i_ld], cscale(du[cc], dt / (T)6));
+          }
+        }
+        a.uout[cc * cs_out + i_out] = out;
+        w[cc][s] = out;
+        const double m2 = (double)out.re * out.re + (double)out.im * out.im;
+        e_all += m2;
+        if (k2 > 0 && k2 <= c.kf2) e_low += m2;
+      }
+      const double wgt = (kz == 0 || kz == N / 2) ? 1.0 : 2.0;
+      epart[0] += wgt * e_all;
+      epart[1] += wgt * e_low;
+    }
+    if (c.partials != nullptr) {
+      block_sum<2>(epart, s_red);
+      if (tid == 0) {
+        const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+        c.partials[2 * bid] = epart[0];
+        c.partials[2 * bid + 1] = epart[1];
+      }
+    }
+  } else {
+    // ---- load the state (optionally rescaling the forced shell) ----
+    const double alpha = c.alpha ? *c.alpha : 1.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int kxi = IF::in_index(j, s);
+      const bool valid = ok && IF::in_valid(j, s);
+      const int kx = kfreq(kxi, N);
+      const int k2 = kx * kx + ky * ky + kzv * kzv;
+      const bool masked = c.alpha && k2 > 0 && k2 <= c.kf2;
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        cx<T> val = {0, 0};
+        if (valid) {
+          const size_t idx = ((size_t)(cc * N + kxi) * N + kyi) * a.ld_un + kz;
+          val = a.un[idx];
+          if (masked) {
+            val = cscale(val, (T)alpha);  // spectral_solver.py:112
+            a.rescale_dst[idx] = val;
+          }
+        }
+        w[cc][s] = val;
+      }
+    }
+  }
+
+  if (a.do_inverse) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      cx<T> v[IF::MAXE];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const int kx = kfreq(IF::in_index(j, s), N);
+        if (f < 3) {
+          v[s] = w[f][s];
+        } else if (f == 3) {  // omega_y^ = i (kz u^ - kx w^)
+          v[s] = cmuli(csub(cscale(w[0][s], (T)kzv), cscale(w[2][s], (T)kx)));
+        } else {  // omega_z^ = i (kx v^ - ky u^)
+          v[s] = cmuli(csub(cscale(w[1][s], (T)kx), cscale(w[0][s], (T)ky)));
+        }
+      }
+      IF::run(v, s_buf + p, C::LS, j, s_tw, 1);
+      if (ok) {
+#pragma unroll
+        for (int s = 0; s < IF::NOUT; ++s)
+          if (IF::out_valid(j, s))
+            a.x5[((size_t)(f * N + IF::out_index(j, s)) * N + kyi) * a.ld + kz] = v[s];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// inverse y-pass: X5 -> Y6 (or the first 3 fields only)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct NS3Y {
+  const cx<T>* in;  // [5][N][N][ld]
+  cx<T>* out;       // [6][N][N][ld]
+  int ld;
+  int nfields;      // 6 (full RHS) or 3 (velocity only)
+  const cx<T>* tw;
+};
+
+template <typename T, int N>
+__global__ void __launch_bounds__(XCfg<N>::THREADS)
+k_ns3_yinv(NS3Y<T> a) {
+  using C = XCfg<N>;
+  constexpr int TP = C::TP;
+  constexpr int Nh = N / 2 + 1;
+  using F = InvFFTp<T, N, TP>;
+  __shared__ cx<T> s_tw[N];
+  __shared__ cx<T> s_buf[N * C::LS];
+  const int tid = threadIdx.x, p = tid % TI, j = tid / TI;
+  const int kz = blockIdx.x * TI + p;
+  const bool ok = kz < Nh;
+  const int xi = blockIdx.y;
+  const int kzv = kz;
+  stage_table(s_tw, a.tw, N);
+  __syncthreads();
+  const size_t fs = (size_t)N * N * a.ld;
+  const size_t base = (size_t)xi * N * a.ld + kz;
+  auto load = [&](int f, cx<T>* v) {
+#pragma unroll
+    for (int s = 0; s < F::NIN; ++s)
+      v[s] = (ok && F::in_valid(j, s)) ? a.in[f * fs + base + (size_t)F::in_index(j, s) * a.ld] : cx<T>{0, 0};
+  };
+  auto store = [&](int f, const cx<T>* v) {
+    if (!ok) return;
+#pragma unroll
+    for (int s = 0; s < F::NOUT; ++s)
+      if (F::out_valid(j, s)) a.out[f * fs + base + (size_t)F::out_index(j, s) * a.ld] = v[s];
+  };
+  cx<T> v[F::MAXE];
+  if (a.nfields == 6) {
+    cx<T> A[F::NIN], B[F::NIN];
+    load(1, A);
+    load(2, B);
+#pragma unroll
+    for (int s = 0; s < F::NIN; ++s) {  // omega_x^ = i (ky w^ - kz v^)
+      const int ky = kfreq(F::in_index(j, s), N);
+      v[s] = cmuli(csub(cscale(B[s], (T)ky), cscale(A[s], (T)kzv)));
+    }
+    F::run(v, s_buf + p, C::LS, j, s_tw, 1);
+    store(3, v);
+#pragma unroll
+    for (int s = 0; s < F::NIN; ++s) v[s] = A[s];
+    F::run(v, s_buf + p, C::LS, j, s_tw, 1);
+    store(1, v);
+#pragma unroll
+    for (int s = 0; s < F::NIN; ++s) v[s] = B[s];
+    F::run(v, s_buf + p, C::LS, j, s_tw, 1);
+    store(2, v);
+    load(0, v);
+    F::run(v, s_buf + p, C::LS, j, s_tw, 1);
+    store(0, v);
+    load(3, v);
+    F::run(v, s_buf + p, C::LS, j, s_tw, 1);
+    store(4, v);
+    load(4, v);
+    F::run(v, s_buf + p, C::LS, j, s_tw, 1);
+    store(5, v);
+  } else {
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      load(f, v);
+      F::run(v, s_buf + p, C::LS, j, s_tw, 1);
+      store(f, v);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// z-pass: Y6 -> (6 x c2r) -> u x omega -> (3 x r2c, kz < kmax) -> NZ
+// ---------------------------------------------------------------------------
+template <typename T>
+struct NS3Z {
+  const cx<T>* in;   // [6][N][N][ld]
+  cx<T>* nz;         // [3][N][N][KZP]
+  T* u_out;          // optional physical velocity [3][N][N][N]
+  const T* u_in;     // optional physical velocity replacing the c2r of fields 0..2
+  int ld, KZD, KZP;
+  const cx<T>* tw;
+};
+
+template <int N>
+struct ZCfg {
+  static constexpr int M = N / 2;
+  static constexpr int TP = M / Plan<M>::R::at(0);  // lanes per pencil = last-pass butterflies of the c2r
+  static constexpr int P0 = 256 / TP;
+  static constexpr int P = (N <= 64) ? (P0 > 8 ? 8 : P0) : (P0 > 16 ? 16 : P0);
+  static constexpr int THREADS = P * TP;
+  static constexpr int LSR = M + 1 + LPAD;
+};
+
+template <typename T, int N>
+__global__ void __launch_bounds__(ZCfg<N>::THREADS)
+k_ns3_z(NS3Z<T> a) {
+  using C = ZCfg<N>;
+  constexpr int M = C::M, TP = C::TP, P = C::P;
+  constexpr int Nh = M + 1;
+  using CI = InvFFT<T, M, TP>;   // reversed plan: last pass has exactly TP butterflies
+  using CF = FwdFFT<T, M, TP>;   // plan order: pass 0 consumes CI's output order
+  static_assert(CI::NOUT == CF::NIN, "z-pass hand-off mismatch");
+  static_assert(CI::BL == TP, "z-pass lanes");
+  constexpr int NO = CI::NOUT;
+  __shared__ cx<T> s_tw[N];
+  __shared__ cx<T> s_buf[P * C::LSR];
+  const int tid = threadIdx.x, p = tid / TP, j = tid % TP;
+  const int row0 = blockIdx.x * P;
+  const int row = row0 + p;  // (x, y) row; N*N is a multiple of P
+  cx<T>* pb = s_buf + p * C::LSR;
+  stage_table(s_tw, a.tw, N);
+  const size_t fs = (size_t)N * N * a.ld;
+  const T scale = (T)1 / ((T)N * (T)N * (T)N);
+  cx<T> ph[6][NO];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    if (f < 3 && a.u_in != nullptr) {
+      const cx<T>* src = reinterpret_cast<const cx<T>*>(a.u_in + ((size_t)f * N * N + row) * N);
+#pragma unroll
+      for (int s = 0; s < NO; ++s) ph[f][s] = src[CI::out_index(j, s)];
+      continue;
+    }
+    __syncthreads();
+    for (int e = tid; e < P * Nh; e += C::THREADS) {
+      const int q = e / Nh, k = e - q * Nh;
+      s_buf[q * C::LSR + k] = a.in[f * fs + (size_t)(row0 + q) * a.ld + k];
+    }
+    __syncthreads();
+    cx<T> v[CI::MAXE];
+#pragma unroll
+    for (int s = 0; s < CI::NIN; ++s) {
+      cx<T> z = {0, 0};
+      if (CI::in_valid(j, s)) {
+        const int n = CI::in_index(j, s);
+        cx<T> xn = pb[n], xr = pb[M - n];
+        if (n == 0) {  // irfft ignores Im of the DC and Nyquist entries
+          xn.im = 0;
+          xr.im = 0;
+        }
+        xr = cconj(xr);
+        z = cadd(cadd(xn, xr), cmuli(cmulc(csub(xn, xr), s_tw[n])));
+      }
+      v[s] = z;
+    }
+    __syncthreads();
+    CI::run(v, pb, 1, j, s_tw, 2);
+#pragma unroll
+    for (int s = 0; s < NO; ++s) ph[f][s] = cscale(v[s], scale);
+  }
+  if (a.u_out != nullptr) {
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      cx<T>* dst = reinterpret_cast<cx<T>*>(a.u_out + ((size_t)f * N * N + row) * N);
+#pragma unroll
+      for (int s = 0; s < NO; ++s) dst[CI::out_index(j, s)] = ph[f][s];
+    }
+  }
+  // u x omega on (even, odd) z pairs held as (re, im)
+  cx<T> nl[3][NO];
+#pragma unroll
+  for (int s = 0; s < NO; ++s) {
+    const cx<T> u = ph[0][s], v = ph[1][s], w = ph[2][s];
+    const cx<T> ox = ph[3][s], oy = ph[4][s], oz = ph[5][s];
+    nl[0][s] = {v.re * oz.re - w.re * oy.re, v.im * oz.im - w.im * oy.im};
+    nl[1][s] = {w.re * ox.re - u.re * oz.re, w.im * ox.im - u.im * oz.im};
+    nl[2][s] = {u.re * oy.re - v.re * ox.re, u.im * oy.im - v.im * ox.im};
+  }
+#pragma unroll
+  for (int cc = 0; cc < 3; ++cc) {
+    cx<T> v[CF::MAXE];
+#pragma unroll
+    for (int s = 0; s < NO; ++s) v[s] = nl[cc][s];
+    __syncthreads();
+    CF::run(v, pb, 1, j, s_tw, 2);
+#pragma unroll
+    for (int s = 0; s < CF::NOUT; ++s)
+      if (CF::out_valid(j, s)) pb[CF::out_index(j, s)] = v[s];
+    __syncthreads();
+    cx<T>* dst = a.nz + ((size_t)cc * N * N + row) * a.KZP;
+    for (int k = j; k < a.KZD; k += TP) {
+      const cx<T> zk = pb[k];
+      const cx<T> zr = cconj(pb[k == 0 ? 0 : M - k]);
+      const cx<T> e = cscale(cadd(zk, zr), (T)0.5);
+      const cx<T> o = crot<T, true>(cscale(csub(zk, zr), (T)0.5));
+      dst[k] = cadd(e, cmul(s_tw[k], o));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// forward y-pass with |ky| < kmax truncation: NZ -> NY
+// ---------------------------------------------------------------------------
+template <typename T>
+struct NS3YF {
+  const cx<T>* nz;  // [3][N][N][KZP]
+  cx<T>* ny;        // [3][N][KYD][KZP]
+  int KZD, KZP, KYD;
+  const cx<T>* tw;
+};
+
+template <typename T, int N>
+__global__ void __launch_bounds__(XCfg<N>::THREADS)
+k_ns3_yfwd(NS3YF<T> a) {
+  using C = XCfg<N>;
+  constexpr int TP = C::TP;
+  using F = FwdFFT<T, N, TP>;
+  __shared__ cx<T> s_tw[N];
+  __shared__ cx<T> s_buf[N * C::LS];
+  const int tid = threadIdx.x, p = tid % TI, j = tid / TI;
+  const int kz = blockIdx.x * TI + p;
+  const bool ok = kz < a.KZD;
+  const int xi = blockIdx.y;
+  stage_table(s_tw, a.tw, N);
+  __syncthreads();
+#pragma unroll
+  for (int cc = 0; cc < 3; ++cc) {
+    cx<T> v[F::MAXE];
+    const size_t bi = ((size_t)cc * N + xi) * N * a.KZP + kz;
+#pragma unroll
+    for (int s = 0; s < F::NIN; ++s)
+      v[s] = (ok && F::in_valid(j, s)) ? a.nz[bi + (size_t)F::in_index(j, s) * a.KZP] : cx<T>{0, 0};
+    F::run(v, s_buf + p, C::LS, j, s_tw, 1);
+    if (ok) {
+      const size_t bo = ((size_t)cc * N + xi) * a.KYD * a.KZP + kz;
+#pragma unroll
+      for (int s = 0; s < F::NOUT; ++s) {
+        const int k = F::out_index(j, s);
+        if (F::out_valid(j, s) && keep_k(k, N, a.KZD)) {
+          const int kd = k < a.KZD ? k : k - (N - a.KYD);
+          a.ny[bo + (size_t)kd * a.KZP] = v[s];
+        }
+      }
+    }
+  }
+}
+
+}  // namespace qk

@@ -1,0 +1,12 @@
+"""l3es_amd: MI355X (gfx950) HIP implementation of the ql-les-data spectral step.
+
+Host-side mirror of the reference package ``l3es`` for the hot path only:
+spectral_solver (RHS, RK4, forcing, setup, driver), init_fields, integrator
+(particle interpolation) and the spectral utilities.  All arithmetic runs in
+``ql-les-data_amd/lib/libqlles.so`` through the C ABI in ``include/qlles.h``.
+"""
+
+from . import _lib  # noqa: F401
+from .spectral_solver import (comp_dt, custom_jit_integrate, rhs_wrapper, rk4_wrapper,  # noqa: F401
+                              set_up_solver, simulate)
+from .integrator import shift_fn, spectral_interpolator_wrapper  # noqa: F401

@@ -1,0 +1,75 @@
+"""Combined driver: DNS step + filtered-field particle advection (mirror of l3es/combined.py:18-219).
+
+Per step (combined.py:97,127,130,199): advance the spectral field, evaluate
+the ckp_N-filtered velocity at the particles, then r <- (r + dt u_r) mod 2 pi.
+The filter, interpolation and shift are one fused GPU call.  SPH relaxation,
+density and plotting are not part of this library.
+"""
+
+import csv
+import os
+from time import time
+
+import numpy as np
+import torch
+
+from .integrator import set_up_integrator
+from .spectral_solver import diagnostics, rk4_wrapper, rhs_wrapper, set_up_solver
+
+
+def combined(case, N=256, dim=3, nu=0.000625, t_final=0.1, t_burnin=0, dt=0.01, u_ref=4.0, ckp_N=64,
+             state_0_path=None, interp_backend="dft", relax_dt_factor=0.0, dft_splits=8, seed=42, rejit=True,
+             forcing_type="none", e_kin_target=None, kf=None, log_freq=100, vis_freq=100, ckp_freq=100,
+             dst_path=None, precision="float64", viscous_mode="explicit", spectrum="k53", power=0.1, drag=0.0,
+             kolm_amp=1.0, kolm_k=4, tol=1e-6, jitter=0.0, verbose=True, write_frames=True):
+    del vis_freq
+    if relax_dt_factor > 0.0:
+        raise NotImplementedError("SPH relaxation (relax.py) is outside this library")
+    int_path = os.path.join(dst_path, "com") if dst_path else None
+    if int_path:
+        os.makedirs(int_path, exist_ok=True)
+    u, u_hat, _, dx, integrate_fn, L, fft_axes, e_kin_init, e_inj = set_up_solver(
+        N, nu, dim, case, dt, seed, ckp_N, kf, e_kin_target, forcing_type, rejit, precision, viscous_mode,
+        spectrum, power, drag, kolm_amp, kolm_k)
+    r, _, interpolator, _, _, _, _ = set_up_integrator(state_0_path, dim, ckp_N, dft_splits, u_ref,
+                                                        interp_backend, precision, tol, jitter, seed)
+    diag_path = os.path.join(dst_path, "diagnostics.csv") if dst_path else None
+    if diag_path:
+        with open(diag_path, "w", newline="") as fh:
+            csv.writer(fh).writerow(["step", "time", "umax", "ekin", "dt_est", "e_inj", "rho_max"])
+    tstep_max = round(t_final / dt)
+    burnin_steps = round(t_burnin / dt) if t_burnin > 0 else 0
+    e_buf = torch.zeros(1, dtype=torch.float64, device=u.device)
+    u_r = torch.empty((r.shape[0], dim), dtype=interpolator.dtype, device=r.device)
+    t_sim, t_int, t0 = 0.0, 0.0, time()
+    e_acc = 0.0
+    rows = []
+    for i in range(tstep_max + 1):
+        if i == burnin_steps and case == "HIT":
+            integrate_fn = rk4_wrapper(dt, rhs_wrapper(N, nu, fft_axes, precision), None, e_kin_init, "none",
+                                       viscous_mode=viscous_mode)
+        t_temp = time()
+        integrate_fn.advance(u, u_hat, 1, u, e_buf)  # combined.py:97
+        torch.cuda.synchronize()
+        t_sim += time() - t_temp
+        e_acc += float(e_buf[0])
+        t_temp = time()
+        is_ckp = (i % ckp_freq == 0) and (i >= burnin_steps) and int_path is not None and write_frames
+        r0 = r.clone() if is_ckp else None
+        interpolator.from_hat(u_hat, r, dt, u_r)  # combined.py:127,130,199
+        torch.cuda.synchronize()
+        t_int += time() - t_temp
+        if i % log_freq == 0:
+            d = diagnostics(u, integrate_fn.plan).cpu().numpy()
+            dt_est = float(1.0 / (d[2:5] / dx + nu / dx ** 2).min())
+            rows.append([i, float(i * dt), float(d[1]), float(d[0]), dt_est, float(e_buf[0]), None])
+            if diag_path:
+                with open(diag_path, "a", newline="") as fh:
+                    csv.writer(fh).writerow(rows[-1])
+            if verbose:
+                print(f"Step {i}/{tstep_max}, u_max = {d[1]:.3f}, E_kin = {d[0]:.3f}, dt_est = {dt_est:.5f}")
+        if is_ckp:
+            np.savez(os.path.join(int_path, f"step_{i:05d}.npz"), r=r0.cpu().numpy(), u=u_r.cpu().numpy())
+    if verbose:
+        print(f"t_tot = {time() - t0:.3f}, t_sim = {t_sim}, t_int = {t_int:.3f}")
+    return dict(u=u, u_hat=u_hat, r=r, u_r=u_r, rows=rows, t_sim=t_sim, t_int=t_int, e_inj_acc=e_acc)

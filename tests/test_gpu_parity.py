@@ -1,0 +1,322 @@
+"""HIP path vs the FP64 oracle on the same seeded inputs (GPU).
+
+Tolerances: FP64 runs must match the oracle to round-off (relative 1e-12 on
+fields after a few steps); FP32 runs to 2e-5 relative on fields (FP32 FFT
+round-off ~ eps*log2(N)) over the short horizons used here.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+F64_TOL = 1e-12
+F32_TOL = 2e-5
+
+
+def dev(a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    return t.to(dtype) if dtype is not None else t
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300))
+
+
+def tol(prec):
+    return F64_TOL if prec == "float64" else F32_TOL
+
+
+def tdt(prec):
+    return torch.float64 if prec == "float64" else torch.float32
+
+
+def ctdt(prec):
+    return torch.complex128 if prec == "float64" else torch.complex64
+
+
+# ---------------------------------------------------------------------------
+# transforms
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("prec", ["float64", "float32"])
+@pytest.mark.parametrize("dim,N", [(3, 8), (3, 16), (3, 32), (3, 64), (3, 96), (3, 128),
+                                   (2, 8), (2, 32), (2, 96), (2, 256), (2, 512)])
+def test_rfftn_irfftn(gpu, dim, N, prec):
+    from l3es_amd import _lib
+
+    lib = _lib.load()
+    rng = np.random.default_rng(N + dim)
+    shape = (3, N, N, N) if dim == 3 else (3, N, N, 1)
+    u = rng.standard_normal(shape)
+    if dim == 2:
+        u[2] = 0
+    axes = (1, 2, 3) if dim == 3 else (1, 2)
+    ref = np.squeeze(np.fft.rfftn(u, axes=axes), axis=-1) if dim == 2 else np.fft.rfftn(u, axes=axes)
+    plan = _lib.Plan(dim, N, tdt(prec), 0.0, 1.0)
+    ud = dev(u, tdt(prec))
+    uh = torch.empty(ref.shape, dtype=ctdt(prec), device="cuda")
+    _lib.check(lib.qlles_rfftn(plan.handle, _lib.ptr(ud), _lib.ptr(uh), _lib.stream_ptr()))
+    assert rel(host(uh), ref) < 50 * tol(prec)
+    # inverse of a non-Hermitian spectrum follows numpy's irfftn (Im of DC/Nyquist ignored)
+    spec = ref + 0.3 * (rng.standard_normal(ref.shape) + 1j * rng.standard_normal(ref.shape)) * np.abs(ref).mean()
+    if dim == 2:
+        spec[2] = 0
+    ref_u = np.fft.irfftn(spec, axes=axes) if dim == 3 else np.fft.irfftn(spec, axes=axes)[..., None]
+    out = torch.empty(shape, dtype=tdt(prec), device="cuda")
+    _lib.check(lib.qlles_irfftn(plan.handle, _lib.ptr(dev(spec, ctdt(prec))), _lib.ptr(out), _lib.stream_ptr()))
+    assert rel(host(out), ref_u) < 50 * tol(prec)
+
+
+# ---------------------------------------------------------------------------
+# RHS and RK4 step (spectral_solver.py:27-120)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("prec", ["float64", "float32"])
+@pytest.mark.parametrize("N", [16, 32, 96])
+@pytest.mark.parametrize("rk", [0, 2])
+def test_rhs_3d(gpu, N, rk, prec):
+    from l3es_amd import rhs_wrapper
+
+    nu = 0.01
+    u = O.init_u_hit(N, seed=11)
+    uh = np.fft.rfftn(u, axes=(1, 2, 3))
+    ref = O.rhs_wrapper(N, nu)(u, uh, rk)
+    du = rhs_wrapper(N, nu, precision=prec)(dev(u, tdt(prec)), dev(uh, ctdt(prec)), rk)
+    assert rel(host(du), ref) < 5 * tol(prec)
+
+
+@pytest.mark.parametrize("prec", ["float64", "float32"])
+@pytest.mark.parametrize("N", [16, 64])
+def test_rhs_2d(gpu, N, prec):
+    from l3es_amd import rhs_wrapper
+
+    nu = 1e-3
+    u = O.init_u_kolm_np(N, seed=2)
+    uh = np.squeeze(np.fft.rfftn(u, axes=(1, 2)), -1)
+    for rk in (0, 1):
+        ref = O.rhs_wrapper(N, nu, (1, 2))(u, uh, rk)
+        du = rhs_wrapper(N, nu, (1, 2), precision=prec)(dev(u, tdt(prec)), dev(uh, ctdt(prec)), rk)
+        assert rel(host(du), ref) < 5 * tol(prec)
+
+
+def _run_gpu(N, nu, dim, case, dt, nsteps, prec, **kw):
+    from l3es_amd import set_up_solver
+
+    u, uh, _, _, step, _, _, e0, _ = set_up_solver(N, nu, dim, case, dt, 42, N, kw.pop("kf", None),
+                                                   precision=prec, warmup=False, **kw)
+    e_inj = []
+    for _ in range(nsteps):
+        u, uh, e = step(u, uh)
+        e_inj.append(float(e))
+    return host(u), host(uh), e0, e_inj
+
+
+def _run_oracle(N, nu, dim, case, dt, nsteps, **kw):
+    u, uh, step, e0, _ = O.set_up_solver(N, nu, dim, case, dt, seed=42, warmup=False, **kw)
+    e_inj = []
+    for _ in range(nsteps):
+        u, uh, e = step(u, uh)
+        e_inj.append(float(e))
+    return u, uh, e0, e_inj
+
+
+@pytest.mark.parametrize("prec", ["float64", "float32"])
+@pytest.mark.parametrize("N", [16, 32])
+def test_step_hit_unforced(gpu, N, prec):
+    kw = dict(forcing_type="none")
+    ug, uhg, _, _ = _run_gpu(N, 0.01, 3, "HIT", 0.005, 4, prec, **kw)
+    uo, uho, _, _ = _run_oracle(N, 0.01, 3, "HIT", 0.005, 4, **kw)
+    assert rel(ug, uo) < 5 * tol(prec)
+    assert rel(uhg, uho) < 5 * tol(prec)
+
+
+def test_tgv_known_answer_gpu(gpu):
+    """KAT (from_spectralDNS/main_numpy.py:136) on the HIP path in FP64."""
+    ug, _, _, _ = _run_gpu(64, 0.000625, 3, "TGV3D", 0.01, 10, "float64", forcing_type="none")
+    e = 0.5 * np.mean(np.sum(ug * ug, axis=0))
+    assert abs(e - 0.124953117517) < 5e-8
+    uo, _, _, _ = _run_oracle(64, 0.000625, 3, "TGV3D", 0.01, 10)
+    assert rel(ug, uo) < 1e-12
+
+
+def test_tgv_fp32_energy(gpu):
+    ug, _, _, _ = _run_gpu(64, 0.000625, 3, "TGV3D", 0.01, 10, "float32", forcing_type="none")
+    e = 0.5 * np.mean(np.sum(ug.astype(np.float64) ** 2, axis=0))
+    assert abs(e - 0.124953117517) / 0.125 < 1e-6
+
+
+@pytest.mark.parametrize("prec", ["float64", "float32"])
+def test_step_hit_ekin_tot(gpu, prec):
+    kw = dict(forcing_type="ekin_tot", kf=3, e_kin_target=1.25)
+    ug, uhg, e0g, eig = _run_gpu(32, 0.002, 3, "HIT", 0.002, 3, prec, **kw)
+    uo, uho, e0o, eio = _run_oracle(32, 0.002, 3, "HIT", 0.002, 3, **kw)
+    assert abs(e0g - e0o) < 1e-12 if prec == "float64" else abs(e0g - e0o) < 1e-5
+    assert rel(ug, uo) < 5 * tol(prec)
+    np.testing.assert_allclose(eig, eio, rtol=1e-9 if prec == "float64" else 2e-3, atol=1e-9)
+    e = 0.5 * np.mean(np.sum(ug.astype(np.float64) ** 2, axis=0))
+    assert abs(e - e0o) < (1e-12 if prec == "float64" else 1e-5)
+
+
+@pytest.mark.parametrize("prec", ["float64", "float32"])
+def test_step_hit_power_forcing(gpu, prec):
+    kw = dict(forcing_type="power", kf=2, power=0.1, spectrum="k4exp", e_kin_target=0.5)
+    ug, uhg, _, _ = _run_gpu(32, 1e-3, 3, "HIT", 5e-4, 3, prec, **kw)
+    uo, uho, _, _ = _run_oracle(32, 1e-3, 3, "HIT", 5e-4, 3, **kw)
+    assert rel(ug, uo) < 5 * tol(prec)
+
+
+@pytest.mark.parametrize("prec", ["float64", "float32"])
+def test_step_if_viscous(gpu, prec):
+    kw = dict(forcing_type="none", viscous_mode="IF")
+    ug, _, _, _ = _run_gpu(16, 0.02, 3, "HIT", 0.005, 3, prec, **kw)
+    uo, _, _, _ = _run_oracle(16, 0.02, 3, "HIT", 0.005, 3, **kw)
+    assert rel(ug, uo) < 5 * tol(prec)
+
+
+@pytest.mark.parametrize("prec", ["float64", "float32"])
+def test_step_2d_tgv(gpu, prec):
+    ug, _, _, _ = _run_gpu(64, 0.06283, 2, "TGV2D", 0.001, 5, prec, forcing_type="none")
+    uo, _, _, _ = _run_oracle(64, 0.06283, 2, "TGV2D", 0.001, 5)
+    assert ug.shape == uo.shape == (3, 64, 64, 1)
+    assert rel(ug, uo) < 5 * tol(prec)
+
+
+@pytest.mark.parametrize("prec", ["float64", "float32"])
+@pytest.mark.parametrize("N", [64, 256])
+def test_step_2d_kolmogorov(gpu, N, prec):
+    kw = dict(forcing_type="kolmogorov", drag=0.1, kolm_amp=1.0, kolm_k=4)
+    ug, _, _, _ = _run_gpu(N, 1e-3, 2, "Kolm", 1e-3, 5, prec, **kw)
+    uo, _, _, _ = _run_oracle(N, 1e-3, 2, "Kolm", 1e-3, 5, **kw)
+    assert rel(ug, uo) < 5 * tol(prec)
+
+
+def test_step_multi_equals_repeated(gpu):
+    """qlles_step(nsteps=K) == K calls of nsteps=1 (in-place chaining), bit for bit."""
+    from l3es_amd import set_up_solver
+
+    u, uh, _, _, step, _, _, _, _ = set_up_solver(32, 0.01, 3, "HIT", 0.005, 42, 32, None, forcing_type="none",
+                                                  precision="float32", warmup=False)
+    u1, uh1 = u.clone(), uh.clone()
+    for _ in range(3):
+        step.advance(u1, uh1, 1, u1)
+    u2, uh2 = u.clone(), uh.clone()
+    step.advance(u2, uh2, 3, u2, use_u=False)
+    u3, uh3 = u.clone(), uh.clone()
+    for _ in range(3):
+        step.advance(None, uh3, 1, u3, use_u=False)
+    assert np.array_equal(host(uh2), host(uh3))
+    assert rel(host(uh1), host(uh2)) < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# init, filter, diagnostics
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("N", [16, 32])
+def test_rogallo_init_gpu(gpu, N):
+    from l3es_amd.init_fields import init_u_hit
+
+    for spectrum in ("k53", "k4exp"):
+        assert rel(host(init_u_hit(N, 7, spectrum=spectrum)), O.init_u_hit(N, 7, spectrum=spectrum)) < 1e-13
+
+
+@pytest.mark.parametrize("prec", ["float64", "float32"])
+@pytest.mark.parametrize("dim,N,n", [(3, 32, 16), (3, 64, 32), (3, 32, 32), (2, 64, 32), (2, 256, 64)])
+def test_filter(gpu, dim, N, n, prec):
+    from l3es_amd import _lib
+    from l3es_amd.utils import spectral_filtering_hat
+
+    if dim == 3:
+        u = O.init_u_hit(N, seed=4)
+        uh = np.fft.rfftn(u, axes=(1, 2, 3))
+        ref = O.spectral_filtering(u, n) if n < N else u
+    else:
+        u = O.init_u_kolm_np(N, seed=4)
+        uh = np.squeeze(np.fft.rfftn(u, axes=(1, 2)), -1)
+        ref = O.spectral_filtering(u[:2, :, :, 0], n) if n < N else u[:2, :, :, 0]
+    plan = _lib.Plan(dim, N, tdt(prec), 0.0, 1.0)
+    out = spectral_filtering_hat(dev(uh, ctdt(prec)), n, plan)
+    assert rel(host(out), ref) < 20 * tol(prec)
+
+
+@pytest.mark.parametrize("prec", ["float64", "float32"])
+def test_diagnostics_and_dissipation(gpu, prec):
+    from l3es_amd import _lib
+    from l3es_amd.spectral_solver import comp_dt, diagnostics
+    from l3es_amd.utils import comp_dissipation_rate, kinetic_energy_hat
+
+    N, nu = 32, 0.01
+    u = O.init_u_hit(N, seed=8)
+    uh = np.fft.rfftn(u, axes=(1, 2, 3))
+    plan = _lib.Plan(3, N, tdt(prec), nu, 1.0)
+    d = host(diagnostics(dev(u, tdt(prec)), plan))
+    t = 1e-13 if prec == "float64" else 1e-6
+    assert abs(d[0] - O.kinetic_energy(u)) < t * O.kinetic_energy(u)
+    assert abs(d[1] - np.abs(u).max()) < t * np.abs(u).max()
+    dx = 2 * np.pi / N
+    assert abs(comp_dt(dev(u, tdt(prec)), dx, nu) - O.comp_dt(u, dx, nu)) < 1e-5 * O.comp_dt(u, dx, nu)
+    eps = comp_dissipation_rate(dev(uh, ctdt(prec)), nu)
+    assert abs(eps - O.comp_dissipation_rate(uh, nu)) < 100 * tol(prec) * O.comp_dissipation_rate(uh, nu)
+    e, e_low = kinetic_energy_hat(dev(uh, ctdt(prec)), kf=3)
+    assert abs(e - O.kinetic_energy_hat(uh, N, 3)) < 10 * tol(prec) * e
+    assert abs(e_low - O.kinetic_energy_hat(uh, N, 3, O.init_forcing_mask_hit(N, 3))) < 10 * tol(prec) * e
+
+
+# ---------------------------------------------------------------------------
+# particle interpolation (integrator.py:26-93) and advection (combined.py:199)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("prec", ["float64", "float32"])
+@pytest.mark.parametrize("dim,n", [(3, 16), (3, 32), (2, 64)])
+def test_interp_dft_exact(gpu, dim, n, prec):
+    from l3es_amd.integrator import spectral_interpolator_wrapper
+
+    rng = np.random.default_rng(n)
+    u = rng.standard_normal((dim,) + (n,) * dim)
+    r = rng.random((333, dim)) * 2 * np.pi
+    ref = O.interpolate_dft(u, r)
+    it = spectral_interpolator_wrapper(n, tuple(range(1, dim + 1)), backend="dft", precision=prec)
+    out = host(it(dev(u, tdt(prec)), dev(r)))
+    assert rel(out, ref) < (1e-12 if prec == "float64" else 5e-6)
+
+
+@pytest.mark.parametrize("prec", ["float64", "float32"])
+@pytest.mark.parametrize("dim,n", [(3, 16), (3, 32), (2, 64)])
+def test_interp_nufft_tolerance(gpu, dim, n, prec):
+    """NUFFT vs the exact DFT: relative l2 error <= tol = 1e-6 (FP64); FP32 to its floor."""
+    from l3es_amd.integrator import spectral_interpolator_wrapper
+
+    rng = np.random.default_rng(n + 1)
+    u = rng.standard_normal((dim,) + (n,) * dim)
+    r = rng.random((500, dim)) * 2 * np.pi
+    ref = O.interpolate_dft(u, r)
+    it = spectral_interpolator_wrapper(n, tuple(range(1, dim + 1)), backend="nufft", precision=prec, tol=1e-6)
+    out = host(it(dev(u, tdt(prec)), dev(r)))
+    err = np.linalg.norm(out - ref) / np.linalg.norm(ref)
+    assert err < (1e-6 if prec == "float64" else 2e-5)
+
+
+@pytest.mark.parametrize("backend", ["nufft", "dft"])
+def test_interp_hat_fused_and_advect(gpu, backend):
+    """filter(irfftn(u_hat), n) -> interpolate -> shift, fused, vs the oracle chain."""
+    from l3es_amd.integrator import spectral_interpolator_wrapper
+
+    N, n, dt = 32, 16, 0.01
+    u = O.init_u_hit(N, seed=5)
+    uh = np.fft.rfftn(u, axes=(1, 2, 3))
+    rng = np.random.default_rng(3)
+    r = rng.random((400, 3)) * 2 * np.pi
+    ref = O.interpolate_dft(O.spectral_filtering(u, n), r)
+    it = spectral_interpolator_wrapper(n, (1, 2, 3), backend=backend, precision="float64")
+    rd = dev(r)
+    ur = it.from_hat(dev(uh), rd, dt)
+    t = 1e-12 if backend == "dft" else 1e-6
+    assert np.linalg.norm(host(ur) - ref) / np.linalg.norm(ref) < t
+    r_new = host(rd)
+    np.testing.assert_allclose(r_new, O.shift_fn(r, dt * host(ur)), atol=1e-12)
+    assert (r_new >= 0).all() and (r_new < 2 * np.pi).all()
