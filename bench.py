@@ -79,7 +79,7 @@ def run_gpu(args, precision, steps, warmup, profile):
     u_fin = torch.empty((3, N, N, N), dtype=step.dtype, device=uh.device)
     _lib.check(lib.qlles_irfftn(step.plan.handle, _lib.ptr(uh), _lib.ptr(u_fin), _lib.stream_ptr()))
     e_fin = float(diagnostics(u_fin, step.plan)[0])
-    finite = bool(torch.isfinite(ur).all()) and np.isfinite(e_fin)
+    finite = bool(torch.isfinite(ur).all()) and bool(np.isfinite(e_fin))
     log(f"[bench] {precision}: {steps} steps in {t1 - t0:.3f}s -> {sps:.1f} steps/s, E={e_fin:.9f}")
     del uh, u_fin, step, it
     torch.cuda.empty_cache()
