@@ -61,10 +61,9 @@ template <typename T, int N>
 __global__ void __launch_bounds__(XCfg<N>::THREADS)
 k_ns2_x(NS2X<T> a) {
   using C = XCfg<N>;
-  constexpr int TP = C::TP;
   constexpr int Nh = N / 2 + 1;
-  using FF = FwdFFT<T, N, TP>;
-  using IF = InvFFT<T, N, TP>;
+  using FF = XFwd<T, N>;
+  using IF = XInv<T, N>;
   static_assert(FF::NOUT == IF::NIN, "fft hand-off mismatch");
   constexpr int NS = FF::NOUT;
   __shared__ cx<T> s_tw[N];
@@ -80,7 +79,6 @@ k_ns2_x(NS2X<T> a) {
   cx<T> w[2][NS] = {};
   if (c.stage >= 0) {
     const bool fwd_ok = ok && ky < a.KZD;
-    cx<T> nh[2][NS];
 #pragma unroll
     for (int cc = 0; cc < 2; ++cc) {
       cx<T> v[FF::MAXE];
@@ -90,7 +88,7 @@ k_ns2_x(NS2X<T> a) {
                                               : cx<T>{0, 0};
       FF::run(v, s_buf + p, C::LS, j, s_tw, 1);
 #pragma unroll
-      for (int s = 0; s < NS; ++s) nh[cc][s] = v[s];
+      for (int s = 0; s < NS; ++s) w[cc][s] = v[s];
     }
     T fscale = 0;
     if (c.force_power && blockIdx.x == 0) {
@@ -103,8 +101,8 @@ k_ns2_x(NS2X<T> a) {
       const int kx = kfreq(kxi, N);
       const bool valid = ok && FF::out_valid(j, s);
       const bool keep_x = keep_k(kxi, N, a.KZD);
-      const cx<T> n0 = keep_x ? nh[0][s] : cx<T>{0, 0};
-      const cx<T> n1 = keep_x ? nh[1][s] : cx<T>{0, 0};
+      const cx<T> n0 = keep_x ? w[0][s] : cx<T>{0, 0};
+      const cx<T> n1 = keep_x ? w[1][s] : cx<T>{0, 0};
       const int k2 = kx * kx + ky * ky;
       const T inv = k2 == 0 ? (T)0 : (T)1 / (T)k2;
       const cx<T> pr = {(n0.re * (T)kx + n1.re * (T)ky) * inv, (n0.im * (T)kx + n1.im * (T)ky) * inv};
@@ -213,8 +211,8 @@ k_ns2_y(NS2Y<T> a) {
   using C = ZCfg<N>;
   constexpr int M = C::M, TP = C::TP, P = C::P;
   constexpr int Nh = M + 1;
-  using CI = InvFFT<T, M, TP>;
-  using CF = FwdFFT<T, M, TP>;
+  using CI = PFFT<T, false, M, TP, Reverse<typename C::R>>;
+  using CF = PFFT<T, true, M, TP, typename C::R>;
   static_assert(CI::NOUT == CF::NIN, "y-pass hand-off mismatch");
   constexpr int NO = CI::NOUT;
   __shared__ cx<T> s_tw[N];

@@ -24,6 +24,49 @@
 #pragma once
 #include "fft_kernels.hpp"
 
+// Occupancy targets (waves per SIMD) for the fused kernels; 0 leaves the
+// choice to the compiler.  These kernels are HBM-bound, so more resident
+// waves hide more memory latency than extra registers buy.
+#ifndef QL_XPLAN_WIDE
+#define QL_XPLAN_WIDE 1
+#endif
+#ifndef QL_ZPLAN_WIDE
+#define QL_ZPLAN_WIDE 1
+#endif
+#ifndef QL_WPE_X
+#define QL_WPE_X 0
+#endif
+#ifndef QL_WPE_YI
+#define QL_WPE_YI 0
+#endif
+#ifndef QL_WPE_Z
+#define QL_WPE_Z 0
+#endif
+#ifndef QL_WPE_YF
+#define QL_WPE_YF 0
+#endif
+#define QL_WPE_ATTR_(n) __attribute__((amdgpu_waves_per_eu(n)))
+#if QL_WPE_X > 0
+#define QL_ATTR_X QL_WPE_ATTR_(QL_WPE_X)
+#else
+#define QL_ATTR_X
+#endif
+#if QL_WPE_YI > 0
+#define QL_ATTR_YI QL_WPE_ATTR_(QL_WPE_YI)
+#else
+#define QL_ATTR_YI
+#endif
+#if QL_WPE_Z > 0
+#define QL_ATTR_Z QL_WPE_ATTR_(QL_WPE_Z)
+#else
+#define QL_ATTR_Z
+#endif
+#if QL_WPE_YF > 0
+#define QL_ATTR_YF QL_WPE_ATTR_(QL_WPE_YF)
+#else
+#define QL_ATTR_YF
+#endif
+
 namespace qk {
 
 // RK / forcing controls shared by the 3-D and 2-D x-pass kernels
@@ -63,12 +106,24 @@ struct NS3X {
   StageCtl<T> c;
 };
 
+// Radix plan of the strided x / y passes.  With QL_XPLAN_WIDE each pencil is
+// spread over more lanes (fewer elements, hence fewer registers, per thread)
+// at the price of one more LDS round trip per transform.
+template <int N> struct XPlan { using R = typename Plan<N>::R; static constexpr int TP = Plan<N>::TP; };
+#if QL_XPLAN_WIDE
+template <> struct XPlan<256> { using R = Radices<8, 4, 8>; static constexpr int TP = 32; };
+template <> struct XPlan<128> { using R = Radices<4, 4, 8>; static constexpr int TP = 16; };
+#endif
+
 template <int N>
 struct XCfg {
-  static constexpr int TP = Plan<N>::TP;
+  using R = typename XPlan<N>::R;
+  static constexpr int TP = XPlan<N>::TP;
   static constexpr int THREADS = TI * TP;
   static constexpr int LS = TI + LPAD;
 };
+template <typename T, int N> using XFwd = PFFT<T, true, N, XCfg<N>::TP, typename XCfg<N>::R>;
+template <typename T, int N> using XInv = PFFT<T, false, N, XCfg<N>::TP, Reverse<typename XCfg<N>::R>>;
 
 __device__ __forceinline__ bool keep_k(int idx, int N, int KZD) { return idx < KZD || idx > N - KZD; }
 
@@ -121,13 +176,12 @@ __device__ double forced_shell_energy3(const cx<T>* us, int ld, int kf2, double*
 }
 
 template <typename T, int N>
-__global__ void __launch_bounds__(XCfg<N>::THREADS)
+__global__ void __launch_bounds__(XCfg<N>::THREADS) QL_ATTR_X
 k_ns3_x(NS3X<T> a) {
   using C = XCfg<N>;
-  constexpr int TP = C::TP;
   constexpr int Nh = N / 2 + 1;
-  using FF = FwdFFT<T, N, TP>;
-  using IF = InvFFT<T, N, TP>;
+  using FF = XFwd<T, N>;
+  using IF = XInv<T, N>;
   static_assert(FF::NOUT == IF::NIN, "fft hand-off mismatch");
   constexpr int NS = FF::NOUT;
   __shared__ cx<T> s_tw[N];
@@ -151,7 +205,8 @@ k_ns3_x(NS3X<T> a) {
     const bool row_kept = keep_k(kyi, N, KZD);
     const int kyd = kyi < KZD ? kyi : kyi - (N - KYD);
     const bool fwd_ok = ok && kz < KZD && row_kept;
-    cx<T> nh[3][NS];
+    // w holds N^ after the forward FFT and is overwritten slot by slot with
+    // the new stage field (one register array for both)
 #pragma unroll
     for (int cc = 0; cc < 3; ++cc) {
       cx<T> v[FF::MAXE];
@@ -163,7 +218,7 @@ k_ns3_x(NS3X<T> a) {
       }
       FF::run(v, s_buf + p, C::LS, j, s_tw, 1);
 #pragma unroll
-      for (int s = 0; s < NS; ++s) nh[cc][s] = v[s];
+      for (int s = 0; s < NS; ++s) w[cc][s] = v[s];
     }
 
     // ---- forcing normalisation (FP64) ----
@@ -185,9 +240,9 @@ k_ns3_x(NS3X<T> a) {
       const int kx = kfreq(kxi, N);
       const bool valid = ok && FF::out_valid(j, s);
       const bool keep_x = keep_k(kxi, N, a.KZD);
-      cx<T> n0 = keep_x ? nh[0][s] : cx<T>{0, 0};
-      cx<T> n1 = keep_x ? nh[1][s] : cx<T>{0, 0};
-      cx<T> n2 = keep_x ? nh[2][s] : cx<T>{0, 0};
+      cx<T> n0 = keep_x ? w[0][s] : cx<T>{0, 0};
+      cx<T> n1 = keep_x ? w[1][s] : cx<T>{0, 0};
+      cx<T> n2 = keep_x ? w[2][s] : cx<T>{0, 0};
       const int k2 = kx * kx + ky * ky + kzv * kzv;
       const T inv = k2 == 0 ? (T)0 : (T)1 / (T)k2;
       // p^ = sum_c N^_c k_c / k^2 (spectral_solver.py:57)
@@ -337,12 +392,11 @@ struct NS3Y {
 };
 
 template <typename T, int N>
-__global__ void __launch_bounds__(XCfg<N>::THREADS)
+__global__ void __launch_bounds__(XCfg<N>::THREADS) QL_ATTR_YI
 k_ns3_yinv(NS3Y<T> a) {
   using C = XCfg<N>;
-  constexpr int TP = C::TP;
   constexpr int Nh = N / 2 + 1;
-  using F = InvFFTp<T, N, TP>;
+  using F = XInv<T, N>;
   __shared__ cx<T> s_tw[N];
   __shared__ cx<T> s_buf[N * C::LS];
   const int tid = threadIdx.x, p = tid % TI, j = tid / TI;
@@ -417,27 +471,77 @@ struct NS3Z {
   const cx<T>* tw;
 };
 
+// Radix plan (forward order) of the half-length FFTs in the z-pass; the c2r
+// runs it reversed so its last pass has exactly TP butterflies.
+template <int M> struct ZPlan { using R = typename Plan<M>::R; };
+#if QL_ZPLAN_WIDE
+template <> struct ZPlan<128> { using R = Radices<4, 4, 8>; };
+template <> struct ZPlan<64> { using R = Radices<4, 4, 4>; };
+#endif
+
 template <int N>
 struct ZCfg {
   static constexpr int M = N / 2;
-  static constexpr int TP = M / Plan<M>::R::at(0);  // lanes per pencil = last-pass butterflies of the c2r
+  using R = typename ZPlan<M>::R;
+  static constexpr int TP = M / R::at(0);  // lanes per pencil = last-pass butterflies of the c2r
   static constexpr int P0 = 256 / TP;
   static constexpr int P = (N <= 64) ? (P0 > 8 ? 8 : P0) : (P0 > 16 ? 16 : P0);
   static constexpr int THREADS = P * TP;
   static constexpr int LSR = M + 1 + LPAD;
 };
 
+// c2r pre-processing of one pencil held in LDS (pb[0..M]) into the pass-0
+// input slots of the M-point inverse FFT: Z'[n] = (X_n + X*_{M-n}) +
+// i W_N^{-n} (X_n - X*_{M-n}); Im of X_0 and X_M is ignored (numpy's irfft).
+template <typename T, int M, class CI>
+__device__ __forceinline__ void c2r_prep(cx<T>* v, const cx<T>* pb, const cx<T>* s_tw, int j) {
+#pragma unroll
+  for (int s = 0; s < CI::NIN; ++s) {
+    cx<T> z = {0, 0};
+    if (CI::in_valid(j, s)) {
+      const int n = CI::in_index(j, s);
+      cx<T> xn = pb[n], xr = pb[M - n];
+      if (n == 0) {
+        xn.im = 0;
+        xr.im = 0;
+      }
+      xr = cconj(xr);
+      z = cadd(cadd(xn, xr), cmuli(cmulc(csub(xn, xr), s_tw[n])));
+    }
+    v[s] = z;
+  }
+}
+
+// r2c post-processing from the M-point forward FFT held in LDS (pb[0..M)):
+// X_k = (Z_k + Z*_{M-k})/2 + W_N^k (Z_k - Z*_{M-k})/(2i), k < KZD
+template <typename T, int M>
+__device__ __forceinline__ void r2c_post_store(cx<T>* dst, const cx<T>* pb, const cx<T>* s_tw, int j, int TP, int KZD) {
+  for (int k = j; k < KZD; k += TP) {
+    const cx<T> zk = pb[k];
+    const cx<T> zr = cconj(pb[k == 0 ? 0 : M - k]);
+    const cx<T> e = cscale(cadd(zk, zr), (T)0.5);
+    const cx<T> o = crot<T, true>(cscale(csub(zk, zr), (T)0.5));
+    dst[k] = cadd(e, cmul(s_tw[k], o));
+  }
+}
+
+// The six input fields are streamed: while field f is transformed, the rows
+// of field f+1 are already in flight into registers (software pipelining),
+// and the cross product is accumulated field by field so only u, v, w and
+// the three products stay live.
 template <typename T, int N>
-__global__ void __launch_bounds__(ZCfg<N>::THREADS)
+__global__ void __launch_bounds__(ZCfg<N>::THREADS) QL_ATTR_Z
 k_ns3_z(NS3Z<T> a) {
   using C = ZCfg<N>;
   constexpr int M = C::M, TP = C::TP, P = C::P;
   constexpr int Nh = M + 1;
-  using CI = InvFFT<T, M, TP>;   // reversed plan: last pass has exactly TP butterflies
-  using CF = FwdFFT<T, M, TP>;   // plan order: pass 0 consumes CI's output order
+  constexpr int NT = C::THREADS;
+  using CI = PFFT<T, false, M, TP, Reverse<typename C::R>>;  // last pass has exactly TP butterflies
+  using CF = PFFT<T, true, M, TP, typename C::R>;           // pass 0 consumes CI's output order
   static_assert(CI::NOUT == CF::NIN, "z-pass hand-off mismatch");
   static_assert(CI::BL == TP, "z-pass lanes");
   constexpr int NO = CI::NOUT;
+  constexpr int NE = (P * Nh + NT - 1) / NT;  // prefetch elements per thread per field
   __shared__ cx<T> s_tw[N];
   __shared__ cx<T> s_buf[P * C::LSR];
   const int tid = threadIdx.x, p = tid / TP, j = tid % TP;
@@ -447,59 +551,90 @@ k_ns3_z(NS3Z<T> a) {
   stage_table(s_tw, a.tw, N);
   const size_t fs = (size_t)N * N * a.ld;
   const T scale = (T)1 / ((T)N * (T)N * (T)N);
-  cx<T> ph[6][NO];
+  const int f0 = a.u_in != nullptr ? 3 : 0;  // first field taken through c2r
+
+  cx<T> pf[NE];
+  auto fetch = [&](int f) {
+    const cx<T>* src = a.in + f * fs + (size_t)row0 * a.ld;
 #pragma unroll
-  for (int f = 0; f < 6; ++f) {
-    if (f < 3 && a.u_in != nullptr) {
+    for (int e = 0; e < NE; ++e) {
+      const int idx = tid + e * NT;
+      if (idx < P * Nh) {
+        const int q = idx / Nh, k = idx - q * Nh;
+        pf[e] = src[(size_t)q * a.ld + k];
+      }
+    }
+  };
+  auto park = [&]() {
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      const int idx = tid + e * NT;
+      if (idx < P * Nh) {
+        const int q = idx / Nh, k = idx - q * Nh;
+        s_buf[q * C::LSR + k] = pf[e];
+      }
+    }
+  };
+
+  cx<T> uu[3][NO];
+  cx<T> nl[3][NO];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int s = 0; s < NO; ++s) nl[c][s] = {0, 0};
+  if (f0 == 3) {
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
       const cx<T>* src = reinterpret_cast<const cx<T>*>(a.u_in + ((size_t)f * N * N + row) * N);
 #pragma unroll
-      for (int s = 0; s < NO; ++s) ph[f][s] = src[CI::out_index(j, s)];
-      continue;
+      for (int s = 0; s < NO; ++s) uu[f][s] = src[CI::out_index(j, s)];
     }
-    __syncthreads();
-    for (int e = tid; e < P * Nh; e += C::THREADS) {
-      const int q = e / Nh, k = e - q * Nh;
-      s_buf[q * C::LSR + k] = a.in[f * fs + (size_t)(row0 + q) * a.ld + k];
-    }
+  }
+  fetch(f0);
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    if (f < f0) continue;
+    __syncthreads();  // previous field's FFT scratch is free
+    park();
+    if (f + 1 < 6) fetch(f + 1);  // in flight during this field's transform
     __syncthreads();
     cx<T> v[CI::MAXE];
-#pragma unroll
-    for (int s = 0; s < CI::NIN; ++s) {
-      cx<T> z = {0, 0};
-      if (CI::in_valid(j, s)) {
-        const int n = CI::in_index(j, s);
-        cx<T> xn = pb[n], xr = pb[M - n];
-        if (n == 0) {  // irfft ignores Im of the DC and Nyquist entries
-          xn.im = 0;
-          xr.im = 0;
-        }
-        xr = cconj(xr);
-        z = cadd(cadd(xn, xr), cmuli(cmulc(csub(xn, xr), s_tw[n])));
-      }
-      v[s] = z;
-    }
+    c2r_prep<T, M, CI>(v, pb, s_tw, j);
     __syncthreads();
     CI::run(v, pb, 1, j, s_tw, 2);
 #pragma unroll
-    for (int s = 0; s < NO; ++s) ph[f][s] = cscale(v[s], scale);
+    for (int s = 0; s < NO; ++s) v[s] = cscale(v[s], scale);
+    // (re, im) hold the (even, odd) z samples; u x omega accumulated per field
+    if (f < 3) {
+#pragma unroll
+      for (int s = 0; s < NO; ++s) uu[f][s] = v[s];
+    } else if (f == 3) {  // omega_x: n_y += w ox, n_z -= v ox
+#pragma unroll
+      for (int s = 0; s < NO; ++s) {
+        nl[1][s] = {nl[1][s].re + uu[2][s].re * v[s].re, nl[1][s].im + uu[2][s].im * v[s].im};
+        nl[2][s] = {nl[2][s].re - uu[1][s].re * v[s].re, nl[2][s].im - uu[1][s].im * v[s].im};
+      }
+    } else if (f == 4) {  // omega_y: n_x -= w oy, n_z += u oy
+#pragma unroll
+      for (int s = 0; s < NO; ++s) {
+        nl[0][s] = {nl[0][s].re - uu[2][s].re * v[s].re, nl[0][s].im - uu[2][s].im * v[s].im};
+        nl[2][s] = {nl[2][s].re + uu[0][s].re * v[s].re, nl[2][s].im + uu[0][s].im * v[s].im};
+      }
+    } else {  // omega_z: n_x += v oz, n_y -= u oz
+#pragma unroll
+      for (int s = 0; s < NO; ++s) {
+        nl[0][s] = {nl[0][s].re + uu[1][s].re * v[s].re, nl[0][s].im + uu[1][s].im * v[s].im};
+        nl[1][s] = {nl[1][s].re - uu[0][s].re * v[s].re, nl[1][s].im - uu[0][s].im * v[s].im};
+      }
+    }
   }
   if (a.u_out != nullptr) {
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
       cx<T>* dst = reinterpret_cast<cx<T>*>(a.u_out + ((size_t)f * N * N + row) * N);
 #pragma unroll
-      for (int s = 0; s < NO; ++s) dst[CI::out_index(j, s)] = ph[f][s];
+      for (int s = 0; s < NO; ++s) dst[CI::out_index(j, s)] = uu[f][s];
     }
-  }
-  // u x omega on (even, odd) z pairs held as (re, im)
-  cx<T> nl[3][NO];
-#pragma unroll
-  for (int s = 0; s < NO; ++s) {
-    const cx<T> u = ph[0][s], v = ph[1][s], w = ph[2][s];
-    const cx<T> ox = ph[3][s], oy = ph[4][s], oz = ph[5][s];
-    nl[0][s] = {v.re * oz.re - w.re * oy.re, v.im * oz.im - w.im * oy.im};
-    nl[1][s] = {w.re * ox.re - u.re * oz.re, w.im * ox.im - u.im * oz.im};
-    nl[2][s] = {u.re * oy.re - v.re * ox.re, u.im * oy.im - v.im * ox.im};
   }
 #pragma unroll
   for (int cc = 0; cc < 3; ++cc) {
@@ -512,14 +647,7 @@ k_ns3_z(NS3Z<T> a) {
     for (int s = 0; s < CF::NOUT; ++s)
       if (CF::out_valid(j, s)) pb[CF::out_index(j, s)] = v[s];
     __syncthreads();
-    cx<T>* dst = a.nz + ((size_t)cc * N * N + row) * a.KZP;
-    for (int k = j; k < a.KZD; k += TP) {
-      const cx<T> zk = pb[k];
-      const cx<T> zr = cconj(pb[k == 0 ? 0 : M - k]);
-      const cx<T> e = cscale(cadd(zk, zr), (T)0.5);
-      const cx<T> o = crot<T, true>(cscale(csub(zk, zr), (T)0.5));
-      dst[k] = cadd(e, cmul(s_tw[k], o));
-    }
+    r2c_post_store<T, M>(a.nz + ((size_t)cc * N * N + row) * a.KZP, pb, s_tw, j, TP, a.KZD);
   }
 }
 
@@ -535,11 +663,10 @@ struct NS3YF {
 };
 
 template <typename T, int N>
-__global__ void __launch_bounds__(XCfg<N>::THREADS)
+__global__ void __launch_bounds__(XCfg<N>::THREADS) QL_ATTR_YF
 k_ns3_yfwd(NS3YF<T> a) {
   using C = XCfg<N>;
-  constexpr int TP = C::TP;
-  using F = FwdFFT<T, N, TP>;
+  using F = XFwd<T, N>;
   __shared__ cx<T> s_tw[N];
   __shared__ cx<T> s_buf[N * C::LS];
   const int tid = threadIdx.x, p = tid % TI, j = tid / TI;
