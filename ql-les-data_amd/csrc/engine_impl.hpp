@@ -63,6 +63,11 @@ class EngineT final : public Engine {
 
   // ---------------- 3-D launches ----------------
   double C() const { return (double)rows_ * Nh * 2.0 * sizeof(T); }  // one spectral component
+  // x-pass / y-inverse tiles along kz: full tiles below N/2 plus one Nyquist-column tile
+  static int xtiles() {
+    constexpr int ntr = (N / 2) / XCfg<N>::TIX;
+    return ntr > 0 ? ntr + 1 : (Nh + XCfg<N>::TIX - 1) / XCfg<N>::TIX;
+  }
   double xpass_bytes(const StageCtl<T>& c, bool inv, int ncomp, int nx) const {
     const double cc = C();
     double b = 0;
@@ -82,7 +87,7 @@ class EngineT final : public Engine {
     return b;
   }
   void x3(NS3X<T>& a, hipStream_t s) {
-    dim3 g((Nh + TI - 1) / TI, N);
+    dim3 g(xtiles(), N);
     hipEvent_t e0 = nullptr;
     prof_begin(s, &e0);
     hipLaunchKernelGGL((k_ns3_x<T, N>), g, dim3(XCfg<N>::THREADS), 0, s, a);
@@ -91,7 +96,7 @@ class EngineT final : public Engine {
   }
   void yinv3(int nf, hipStream_t s) {
     NS3Y<T> a{X5, Y6, LD, nf, twN_};
-    dim3 g((Nh + TI - 1) / TI, N);
+    dim3 g(xtiles(), N);
     hipEvent_t e0 = nullptr;
     prof_begin(s, &e0);
     hipLaunchKernelGGL((k_ns3_yinv<T, N>), g, dim3(XCfg<N>::THREADS), 0, s, a);
@@ -110,7 +115,7 @@ class EngineT final : public Engine {
   }
   void yfwd3(hipStream_t s) {
     NS3YF<T> a{NZ, NY, KZD, KZP, KYD, twN_};
-    dim3 g((KZD + TI - 1) / TI, N);
+    dim3 g((KZD + XCfg<N>::TIX - 1) / XCfg<N>::TIX, N);
     hipEvent_t e0 = nullptr;
     prof_begin(s, &e0);
     hipLaunchKernelGGL((k_ns3_yfwd<T, N>), g, dim3(XCfg<N>::THREADS), 0, s, a);
@@ -136,7 +141,7 @@ class EngineT final : public Engine {
   void x2(NS2X<T>& a, hipStream_t s) {
     hipEvent_t e0 = nullptr;
     prof_begin(s, &e0);
-    hipLaunchKernelGGL((k_ns2_x<T, N>), dim3((Nh + TI - 1) / TI), dim3(XCfg<N>::THREADS), 0, s, a);
+    hipLaunchKernelGGL((k_ns2_x<T, N>), dim3((Nh + XCfg<N>::TIX - 1) / XCfg<N>::TIX), dim3(XCfg<N>::THREADS), 0, s, a);
     QL_LAUNCHED();
     prof_end(s, K_XPASS, xpass_bytes(a.c, a.do_inverse, 2, 3), e0);
   }
@@ -206,7 +211,7 @@ class EngineT final : public Engine {
       Y6 = alloc<cx<T>>(6 * fsz);
       NZ = alloc<cx<T>>((size_t)3 * N * N * KZP);
       NY = alloc<cx<T>>((size_t)3 * N * KYD * KZP);
-      nxblk_ = (Nh + TI - 1) / TI * N;
+      nxblk_ = xtiles() * N;
     } else {
       S1a = alloc<cx<T>>(2 * fsz);
       S1b = alloc<cx<T>>(2 * fsz);
@@ -214,7 +219,7 @@ class EngineT final : public Engine {
       X5 = alloc<cx<T>>(3 * fsz);
       Y6 = alloc<cx<T>>(3 * fsz);  // scratch for transforms / diagnostics
       NY = alloc<cx<T>>((size_t)2 * N * KZP);
-      nxblk_ = (Nh + TI - 1) / TI;
+      nxblk_ = (Nh + XCfg<N>::TIX - 1) / XCfg<N>::TIX;
     }
     partials_ = alloc<double>(std::max(RED_BLOCKS * 4, nxblk_ * 2));
     alpha_ = alloc<double>(8);

@@ -70,8 +70,8 @@ k_ns2_x(NS2X<T> a) {
   __shared__ cx<T> s_buf[N * C::LS];
   __shared__ double s_red[C::THREADS];
 
-  const int tid = threadIdx.x, p = tid % TI, j = tid / TI;
-  const int ky = blockIdx.x * TI + p;  // rfft column (Nyquist carries +N/2)
+  const int tid = threadIdx.x, p = tid % C::TIX, j = tid / C::TIX;
+  const int ky = blockIdx.x * C::TIX + p;  // rfft column (Nyquist carries +N/2)
   const bool ok = ky < Nh;
   stage_table(s_tw, a.tw, N);
   __syncthreads();
@@ -91,7 +91,8 @@ k_ns2_x(NS2X<T> a) {
       for (int s = 0; s < NS; ++s) w[cc][s] = v[s];
     }
     T fscale = 0;
-    if (c.force_power && blockIdx.x == 0) {
+    const int ky0 = blockIdx.x * C::TIX;
+    if (c.force_power && ky0 * ky0 <= c.kf2) {
       const double ef = forced_shell_energy2<T, N>(a.us, a.ld_us, c.kf2, s_red) / ((double)N * N * (double)N * N);
       fscale = (T)(c.power / (2.0 * ef + 2.220446049250313e-16));
     }

@@ -34,7 +34,7 @@
 #define QL_ZPLAN_WIDE 1
 #endif
 #ifndef QL_WPE_X
-#define QL_WPE_X 0
+#define QL_WPE_X 4
 #endif
 #ifndef QL_WPE_YI
 #define QL_WPE_YI 0
@@ -115,12 +115,17 @@ template <> struct XPlan<256> { using R = Radices<8, 4, 8>; static constexpr int
 template <> struct XPlan<128> { using R = Radices<4, 4, 8>; static constexpr int TP = 16; };
 #endif
 
+#ifndef QL_TIX
+#define QL_TIX 16
+#endif
+
 template <int N>
 struct XCfg {
   using R = typename XPlan<N>::R;
   static constexpr int TP = XPlan<N>::TP;
-  static constexpr int THREADS = TI * TP;
-  static constexpr int LS = TI + LPAD;
+  static constexpr int TIX = QL_TIX;  // columns (contiguous kz / ky) per workgroup tile
+  static constexpr int THREADS = TIX * TP;
+  static constexpr int LS = TIX + LPAD;
 };
 template <typename T, int N> using XFwd = PFFT<T, true, N, XCfg<N>::TP, typename XCfg<N>::R>;
 template <typename T, int N> using XInv = PFFT<T, false, N, XCfg<N>::TP, Reverse<typename XCfg<N>::R>>;
@@ -188,42 +193,67 @@ k_ns3_x(NS3X<T> a) {
   __shared__ cx<T> s_buf[N * C::LS];
   __shared__ double s_red[2 * C::THREADS];
 
-  const int tid = threadIdx.x, p = tid % TI, j = tid / TI;
-  const int kz = blockIdx.x * TI + p;
+  const int tid = threadIdx.x, p = tid % C::TIX, j = tid / C::TIX;
+  // Tiles 0..NTR-1 hold TIX consecutive kz < N/2 of one ky row; tile NTR holds
+  // the kz = N/2 column for TIX consecutive ky rows (so no workgroup runs its
+  // transforms for a single valid column).
+  constexpr int NTR = (N / 2) / C::TIX;
+  const bool nyq = NTR > 0 && (int)blockIdx.x == NTR;
+  const StageCtl<T>& c = a.c;
+  if (nyq && (int)blockIdx.y >= N / C::TIX) {
+    if (c.partials != nullptr && tid == 0) {
+      const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+      c.partials[2 * bid] = 0.0;
+      c.partials[2 * bid + 1] = 0.0;
+    }
+    return;  // whole workgroup leaves before any barrier
+  }
+  const int kz = nyq ? N / 2 : (int)blockIdx.x * C::TIX + p;
   const bool ok = kz < Nh;
-  const int kyi = blockIdx.y;
+  const int kyi = nyq ? (int)blockIdx.y * C::TIX + p : (int)blockIdx.y;
   const int ky = kfreq(kyi, N);
   const int kzv = kz;  // rfft axis: index N/2 carries +N/2 (spectral_solver.py:34-36)
   stage_table(s_tw, a.tw, N);
   __syncthreads();
-  const StageCtl<T>& c = a.c;
 
   cx<T> w[3][NS] = {};  // new stage field (input of the inverse x-FFT)
   if (c.stage >= 0) {
     // ---- forward x-FFT of the dealiased nonlinear term ----
     const int KZD = a.KZD, KYD = a.KYD;
-    const bool row_kept = keep_k(kyi, N, KZD);
-    const int kyd = kyi < KZD ? kyi : kyi - (N - KYD);
-    const bool fwd_ok = ok && kz < KZD && row_kept;
-    // w holds N^ after the forward FFT and is overwritten slot by slot with
-    // the new stage field (one register array for both)
+    // N^ is zero outside |k| < kmax: tiles without a kept mode skip the
+    // forward transform altogether (block-uniform)
+    const bool blk_fwd = !nyq && (int)blockIdx.x * C::TIX < KZD && keep_k(kyi, N, KZD);
+    if (blk_fwd) {
+      const int kyd = kyi < KZD ? kyi : kyi - (N - KYD);
+      const bool fwd_ok = ok && kz < KZD;
+      // all three components are requested before the first transform so their
+      // HBM latency overlaps; w holds N^ after the forward FFT and is
+      // overwritten slot by slot with the new stage field
+      cx<T> pre[3][FF::NIN];
 #pragma unroll
-    for (int cc = 0; cc < 3; ++cc) {
-      cx<T> v[FF::MAXE];
+      for (int cc = 0; cc < 3; ++cc) {
 #pragma unroll
-      for (int s = 0; s < FF::NIN; ++s) {
-        v[s] = (fwd_ok && FF::in_valid(j, s))
-                   ? a.ny[((size_t)(cc * N + FF::in_index(j, s)) * KYD + kyd) * a.KZP + kz]
-                   : cx<T>{0, 0};
+        for (int s = 0; s < FF::NIN; ++s) {
+          pre[cc][s] = (fwd_ok && FF::in_valid(j, s))
+                           ? a.ny[((size_t)(cc * N + FF::in_index(j, s)) * KYD + kyd) * a.KZP + kz]
+                           : cx<T>{0, 0};
+        }
       }
-      FF::run(v, s_buf + p, C::LS, j, s_tw, 1);
 #pragma unroll
-      for (int s = 0; s < NS; ++s) w[cc][s] = v[s];
+      for (int cc = 0; cc < 3; ++cc) {
+        cx<T> v[FF::MAXE];
+#pragma unroll
+        for (int s = 0; s < FF::NIN; ++s) v[s] = pre[cc][s];
+        FF::run(v, s_buf + p, C::LS, j, s_tw, 1);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) w[cc][s] = v[s];
+      }
     }
 
     // ---- forcing normalisation (FP64) ----
     T fscale = 0;
-    if (c.force_power && kz < TI && blockIdx.x == 0) {
+    const int kz0 = blockIdx.x * C::TIX;
+    if (c.force_power && !nyq && kz0 * kz0 <= c.kf2) {
       // only blocks that hold forced modes need E_f (uniform branch per block)
       if (ky * ky <= c.kf2) {
         const double ef = forced_shell_energy3<T, N>(a.us, a.ld_us, c.kf2, s_red) /
@@ -399,10 +429,15 @@ k_ns3_yinv(NS3Y<T> a) {
   using F = XInv<T, N>;
   __shared__ cx<T> s_tw[N];
   __shared__ cx<T> s_buf[N * C::LS];
-  const int tid = threadIdx.x, p = tid % TI, j = tid / TI;
-  const int kz = blockIdx.x * TI + p;
+  const int tid = threadIdx.x, p = tid % C::TIX, j = tid / C::TIX;
+  // tiles 0..NTR-1: TIX consecutive kz < N/2 of one x row; tile NTR: the
+  // kz = N/2 column of TIX consecutive x rows
+  constexpr int NTR = (N / 2) / C::TIX;
+  const bool nyq = NTR > 0 && (int)blockIdx.x == NTR;
+  if (nyq && (int)blockIdx.y >= N / C::TIX) return;  // whole workgroup, before any barrier
+  const int kz = nyq ? N / 2 : (int)blockIdx.x * C::TIX + p;
   const bool ok = kz < Nh;
-  const int xi = blockIdx.y;
+  const int xi = nyq ? (int)blockIdx.y * C::TIX + p : (int)blockIdx.y;
   const int kzv = kz;
   stage_table(s_tw, a.tw, N);
   __syncthreads();
@@ -419,11 +454,18 @@ k_ns3_yinv(NS3Y<T> a) {
     for (int s = 0; s < F::NOUT; ++s)
       if (F::out_valid(j, s)) a.out[f * fs + base + (size_t)F::out_index(j, s) * a.ld] = v[s];
   };
+  // loads run ahead of the transforms: each field is requested one or two
+  // transforms before it is consumed (software pipelining)
   cx<T> v[F::MAXE];
+  auto take = [&](const cx<T>* src) {
+#pragma unroll
+    for (int s = 0; s < F::NIN; ++s) v[s] = src[s];
+  };
   if (a.nfields == 6) {
-    cx<T> A[F::NIN], B[F::NIN];
+    cx<T> A[F::NIN], B[F::NIN], U[F::NIN], D[F::NIN], E[F::NIN];
     load(1, A);
     load(2, B);
+    load(0, U);
 #pragma unroll
     for (int s = 0; s < F::NIN; ++s) {  // omega_x^ = i (ky w^ - kz v^)
       const int ky = kfreq(F::in_index(j, s), N);
@@ -431,30 +473,37 @@ k_ns3_yinv(NS3Y<T> a) {
     }
     F::run(v, s_buf + p, C::LS, j, s_tw, 1);
     store(3, v);
-#pragma unroll
-    for (int s = 0; s < F::NIN; ++s) v[s] = A[s];
+    load(3, D);
+    take(A);
     F::run(v, s_buf + p, C::LS, j, s_tw, 1);
     store(1, v);
-#pragma unroll
-    for (int s = 0; s < F::NIN; ++s) v[s] = B[s];
+    load(4, E);
+    take(B);
     F::run(v, s_buf + p, C::LS, j, s_tw, 1);
     store(2, v);
-    load(0, v);
+    take(U);
     F::run(v, s_buf + p, C::LS, j, s_tw, 1);
     store(0, v);
-    load(3, v);
+    take(D);
     F::run(v, s_buf + p, C::LS, j, s_tw, 1);
     store(4, v);
-    load(4, v);
+    take(E);
     F::run(v, s_buf + p, C::LS, j, s_tw, 1);
     store(5, v);
   } else {
-#pragma unroll
-    for (int f = 0; f < 3; ++f) {
-      load(f, v);
-      F::run(v, s_buf + p, C::LS, j, s_tw, 1);
-      store(f, v);
-    }
+    cx<T> A[F::NIN], B[F::NIN], U[F::NIN];
+    load(0, U);
+    load(1, A);
+    load(2, B);
+    take(U);
+    F::run(v, s_buf + p, C::LS, j, s_tw, 1);
+    store(0, v);
+    take(A);
+    F::run(v, s_buf + p, C::LS, j, s_tw, 1);
+    store(1, v);
+    take(B);
+    F::run(v, s_buf + p, C::LS, j, s_tw, 1);
+    store(2, v);
   }
 }
 
@@ -669,19 +718,25 @@ k_ns3_yfwd(NS3YF<T> a) {
   using F = XFwd<T, N>;
   __shared__ cx<T> s_tw[N];
   __shared__ cx<T> s_buf[N * C::LS];
-  const int tid = threadIdx.x, p = tid % TI, j = tid / TI;
-  const int kz = blockIdx.x * TI + p;
+  const int tid = threadIdx.x, p = tid % C::TIX, j = tid / C::TIX;
+  const int kz = blockIdx.x * C::TIX + p;
   const bool ok = kz < a.KZD;
   const int xi = blockIdx.y;
   stage_table(s_tw, a.tw, N);
   __syncthreads();
+  cx<T> pre[3][F::NIN];
 #pragma unroll
   for (int cc = 0; cc < 3; ++cc) {
-    cx<T> v[F::MAXE];
     const size_t bi = ((size_t)cc * N + xi) * N * a.KZP + kz;
 #pragma unroll
     for (int s = 0; s < F::NIN; ++s)
-      v[s] = (ok && F::in_valid(j, s)) ? a.nz[bi + (size_t)F::in_index(j, s) * a.KZP] : cx<T>{0, 0};
+      pre[cc][s] = (ok && F::in_valid(j, s)) ? a.nz[bi + (size_t)F::in_index(j, s) * a.KZP] : cx<T>{0, 0};
+  }
+#pragma unroll
+  for (int cc = 0; cc < 3; ++cc) {
+    cx<T> v[F::MAXE];
+#pragma unroll
+    for (int s = 0; s < F::NIN; ++s) v[s] = pre[cc][s];
     F::run(v, s_buf + p, C::LS, j, s_tw, 1);
     if (ok) {
       const size_t bo = ((size_t)cc * N + xi) * a.KYD * a.KZP + kz;
