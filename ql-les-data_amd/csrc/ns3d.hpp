@@ -46,8 +46,11 @@
 #define QL_WPE_YF 0
 #endif
 #define QL_WPE_ATTR_(n) __attribute__((amdgpu_waves_per_eu(n)))
+// the x-pass target applies to FP32; FP64 keeps twice the registers per
+// element and is left to the compiler
+template <typename T> struct WpeX { static constexpr int value = sizeof(T) == 4 ? QL_WPE_X : 0; };
 #if QL_WPE_X > 0
-#define QL_ATTR_X QL_WPE_ATTR_(QL_WPE_X)
+#define QL_ATTR_X __attribute__((amdgpu_waves_per_eu(WpeX<T>::value > 0 ? WpeX<T>::value : 1)))
 #else
 #define QL_ATTR_X
 #endif
@@ -263,6 +266,12 @@ k_ns3_x(NS3X<T> a) {
     }
 
     // ---- pointwise: dealias kx, projection, viscous, forcing, RK update ----
+    // restrict-qualified views: the loads of later slots may be issued ahead
+    // of the stores of earlier slots (the arrays never overlap)
+    const cx<T>* __restrict__ p_us = a.us;
+    const cx<T>* __restrict__ p_un = a.un;
+    cx<T>* __restrict__ p_acc = a.acc;
+    cx<T>* __restrict__ p_out = a.uout;
     double epart[2] = {0.0, 0.0};
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -285,7 +294,7 @@ k_ns3_x(NS3X<T> a) {
       cx<T> us3[3] = {{0, 0}, {0, 0}, {0, 0}};
       if (valid) {
 #pragma unroll
-        for (int cc = 0; cc < 3; ++cc) us3[cc] = a.us[((size_t)(cc * N + kxi) * N + kyi) * a.ld_us + kz];
+        for (int cc = 0; cc < 3; ++cc) us3[cc] = p_us[((size_t)(cc * N + kxi) * N + kyi) * a.ld_us + kz];
       }
       const T nuk2 = c.nu * (T)k2;
       const bool forced = fscale != (T)0 && k2 > 0 && k2 <= c.kf2;
@@ -313,33 +322,33 @@ k_ns3_x(NS3X<T> a) {
         } else if (!c.visc_if) {
           // spectral_solver.py:74-80
           if (c.stage == 0) {
-            a.acc[cc * cs_ld + i_ld] = cadd(us3[cc], cscale(du[cc], c.ca));
+            p_acc[cc * cs_ld + i_ld] = cadd(us3[cc], cscale(du[cc], c.ca));
             out = cadd(us3[cc], cscale(du[cc], c.cb));
           } else if (c.stage < 3) {
-            const cx<T> un = a.un[cc * cs_un + i_un];
-            a.acc[cc * cs_ld + i_ld] = cadd(a.acc[cc * cs_ld + i_ld], cscale(du[cc], c.ca));
+            const cx<T> un = p_un[cc * cs_un + i_un];
+            p_acc[cc * cs_ld + i_ld] = cadd(p_acc[cc * cs_ld + i_ld], cscale(du[cc], c.ca));
             out = cadd(un, cscale(du[cc], c.cb));
           } else {
-            out = cadd(a.acc[cc * cs_ld + i_ld], cscale(du[cc], c.ca));
+            out = cadd(p_acc[cc * cs_ld + i_ld], cscale(du[cc], c.ca));
           }
         } else {
           const T dt = c.dt;
           if (c.stage == 0) {
-            a.acc[cc * cs_ld + i_ld] = cadd(cscale(us3[cc], e1), cscale(du[cc], dt / (T)6 * e1));
+            p_acc[cc * cs_ld + i_ld] = cadd(cscale(us3[cc], e1), cscale(du[cc], dt / (T)6 * e1));
             out = cscale(cadd(us3[cc], cscale(du[cc], dt / (T)2)), eh);
           } else if (c.stage == 1) {
-            const cx<T> un = a.un[cc * cs_un + i_un];
-            a.acc[cc * cs_ld + i_ld] = cadd(a.acc[cc * cs_ld + i_ld], cscale(du[cc], dt / (T)3 * eh));
+            const cx<T> un = p_un[cc * cs_un + i_un];
+            p_acc[cc * cs_ld + i_ld] = cadd(p_acc[cc * cs_ld + i_ld], cscale(du[cc], dt / (T)3 * eh));
             out = cadd(cscale(un, eh), cscale(du[cc], dt / (T)2));
           } else if (c.stage == 2) {
-            const cx<T> un = a.un[cc * cs_un + i_un];
-            a.acc[cc * cs_ld + i_ld] = cadd(a.acc[cc * cs_ld + i_ld], cscale(du[cc], dt / (T)3 * eh));
+            const cx<T> un = p_un[cc * cs_un + i_un];
+            p_acc[cc * cs_ld + i_ld] = cadd(p_acc[cc * cs_ld + i_ld], cscale(du[cc], dt / (T)3 * eh));
             out = cadd(cscale(un, e1), cscale(du[cc], dt * eh));
           } else {
-            out = cadd(a.acc[cc * cs_ld + i_ld], cscale(du[cc], dt / (T)6));
+            out = cadd(p_acc[cc * cs_ld + i_ld], cscale(du[cc], dt / (T)6));
           }
         }
-        a.uout[cc * cs_out + i_out] = out;
+        p_out[cc * cs_out + i_out] = out;
         w[cc][s] = out;
         const double m2 = (double)out.re * out.re + (double)out.im * out.im;
         e_all += m2;
@@ -602,25 +611,26 @@ k_ns3_z(NS3Z<T> a) {
   const T scale = (T)1 / ((T)N * (T)N * (T)N);
   const int f0 = a.u_in != nullptr ? 3 : 0;  // first field taken through c2r
 
-  cx<T> pf[NE];
-  auto fetch = [&](int f) {
+  // two fields in flight: pf[f & 1] holds field f until it is parked in LDS
+  cx<T> pf[2][NE];
+  auto fetch = [&](int f, cx<T>* dst) {
     const cx<T>* src = a.in + f * fs + (size_t)row0 * a.ld;
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
       const int idx = tid + e * NT;
       if (idx < P * Nh) {
         const int q = idx / Nh, k = idx - q * Nh;
-        pf[e] = src[(size_t)q * a.ld + k];
+        dst[e] = src[(size_t)q * a.ld + k];
       }
     }
   };
-  auto park = [&]() {
+  auto park = [&](const cx<T>* srcr) {
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
       const int idx = tid + e * NT;
       if (idx < P * Nh) {
         const int q = idx / Nh, k = idx - q * Nh;
-        s_buf[q * C::LSR + k] = pf[e];
+        s_buf[q * C::LSR + k] = srcr[e];
       }
     }
   };
@@ -639,13 +649,14 @@ k_ns3_z(NS3Z<T> a) {
       for (int s = 0; s < NO; ++s) uu[f][s] = src[CI::out_index(j, s)];
     }
   }
-  fetch(f0);
+  fetch(f0, pf[f0 & 1]);
+  fetch(f0 + 1, pf[(f0 + 1) & 1]);
 #pragma unroll
   for (int f = 0; f < 6; ++f) {
     if (f < f0) continue;
     __syncthreads();  // previous field's FFT scratch is free
-    park();
-    if (f + 1 < 6) fetch(f + 1);  // in flight during this field's transform
+    park(pf[f & 1]);
+    if (f + 2 < 6) fetch(f + 2, pf[f & 1]);  // in flight during the next two transforms
     __syncthreads();
     cx<T> v[CI::MAXE];
     c2r_prep<T, M, CI>(v, pb, s_tw, j);
