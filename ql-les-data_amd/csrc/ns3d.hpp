@@ -587,33 +587,37 @@ __device__ __forceinline__ void r2c_post_store(cx<T>* dst, const cx<T>* pb, cons
 // of field f+1 are already in flight into registers (software pipelining),
 // and the cross product is accumulated field by field so only u, v, w and
 // the three products stay live.
+// state of one z-pass workgroup; the prefetch ring is two named buffers so
+// every index is a compile-time constant (no private-memory arrays)
 template <typename T, int N>
-__global__ void __launch_bounds__(ZCfg<N>::THREADS) QL_ATTR_Z
-k_ns3_z(NS3Z<T> a) {
+struct ZState {
   using C = ZCfg<N>;
-  constexpr int M = C::M, TP = C::TP, P = C::P;
-  constexpr int Nh = M + 1;
-  constexpr int NT = C::THREADS;
+  static constexpr int M = C::M, TP = C::TP, P = C::P, Nh = M + 1, NT = C::THREADS;
   using CI = PFFT<T, false, M, TP, Reverse<typename C::R>>;  // last pass has exactly TP butterflies
   using CF = PFFT<T, true, M, TP, typename C::R>;           // pass 0 consumes CI's output order
-  static_assert(CI::NOUT == CF::NIN, "z-pass hand-off mismatch");
-  static_assert(CI::BL == TP, "z-pass lanes");
-  constexpr int NO = CI::NOUT;
-  constexpr int NE = (P * Nh + NT - 1) / NT;  // prefetch elements per thread per field
-  __shared__ cx<T> s_tw[N];
-  __shared__ cx<T> s_buf[P * C::LSR];
-  const int tid = threadIdx.x, p = tid / TP, j = tid % TP;
-  const int row0 = blockIdx.x * P;
-  const int row = row0 + p;  // (x, y) row; N*N is a multiple of P
-  cx<T>* pb = s_buf + p * C::LSR;
-  stage_table(s_tw, a.tw, N);
-  const size_t fs = (size_t)N * N * a.ld;
-  const T scale = (T)1 / ((T)N * (T)N * (T)N);
-  const int f0 = a.u_in != nullptr ? 3 : 0;  // first field taken through c2r
+  static constexpr int NO = CI::NOUT;
+  static constexpr int NE = (P * Nh + NT - 1) / NT;  // prefetch elements per thread per field
+  // fields in flight: two in FP32; one in FP64, whose larger registers would
+  // otherwise push the ring into private memory
+  static constexpr int DEPTH = sizeof(T) == 4 ? 2 : 1;
+  const NS3Z<T>& a;
+  cx<T>* s_tw;
+  cx<T>* s_buf;
+  int tid, p, j, row0, row;
+  size_t fs;
+  cx<T> pfa[NE], pfb[NE];
+  cx<T> uu[3][NO];
+  cx<T> nl[3][NO];
 
-  // two fields in flight: pf[f & 1] holds field f until it is parked in LDS
-  cx<T> pf[2][NE];
-  auto fetch = [&](int f, cx<T>* dst) {
+  __device__ __forceinline__ ZState(const NS3Z<T>& a_, cx<T>* tw, cx<T>* buf) : a(a_), s_tw(tw), s_buf(buf) {
+    tid = threadIdx.x;
+    p = tid / TP;
+    j = tid % TP;
+    row0 = blockIdx.x * P;
+    row = row0 + p;
+    fs = (size_t)N * N * a.ld;
+  }
+  __device__ __forceinline__ void fetch(int f, cx<T>* dst) {
     const cx<T>* src = a.in + f * fs + (size_t)row0 * a.ld;
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
@@ -623,58 +627,48 @@ k_ns3_z(NS3Z<T> a) {
         dst[e] = src[(size_t)q * a.ld + k];
       }
     }
-  };
-  auto park = [&](const cx<T>* srcr) {
+  }
+  __device__ __forceinline__ void park(const cx<T>* src) {
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
       const int idx = tid + e * NT;
       if (idx < P * Nh) {
         const int q = idx / Nh, k = idx - q * Nh;
-        s_buf[q * C::LSR + k] = srcr[e];
+        s_buf[q * C::LSR + k] = src[e];
       }
     }
-  };
-
-  cx<T> uu[3][NO];
-  cx<T> nl[3][NO];
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-#pragma unroll
-    for (int s = 0; s < NO; ++s) nl[c][s] = {0, 0};
-  if (f0 == 3) {
-#pragma unroll
-    for (int f = 0; f < 3; ++f) {
-      const cx<T>* src = reinterpret_cast<const cx<T>*>(a.u_in + ((size_t)f * N * N + row) * N);
-#pragma unroll
-      for (int s = 0; s < NO; ++s) uu[f][s] = src[CI::out_index(j, s)];
-    }
   }
-  fetch(f0, pf[f0 & 1]);
-  fetch(f0 + 1, pf[(f0 + 1) & 1]);
-#pragma unroll
-  for (int f = 0; f < 6; ++f) {
-    if (f < f0) continue;
+  template <int F>
+  __device__ __forceinline__ cx<T>* slot() {
+    if constexpr (DEPTH == 1 || (F & 1) == 0) return pfa;
+    else return pfb;
+  }
+  // c2r of field F from its parked rows; field F+2 is requested meanwhile
+  template <int F>
+  __device__ __forceinline__ void field() {
+    cx<T>* pb = s_buf + p * C::LSR;
     __syncthreads();  // previous field's FFT scratch is free
-    park(pf[f & 1]);
-    if (f + 2 < 6) fetch(f + 2, pf[f & 1]);  // in flight during the next two transforms
+    park(slot<F>());
+    if constexpr (F + DEPTH < 6) fetch(F + DEPTH, slot<F>());
     __syncthreads();
     cx<T> v[CI::MAXE];
     c2r_prep<T, M, CI>(v, pb, s_tw, j);
     __syncthreads();
     CI::run(v, pb, 1, j, s_tw, 2);
+    const T scale = (T)1 / ((T)N * (T)N * (T)N);
 #pragma unroll
     for (int s = 0; s < NO; ++s) v[s] = cscale(v[s], scale);
     // (re, im) hold the (even, odd) z samples; u x omega accumulated per field
-    if (f < 3) {
+    if constexpr (F < 3) {
 #pragma unroll
-      for (int s = 0; s < NO; ++s) uu[f][s] = v[s];
-    } else if (f == 3) {  // omega_x: n_y += w ox, n_z -= v ox
+      for (int s = 0; s < NO; ++s) uu[F][s] = v[s];
+    } else if constexpr (F == 3) {  // omega_x: n_y += w ox, n_z -= v ox
 #pragma unroll
       for (int s = 0; s < NO; ++s) {
         nl[1][s] = {nl[1][s].re + uu[2][s].re * v[s].re, nl[1][s].im + uu[2][s].im * v[s].im};
         nl[2][s] = {nl[2][s].re - uu[1][s].re * v[s].re, nl[2][s].im - uu[1][s].im * v[s].im};
       }
-    } else if (f == 4) {  // omega_y: n_x -= w oy, n_z += u oy
+    } else if constexpr (F == 4) {  // omega_y: n_x -= w oy, n_z += u oy
 #pragma unroll
       for (int s = 0; s < NO; ++s) {
         nl[0][s] = {nl[0][s].re - uu[2][s].re * v[s].re, nl[0][s].im - uu[2][s].im * v[s].im};
@@ -688,27 +682,73 @@ k_ns3_z(NS3Z<T> a) {
       }
     }
   }
+  template <int F>
+  __device__ __forceinline__ void fields() {
+    if constexpr (F < 6) {
+      field<F>();
+      fields<F + 1>();
+    }
+  }
+};
+
+template <typename T, int N, bool UIN>
+__device__ __forceinline__ void ns3_z_body(const NS3Z<T>& a, cx<T>* s_tw, cx<T>* s_buf) {
+  using St = ZState<T, N>;
+  constexpr int M = St::M, TP = St::TP, NO = St::NO;
+  using CI = typename St::CI;
+  using CF = typename St::CF;
+  static_assert(CI::NOUT == CF::NIN, "z-pass hand-off mismatch");
+  static_assert(CI::BL == TP, "z-pass lanes");
+  St z(a, s_tw, s_buf);
+  stage_table(s_tw, a.tw, N);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int s = 0; s < NO; ++s) z.nl[c][s] = {0, 0};
+  constexpr int f0 = UIN ? 3 : 0;  // first field taken through c2r
+  if constexpr (UIN) {
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const cx<T>* src = reinterpret_cast<const cx<T>*>(a.u_in + ((size_t)f * N * N + z.row) * N);
+#pragma unroll
+      for (int s = 0; s < NO; ++s) z.uu[f][s] = src[CI::out_index(z.j, s)];
+    }
+  }
+  z.fetch(f0, z.template slot<f0>());
+  if constexpr (St::DEPTH == 2) z.fetch(f0 + 1, z.template slot<f0 + 1>());
+  z.template fields<f0>();
   if (a.u_out != nullptr) {
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
-      cx<T>* dst = reinterpret_cast<cx<T>*>(a.u_out + ((size_t)f * N * N + row) * N);
+      cx<T>* dst = reinterpret_cast<cx<T>*>(a.u_out + ((size_t)f * N * N + z.row) * N);
 #pragma unroll
-      for (int s = 0; s < NO; ++s) dst[CI::out_index(j, s)] = uu[f][s];
+      for (int s = 0; s < NO; ++s) dst[CI::out_index(z.j, s)] = z.uu[f][s];
     }
   }
+  cx<T>* pb = s_buf + z.p * ZCfg<N>::LSR;
 #pragma unroll
   for (int cc = 0; cc < 3; ++cc) {
     cx<T> v[CF::MAXE];
 #pragma unroll
-    for (int s = 0; s < NO; ++s) v[s] = nl[cc][s];
+    for (int s = 0; s < NO; ++s) v[s] = z.nl[cc][s];
     __syncthreads();
-    CF::run(v, pb, 1, j, s_tw, 2);
+    CF::run(v, pb, 1, z.j, s_tw, 2);
 #pragma unroll
     for (int s = 0; s < CF::NOUT; ++s)
-      if (CF::out_valid(j, s)) pb[CF::out_index(j, s)] = v[s];
+      if (CF::out_valid(z.j, s)) pb[CF::out_index(z.j, s)] = v[s];
     __syncthreads();
-    r2c_post_store<T, M>(a.nz + ((size_t)cc * N * N + row) * a.KZP, pb, s_tw, j, TP, a.KZD);
+    r2c_post_store<T, M>(a.nz + ((size_t)cc * N * N + z.row) * a.KZP, pb, s_tw, z.j, TP, a.KZD);
   }
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__(ZCfg<N>::THREADS) QL_ATTR_Z
+k_ns3_z(NS3Z<T> a) {
+  __shared__ cx<T> s_tw[N];
+  __shared__ cx<T> s_buf[ZCfg<N>::P * ZCfg<N>::LSR];
+  // the register prefetch ring is indexed by compile-time field numbers only
+  if (a.u_in != nullptr) ns3_z_body<T, N, true>(a, s_tw, s_buf);
+  else ns3_z_body<T, N, false>(a, s_tw, s_buf);
 }
 
 // ---------------------------------------------------------------------------
