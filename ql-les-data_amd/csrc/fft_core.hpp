@@ -242,7 +242,16 @@ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 // Every thread of the workgroup must call run() (it contains barriers) unless
 // the plan has a single pass.
 // ---------------------------------------------------------------------------
-template <typename T, bool FWD, int M, int TP, typename RL>
+// LDS element address of pencil element n: n*ls, plus one padding slot every
+// 2^PAD elements when PAD > 0 (breaks the stride-R bank pattern of the
+// first Stockham pass when a pencil is contiguous in LDS).
+template <int PAD>
+__device__ __forceinline__ int lds_at(int n) {
+  if constexpr (PAD > 0) return n + (n >> PAD);
+  else return n;
+}
+
+template <typename T, bool FWD, int M, int TP, typename RL, int PAD = 0>
 struct PFFT {
   static constexpr int NP = RL::n;
   static_assert(RL::total() == M, "radix plan does not multiply to M");
@@ -301,7 +310,7 @@ struct PFFT {
       if ((B(p) % TP == 0) || b < B(p)) {
         const int base = (b / Ns) * Ns * Rp + (b % Ns);
 #pragma unroll
-        for (int r = 0; r < Rp; ++r) lds[(base + Ns * r) * ls] = v[t * Rp + r];
+        for (int r = 0; r < Rp; ++r) lds[lds_at<PAD>(base + Ns * r) * ls] = v[t * Rp + r];
       }
     }
   }
@@ -314,7 +323,7 @@ struct PFFT {
       const int b = j + TP * t;
       if ((B(p) % TP == 0) || b < B(p)) {
 #pragma unroll
-        for (int r = 0; r < Rp; ++r) v[t * Rp + r] = lds[(b + B(p) * r) * ls];
+        for (int r = 0; r < Rp; ++r) v[t * Rp + r] = lds[lds_at<PAD>(b + B(p) * r) * ls];
       }
     }
   }

@@ -212,8 +212,9 @@ k_ns2_y(NS2Y<T> a) {
   using C = ZCfg<N>;
   constexpr int M = C::M, TP = C::TP, P = C::P;
   constexpr int Nh = M + 1;
-  using CI = PFFT<T, false, M, TP, Reverse<typename C::R>>;
-  using CF = PFFT<T, true, M, TP, typename C::R>;
+  constexpr int ZP = C::ZPAD;
+  using CI = PFFT<T, false, M, TP, Reverse<typename C::R>, ZP>;
+  using CF = PFFT<T, true, M, TP, typename C::R, ZP>;
   static_assert(CI::NOUT == CF::NIN, "y-pass hand-off mismatch");
   constexpr int NO = CI::NOUT;
   __shared__ cx<T> s_tw[N];
@@ -238,7 +239,7 @@ k_ns2_y(NS2Y<T> a) {
     __syncthreads();
     for (int e = tid; e < P * Nh; e += C::THREADS) {
       const int q = e / Nh, k = e - q * Nh;
-      if (row0 + q < N) s_buf[q * C::LSR + k] = a.in[f * fs + (size_t)(row0 + q) * a.ld + k];
+      if (row0 + q < N) s_buf[q * C::LSR + lds_at<ZP>(k)] = a.in[f * fs + (size_t)(row0 + q) * a.ld + k];
     }
     __syncthreads();
     cx<T> v[CI::MAXE];
@@ -247,7 +248,7 @@ k_ns2_y(NS2Y<T> a) {
       cx<T> z = {0, 0};
       if (CI::in_valid(j, s)) {
         const int n = CI::in_index(j, s);
-        cx<T> xn = pb[n], xr = pb[M - n];
+        cx<T> xn = pb[lds_at<ZP>(n)], xr = pb[lds_at<ZP>(M - n)];
         if (n == 0) {
           xn.im = 0;
           xr.im = 0;
@@ -287,13 +288,13 @@ k_ns2_y(NS2Y<T> a) {
     CF::run(v, pb, 1, j, s_tw, 2);
 #pragma unroll
     for (int s = 0; s < CF::NOUT; ++s)
-      if (CF::out_valid(j, s)) pb[CF::out_index(j, s)] = v[s];
+      if (CF::out_valid(j, s)) pb[lds_at<ZP>(CF::out_index(j, s))] = v[s];
     __syncthreads();
     if (ok) {
       cx<T>* dst = a.ny + ((size_t)cc * N + row) * a.KZP;
       for (int k = j; k < a.KZD; k += TP) {
-        const cx<T> zk = pb[k];
-        const cx<T> zr = cconj(pb[k == 0 ? 0 : M - k]);
+        const cx<T> zk = pb[lds_at<ZP>(k)];
+        const cx<T> zr = cconj(pb[lds_at<ZP>(k == 0 ? 0 : M - k)]);
         const cx<T> e = cscale(cadd(zk, zr), (T)0.5);
         const cx<T> o = crot<T, true>(cscale(csub(zk, zr), (T)0.5));
         dst[k] = cadd(e, cmul(s_tw[k], o));

@@ -545,20 +545,21 @@ struct ZCfg {
   static constexpr int P0 = 256 / TP;
   static constexpr int P = (N <= 64) ? (P0 > 8 ? 8 : P0) : (P0 > 16 ? 16 : P0);
   static constexpr int THREADS = P * TP;
-  static constexpr int LSR = M + 1 + LPAD;
+  static constexpr int ZPAD = 3;  // one LDS padding slot per 8 elements of a pencil
+  static constexpr int LSR = M + (M >> ZPAD) + 1 + LPAD;
 };
 
 // c2r pre-processing of one pencil held in LDS (pb[0..M]) into the pass-0
 // input slots of the M-point inverse FFT: Z'[n] = (X_n + X*_{M-n}) +
 // i W_N^{-n} (X_n - X*_{M-n}); Im of X_0 and X_M is ignored (numpy's irfft).
-template <typename T, int M, class CI>
+template <typename T, int M, class CI, int PAD = 0>
 __device__ __forceinline__ void c2r_prep(cx<T>* v, const cx<T>* pb, const cx<T>* s_tw, int j) {
 #pragma unroll
   for (int s = 0; s < CI::NIN; ++s) {
     cx<T> z = {0, 0};
     if (CI::in_valid(j, s)) {
       const int n = CI::in_index(j, s);
-      cx<T> xn = pb[n], xr = pb[M - n];
+      cx<T> xn = pb[lds_at<PAD>(n)], xr = pb[lds_at<PAD>(M - n)];
       if (n == 0) {
         xn.im = 0;
         xr.im = 0;
@@ -572,11 +573,11 @@ __device__ __forceinline__ void c2r_prep(cx<T>* v, const cx<T>* pb, const cx<T>*
 
 // r2c post-processing from the M-point forward FFT held in LDS (pb[0..M)):
 // X_k = (Z_k + Z*_{M-k})/2 + W_N^k (Z_k - Z*_{M-k})/(2i), k < KZD
-template <typename T, int M>
+template <typename T, int M, int PAD = 0>
 __device__ __forceinline__ void r2c_post_store(cx<T>* dst, const cx<T>* pb, const cx<T>* s_tw, int j, int TP, int KZD) {
   for (int k = j; k < KZD; k += TP) {
-    const cx<T> zk = pb[k];
-    const cx<T> zr = cconj(pb[k == 0 ? 0 : M - k]);
+    const cx<T> zk = pb[lds_at<PAD>(k)];
+    const cx<T> zr = cconj(pb[lds_at<PAD>(k == 0 ? 0 : M - k)]);
     const cx<T> e = cscale(cadd(zk, zr), (T)0.5);
     const cx<T> o = crot<T, true>(cscale(csub(zk, zr), (T)0.5));
     dst[k] = cadd(e, cmul(s_tw[k], o));
@@ -593,8 +594,9 @@ template <typename T, int N>
 struct ZState {
   using C = ZCfg<N>;
   static constexpr int M = C::M, TP = C::TP, P = C::P, Nh = M + 1, NT = C::THREADS;
-  using CI = PFFT<T, false, M, TP, Reverse<typename C::R>>;  // last pass has exactly TP butterflies
-  using CF = PFFT<T, true, M, TP, typename C::R>;           // pass 0 consumes CI's output order
+  static constexpr int ZP = C::ZPAD;
+  using CI = PFFT<T, false, M, TP, Reverse<typename C::R>, ZP>;  // last pass has exactly TP butterflies
+  using CF = PFFT<T, true, M, TP, typename C::R, ZP>;           // pass 0 consumes CI's output order
   static constexpr int NO = CI::NOUT;
   static constexpr int NE = (P * Nh + NT - 1) / NT;  // prefetch elements per thread per field
   // fields in flight: two in FP32; one in FP64, whose larger registers would
@@ -634,7 +636,7 @@ struct ZState {
       const int idx = tid + e * NT;
       if (idx < P * Nh) {
         const int q = idx / Nh, k = idx - q * Nh;
-        s_buf[q * C::LSR + k] = src[e];
+        s_buf[q * C::LSR + lds_at<ZP>(k)] = src[e];
       }
     }
   }
@@ -652,7 +654,7 @@ struct ZState {
     if constexpr (F + DEPTH < 6) fetch(F + DEPTH, slot<F>());
     __syncthreads();
     cx<T> v[CI::MAXE];
-    c2r_prep<T, M, CI>(v, pb, s_tw, j);
+    c2r_prep<T, M, CI, ZP>(v, pb, s_tw, j);
     __syncthreads();
     CI::run(v, pb, 1, j, s_tw, 2);
     const T scale = (T)1 / ((T)N * (T)N * (T)N);
@@ -735,9 +737,9 @@ __device__ __forceinline__ void ns3_z_body(const NS3Z<T>& a, cx<T>* s_tw, cx<T>*
     CF::run(v, pb, 1, z.j, s_tw, 2);
 #pragma unroll
     for (int s = 0; s < CF::NOUT; ++s)
-      if (CF::out_valid(z.j, s)) pb[CF::out_index(z.j, s)] = v[s];
+      if (CF::out_valid(z.j, s)) pb[lds_at<St::ZP>(CF::out_index(z.j, s))] = v[s];
     __syncthreads();
-    r2c_post_store<T, M>(a.nz + ((size_t)cc * N * N + z.row) * a.KZP, pb, s_tw, z.j, TP, a.KZD);
+    r2c_post_store<T, M, St::ZP>(a.nz + ((size_t)cc * N * N + z.row) * a.KZP, pb, s_tw, z.j, TP, a.KZD);
   }
 }
 
