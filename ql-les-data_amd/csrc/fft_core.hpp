@@ -238,7 +238,8 @@ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 //   out: v[t*RL + r] = X[(j + TP*t) + (M/RL)*r]          (final order)
 // Slots whose butterfly index is >= M/R of that pass are ignored.
 // lds points at this pencil's element 0; element n lives at lds[n*ls].
-// tw is a table exp(-2 pi i q / TN) with TN = M * TWS.
+// tw is either a flat table exp(-2 pi i q / TN) with TN = M * tws (tws > 0),
+// or, with tws == 0, the per-plan layout written by fill_tw.
 // Every thread of the workgroup must call run() (it contains barriers) unless
 // the plan has a single pass.
 // ---------------------------------------------------------------------------
@@ -279,6 +280,33 @@ struct PFFT {
   }
   __device__ __forceinline__ static bool out_valid(int j, int s) { return (j + TP * (s / RF)) < BL; }
 
+  // Per-plan twiddle layout: pass p >= 1 owns Ns*(R-1) entries ordered
+  // [b mod Ns][r-1], so lanes with consecutive butterflies read LDS at a
+  // stride of R-1 entries (a flat table is read at strides of many entries,
+  // which serializes on LDS banks).  The inverse uses the conjugates.
+  static constexpr int tw_off(int p) {
+    int o = 0;
+    for (int q = 1; q < p; ++q) o += RL::prod_before(q) * (RL::at(q) - 1);
+    return o;
+  }
+  static constexpr int TWL = tw_off(NP);  // entries of the per-plan table
+  // fill dst[0..TWL) from a table src[q] = exp(-2 pi i q / (M*tws)) (cooperative)
+  __device__ __forceinline__ static void fill_tw(cx<T>* dst, const cx<T>* src, int tws) {
+    fill_tw_pass<1>(dst, src, tws);
+  }
+  template <int p>
+  __device__ __forceinline__ static void fill_tw_pass(cx<T>* dst, const cx<T>* src, int tws) {
+    if constexpr (p < NP) {
+      constexpr int Rp = R(p);
+      constexpr int Ns = RL::prod_before(p);
+      for (int i = threadIdx.x; i < Ns * (Rp - 1); i += blockDim.x) {
+        const int bm = i / (Rp - 1), r = i % (Rp - 1) + 1;
+        dst[tw_off(p) + i] = src[bm * r * (M / (Ns * Rp)) * tws];
+      }
+      fill_tw_pass<p + 1>(dst, src, tws);
+    }
+  }
+
   template <int p>
   __device__ __forceinline__ static void butterflies(cx<T>* v, int j, const cx<T>* tw, int twstride) {
     constexpr int Rp = R(p);
@@ -288,12 +316,22 @@ struct PFFT {
       const int b = j + TP * t;
       if constexpr (Ns > 1) {
         const int bm = b % Ns;
-        // W_{Ns*Rp}^{bm*r} = tw[bm*r*(M/(Ns*Rp)) * twstride]
-        const int step = bm * (M / (Ns * Rp)) * twstride;
+        if (twstride == 0) {
+          // per-plan layout (fill_tw)
+          const cx<T>* tp = tw + tw_off(p) + bm * (Rp - 1) - 1;
 #pragma unroll
-        for (int r = 1; r < Rp; ++r) {
-          cx<T> w = tw[step * r];
-          v[t * Rp + r] = FWD ? cmul(v[t * Rp + r], w) : cmulc(v[t * Rp + r], w);
+          for (int r = 1; r < Rp; ++r) {
+            cx<T> w = tp[r];
+            v[t * Rp + r] = FWD ? cmul(v[t * Rp + r], w) : cmulc(v[t * Rp + r], w);
+          }
+        } else {
+          // flat table: W_{Ns*Rp}^{bm*r} = tw[bm*r*(M/(Ns*Rp)) * twstride]
+          const int step = bm * (M / (Ns * Rp)) * twstride;
+#pragma unroll
+          for (int r = 1; r < Rp; ++r) {
+            cx<T> w = tw[step * r];
+            v[t * Rp + r] = FWD ? cmul(v[t * Rp + r], w) : cmulc(v[t * Rp + r], w);
+          }
         }
       }
       DFT<T, FWD, Rp>::template run<1>(v + t * Rp);

@@ -192,7 +192,8 @@ k_ns3_x(NS3X<T> a) {
   using IF = XInv<T, N>;
   static_assert(FF::NOUT == IF::NIN, "fft hand-off mismatch");
   constexpr int NS = FF::NOUT;
-  __shared__ cx<T> s_tw[N];
+  __shared__ cx<T> s_twf[FF::TWL > 0 ? FF::TWL : 1];
+  __shared__ cx<T> s_twi[IF::TWL > 0 ? IF::TWL : 1];
   __shared__ cx<T> s_buf[N * C::LS];
   __shared__ double s_red[2 * C::THREADS];
 
@@ -216,7 +217,8 @@ k_ns3_x(NS3X<T> a) {
   const int kyi = nyq ? (int)blockIdx.y * C::TIX + p : (int)blockIdx.y;
   const int ky = kfreq(kyi, N);
   const int kzv = kz;  // rfft axis: index N/2 carries +N/2 (spectral_solver.py:34-36)
-  stage_table(s_tw, a.tw, N);
+  FF::fill_tw(s_twf, a.tw, 1);
+  IF::fill_tw(s_twi, a.tw, 1);
   __syncthreads();
 
   cx<T> w[3][NS] = {};  // new stage field (input of the inverse x-FFT)
@@ -247,7 +249,7 @@ k_ns3_x(NS3X<T> a) {
         cx<T> v[FF::MAXE];
 #pragma unroll
         for (int s = 0; s < FF::NIN; ++s) v[s] = pre[cc][s];
-        FF::run(v, s_buf + p, C::LS, j, s_tw, 1);
+        FF::run(v, s_buf + p, C::LS, j, s_twf, 0);
 #pragma unroll
         for (int s = 0; s < NS; ++s) w[cc][s] = v[s];
       }
@@ -407,7 +409,7 @@ k_ns3_x(NS3X<T> a) {
           v[s] = cmuli(csub(cscale(w[1][s], (T)kx), cscale(w[0][s], (T)ky)));
         }
       }
-      IF::run(v, s_buf + p, C::LS, j, s_tw, 1);
+      IF::run(v, s_buf + p, C::LS, j, s_twi, 0);
       if (ok) {
 #pragma unroll
         for (int s = 0; s < IF::NOUT; ++s)
@@ -436,7 +438,7 @@ k_ns3_yinv(NS3Y<T> a) {
   using C = XCfg<N>;
   constexpr int Nh = N / 2 + 1;
   using F = XInv<T, N>;
-  __shared__ cx<T> s_tw[N];
+  __shared__ cx<T> s_tw[F::TWL > 0 ? F::TWL : 1];
   __shared__ cx<T> s_buf[N * C::LS];
   const int tid = threadIdx.x, p = tid % C::TIX, j = tid / C::TIX;
   // tiles 0..NTR-1: TIX consecutive kz < N/2 of one x row; tile NTR: the
@@ -448,7 +450,7 @@ k_ns3_yinv(NS3Y<T> a) {
   const bool ok = kz < Nh;
   const int xi = nyq ? (int)blockIdx.y * C::TIX + p : (int)blockIdx.y;
   const int kzv = kz;
-  stage_table(s_tw, a.tw, N);
+  F::fill_tw(s_tw, a.tw, 1);
   __syncthreads();
   const size_t fs = (size_t)N * N * a.ld;
   const size_t base = (size_t)xi * N * a.ld + kz;
@@ -480,24 +482,24 @@ k_ns3_yinv(NS3Y<T> a) {
       const int ky = kfreq(F::in_index(j, s), N);
       v[s] = cmuli(csub(cscale(B[s], (T)ky), cscale(A[s], (T)kzv)));
     }
-    F::run(v, s_buf + p, C::LS, j, s_tw, 1);
+    F::run(v, s_buf + p, C::LS, j, s_tw, 0);
     store(3, v);
     load(3, D);
     take(A);
-    F::run(v, s_buf + p, C::LS, j, s_tw, 1);
+    F::run(v, s_buf + p, C::LS, j, s_tw, 0);
     store(1, v);
     load(4, E);
     take(B);
-    F::run(v, s_buf + p, C::LS, j, s_tw, 1);
+    F::run(v, s_buf + p, C::LS, j, s_tw, 0);
     store(2, v);
     take(U);
-    F::run(v, s_buf + p, C::LS, j, s_tw, 1);
+    F::run(v, s_buf + p, C::LS, j, s_tw, 0);
     store(0, v);
     take(D);
-    F::run(v, s_buf + p, C::LS, j, s_tw, 1);
+    F::run(v, s_buf + p, C::LS, j, s_tw, 0);
     store(4, v);
     take(E);
-    F::run(v, s_buf + p, C::LS, j, s_tw, 1);
+    F::run(v, s_buf + p, C::LS, j, s_tw, 0);
     store(5, v);
   } else {
     cx<T> A[F::NIN], B[F::NIN], U[F::NIN];
@@ -505,13 +507,13 @@ k_ns3_yinv(NS3Y<T> a) {
     load(1, A);
     load(2, B);
     take(U);
-    F::run(v, s_buf + p, C::LS, j, s_tw, 1);
+    F::run(v, s_buf + p, C::LS, j, s_tw, 0);
     store(0, v);
     take(A);
-    F::run(v, s_buf + p, C::LS, j, s_tw, 1);
+    F::run(v, s_buf + p, C::LS, j, s_tw, 0);
     store(1, v);
     take(B);
-    F::run(v, s_buf + p, C::LS, j, s_tw, 1);
+    F::run(v, s_buf + p, C::LS, j, s_tw, 0);
     store(2, v);
   }
 }
@@ -603,7 +605,9 @@ struct ZState {
   // otherwise push the ring into private memory
   static constexpr int DEPTH = sizeof(T) == 4 ? 2 : 1;
   const NS3Z<T>& a;
-  cx<T>* s_tw;
+  cx<T>* s_tw;   // flat N-point table (c2r / r2c pre- and post-processing)
+  cx<T>* s_twi;  // per-plan table of the inverse half-length FFT
+  cx<T>* s_twf;  // per-plan table of the forward half-length FFT
   cx<T>* s_buf;
   int tid, p, j, row0, row;
   size_t fs;
@@ -611,7 +615,8 @@ struct ZState {
   cx<T> uu[3][NO];
   cx<T> nl[3][NO];
 
-  __device__ __forceinline__ ZState(const NS3Z<T>& a_, cx<T>* tw, cx<T>* buf) : a(a_), s_tw(tw), s_buf(buf) {
+  __device__ __forceinline__ ZState(const NS3Z<T>& a_, cx<T>* tw, cx<T>* twi, cx<T>* twf, cx<T>* buf)
+      : a(a_), s_tw(tw), s_twi(twi), s_twf(twf), s_buf(buf) {
     tid = threadIdx.x;
     p = tid / TP;
     j = tid % TP;
@@ -656,7 +661,7 @@ struct ZState {
     cx<T> v[CI::MAXE];
     c2r_prep<T, M, CI, ZP>(v, pb, s_tw, j);
     __syncthreads();
-    CI::run(v, pb, 1, j, s_tw, 2);
+    CI::run(v, pb, 1, j, s_twi, 0);
     const T scale = (T)1 / ((T)N * (T)N * (T)N);
 #pragma unroll
     for (int s = 0; s < NO; ++s) v[s] = cscale(v[s], scale);
@@ -694,15 +699,18 @@ struct ZState {
 };
 
 template <typename T, int N, bool UIN>
-__device__ __forceinline__ void ns3_z_body(const NS3Z<T>& a, cx<T>* s_tw, cx<T>* s_buf) {
+__device__ __forceinline__ void ns3_z_body(const NS3Z<T>& a, cx<T>* s_tw, cx<T>* s_twi, cx<T>* s_twf,
+                                           cx<T>* s_buf) {
   using St = ZState<T, N>;
   constexpr int M = St::M, TP = St::TP, NO = St::NO;
   using CI = typename St::CI;
   using CF = typename St::CF;
   static_assert(CI::NOUT == CF::NIN, "z-pass hand-off mismatch");
   static_assert(CI::BL == TP, "z-pass lanes");
-  St z(a, s_tw, s_buf);
+  St z(a, s_tw, s_twi, s_twf, s_buf);
   stage_table(s_tw, a.tw, N);
+  CI::fill_tw(s_twi, a.tw, 2);
+  CF::fill_tw(s_twf, a.tw, 2);
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -734,7 +742,7 @@ __device__ __forceinline__ void ns3_z_body(const NS3Z<T>& a, cx<T>* s_tw, cx<T>*
 #pragma unroll
     for (int s = 0; s < NO; ++s) v[s] = z.nl[cc][s];
     __syncthreads();
-    CF::run(v, pb, 1, z.j, s_tw, 2);
+    CF::run(v, pb, 1, z.j, s_twf, 0);
 #pragma unroll
     for (int s = 0; s < CF::NOUT; ++s)
       if (CF::out_valid(z.j, s)) pb[lds_at<St::ZP>(CF::out_index(z.j, s))] = v[s];
@@ -746,11 +754,14 @@ __device__ __forceinline__ void ns3_z_body(const NS3Z<T>& a, cx<T>* s_tw, cx<T>*
 template <typename T, int N>
 __global__ void __launch_bounds__(ZCfg<N>::THREADS) QL_ATTR_Z
 k_ns3_z(NS3Z<T> a) {
+  using St = ZState<T, N>;
   __shared__ cx<T> s_tw[N];
+  __shared__ cx<T> s_twi[St::CI::TWL > 0 ? St::CI::TWL : 1];
+  __shared__ cx<T> s_twf[St::CF::TWL > 0 ? St::CF::TWL : 1];
   __shared__ cx<T> s_buf[ZCfg<N>::P * ZCfg<N>::LSR];
   // the register prefetch ring is indexed by compile-time field numbers only
-  if (a.u_in != nullptr) ns3_z_body<T, N, true>(a, s_tw, s_buf);
-  else ns3_z_body<T, N, false>(a, s_tw, s_buf);
+  if (a.u_in != nullptr) ns3_z_body<T, N, true>(a, s_tw, s_twi, s_twf, s_buf);
+  else ns3_z_body<T, N, false>(a, s_tw, s_twi, s_twf, s_buf);
 }
 
 // ---------------------------------------------------------------------------
@@ -769,13 +780,13 @@ __global__ void __launch_bounds__(XCfg<N>::THREADS) QL_ATTR_YF
 k_ns3_yfwd(NS3YF<T> a) {
   using C = XCfg<N>;
   using F = XFwd<T, N>;
-  __shared__ cx<T> s_tw[N];
+  __shared__ cx<T> s_tw[F::TWL > 0 ? F::TWL : 1];
   __shared__ cx<T> s_buf[N * C::LS];
   const int tid = threadIdx.x, p = tid % C::TIX, j = tid / C::TIX;
   const int kz = blockIdx.x * C::TIX + p;
   const bool ok = kz < a.KZD;
   const int xi = blockIdx.y;
-  stage_table(s_tw, a.tw, N);
+  F::fill_tw(s_tw, a.tw, 1);
   __syncthreads();
   cx<T> pre[3][F::NIN];
 #pragma unroll
@@ -790,7 +801,7 @@ k_ns3_yfwd(NS3YF<T> a) {
     cx<T> v[F::MAXE];
 #pragma unroll
     for (int s = 0; s < F::NIN; ++s) v[s] = pre[cc][s];
-    F::run(v, s_buf + p, C::LS, j, s_tw, 1);
+    F::run(v, s_buf + p, C::LS, j, s_tw, 0);
     if (ok) {
       const size_t bo = ((size_t)cc * N + xi) * a.KYD * a.KZP + kz;
 #pragma unroll
