@@ -20,6 +20,10 @@ class EngineT final : public Engine {
   size_t rows_;  // N*N (3-D) or N (2-D)
   size_t bytes_ = 0;
   std::vector<void*> allocs_;
+  // P: u_n in the padded row stride LD (the caller's spectrum has rows of
+  // N/2+1, which are not cache-line aligned), refreshed by the state load of
+  // each multi-step call and written back to the caller's array on the last step
+  cx<T>* P = nullptr;
   cx<T>*S1a = nullptr, *S1b = nullptr, *S2 = nullptr, *X5 = nullptr, *Y6 = nullptr, *NZ = nullptr, *NY = nullptr;
   double* partials_ = nullptr;
   double* alpha_ = nullptr;
@@ -62,6 +66,23 @@ class EngineT final : public Engine {
   }
 
   // ---------------- 3-D launches ----------------
+  // stage and viscous mode select a compile-time specialisation of the x-pass
+  template <int SG, bool VIF>
+  static void launch_x3_t(NS3X<T>& a, dim3 g, hipStream_t s) {
+    hipLaunchKernelGGL((k_ns3_x<T, N, SG, VIF>), g, dim3(XCfg<N>::THREADS), 0, s, a);
+  }
+  static void launch_x3(NS3X<T>& a, dim3 g, hipStream_t s) {
+    const bool vif = a.c.visc_if != 0;
+    switch (a.c.stage) {
+      case -1: launch_x3_t<-1, false>(a, g, s); break;
+      case 0: vif ? launch_x3_t<0, true>(a, g, s) : launch_x3_t<0, false>(a, g, s); break;
+      case 1: vif ? launch_x3_t<1, true>(a, g, s) : launch_x3_t<1, false>(a, g, s); break;
+      case 2: vif ? launch_x3_t<2, true>(a, g, s) : launch_x3_t<2, false>(a, g, s); break;
+      case 3: vif ? launch_x3_t<3, true>(a, g, s) : launch_x3_t<3, false>(a, g, s); break;
+      case 4: vif ? launch_x3_t<4, true>(a, g, s) : launch_x3_t<4, false>(a, g, s); break;
+      default: throw std::invalid_argument("bad stage");
+    }
+  }
   double C() const { return (double)rows_ * Nh * 2.0 * sizeof(T); }  // one spectral component
   // x-pass / y-inverse tiles along kz: full tiles below N/2 plus one Nyquist-column tile
   static int xtiles() {
@@ -90,9 +111,9 @@ class EngineT final : public Engine {
     dim3 g(xtiles(), N);
     hipEvent_t e0 = nullptr;
     prof_begin(s, &e0);
-    hipLaunchKernelGGL((k_ns3_x<T, N>), g, dim3(XCfg<N>::THREADS), 0, s, a);
+    launch_x3(a, g, s);
     QL_LAUNCHED();
-    prof_end(s, K_XPASS, xpass_bytes(a.c, a.do_inverse, 3, 5), e0);
+    prof_end(s, K_XPASS, xpass_bytes(a.c, a.do_inverse, 3, 5) + (a.copy_dst ? 3.0 * C() : 0.0), e0);
   }
   void yinv3(int nf, hipStream_t s) {
     NS3Y<T> a{X5, Y6, LD, nf, twN_};
@@ -122,11 +143,11 @@ class EngineT final : public Engine {
     QL_LAUNCHED();
     prof_end(s, K_YFWD, 3.0 * N * (N + KYD) * KZD * 2.0 * sizeof(T), e0);
   }
-  NS3X<T> xargs3(cx<T>* S0) const {
+  NS3X<T> xargs3(cx<T>* un, int ld_un) const {
     NS3X<T> a{};
     a.ny = NY;
-    a.un = S0;
-    a.ld_un = Nh;
+    a.un = un;
+    a.ld_un = ld_un;
     a.acc = S2;
     a.x5 = X5;
     a.ld = LD;
@@ -204,6 +225,7 @@ class EngineT final : public Engine {
     rows_ = DIM == 3 ? (size_t)N * N : (size_t)N;
     const size_t fsz = rows_ * LD;
     if (DIM == 3) {
+      P = alloc<cx<T>>(3 * fsz);
       S1a = alloc<cx<T>>(3 * fsz);
       S1b = alloc<cx<T>>(3 * fsz);
       S2 = alloc<cx<T>>(3 * fsz);
@@ -239,24 +261,29 @@ class EngineT final : public Engine {
     const T* u_in = static_cast<const T*>(u_in_v);
     const bool ekin = d_.forcing == QLLES_FORCING_EKIN_TOT;
     if (DIM == 3) {
-      NS3X<T> a = xargs3(S0);
+      NS3X<T> a = xargs3(S0, Nh);
       a.c = ctl(-1);
       a.us = S0;
       a.ld_us = Nh;
+      a.copy_dst = P;
+      a.ld_copy = LD;
       a.do_inverse = 1;
       x3(a, s);
       for (int n = 0; n < nsteps; ++n) {
         const bool more = (n < nsteps - 1) || (u_out != nullptr);
+        // u_{n+1} lands in P, except on the call's last step (caller's array)
+        cx<T>* nxt = n == nsteps - 1 ? S0 : P;
+        const int ld_nxt = n == nsteps - 1 ? Nh : LD;
         for (int st = 0; st < 4; ++st) {
           yinv3(6, s);
           z3((n == 0 && st == 0) ? u_in : nullptr, nullptr, s);
           yfwd3(s);
-          a = xargs3(S0);
+          a = xargs3(P, LD);
           a.c = ctl(st);
-          a.us = st == 0 ? S0 : (st == 2 ? S1b : S1a);
-          a.ld_us = st == 0 ? Nh : LD;
-          a.uout = st == 0 ? S1a : (st == 1 ? S1b : (st == 2 ? S1a : S0));
-          a.ld_uout = st == 3 ? Nh : LD;
+          a.us = st == 0 ? P : (st == 2 ? S1b : S1a);
+          a.ld_us = LD;
+          a.uout = st == 0 ? S1a : (st == 1 ? S1b : (st == 2 ? S1a : nxt));
+          a.ld_uout = st == 3 ? ld_nxt : LD;
           a.do_inverse = (st < 3) || (!ekin && more);
           if (st == 3 && ekin) a.c.partials = partials_;
           x3(a, s);
@@ -266,19 +293,19 @@ class EngineT final : public Engine {
           hipLaunchKernelGGL(k_ekin_alpha, dim3(1), dim3(RED_THREADS), 0, s, partials_, nxblk_, 1.0 / n6,
                              d_.e_kin_init, d_.dt, alpha_, e_inj ? e_inj + n : nullptr);
           QL_LAUNCHED();
-          a = xargs3(S0);
+          a = xargs3(nxt, ld_nxt);
           a.c = ctl(-1);
           a.c.alpha = alpha_;
-          a.rescale_dst = S0;
-          a.us = S0;
-          a.ld_us = Nh;
+          a.rescale_dst = nxt;
+          a.us = nxt;
+          a.ld_us = ld_nxt;
           a.do_inverse = more ? 1 : 0;
           x3(a, s);
         } else if (e_inj) {
           hipLaunchKernelGGL(k_fill_double, dim3(1), dim3(32), 0, s, e_inj + n, 1, 0.0);
           QL_LAUNCHED();
         }
-        if (after_step) after_step(s);
+        if (after_step) after_step(s, nxt, ld_nxt);
       }
       if (u_out) {
         yinv3(3, s);
@@ -309,7 +336,7 @@ class EngineT final : public Engine {
           hipLaunchKernelGGL(k_fill_double, dim3(1), dim3(32), 0, s, e_inj + n, 1, 0.0);
           QL_LAUNCHED();
         }
-        if (after_step) after_step(s);
+        if (after_step) after_step(s, S0, Nh);
       }
       if (u_out) {
         c2r_contig<T>(N, X5, u_out, 2 * N, LD, (T)1 / ((T)N * (T)N), twN_, s);
@@ -324,7 +351,7 @@ class EngineT final : public Engine {
     cx<T>* du = static_cast<cx<T>*>(du_v);
     const T* u = (rk == 0) ? static_cast<const T*>(u_v) : nullptr;
     if (DIM == 3) {
-      NS3X<T> a = xargs3(uh);
+      NS3X<T> a = xargs3(uh, Nh);
       a.c = ctl(-1);
       a.us = uh;
       a.ld_us = Nh;
@@ -333,7 +360,7 @@ class EngineT final : public Engine {
       yinv3(6, s);
       z3(u, nullptr, s);
       yfwd3(s);
-      a = xargs3(uh);
+      a = xargs3(uh, Nh);
       a.c = ctl(4);
       a.us = uh;
       a.ld_us = Nh;

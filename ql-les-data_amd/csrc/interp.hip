@@ -437,9 +437,11 @@ void interp_field(qlles_interp* it, const void* u, const double* r, int np, void
     field_t<double>(it, static_cast<const double*>(u), r, np, static_cast<double*>(u_r), s);
 }
 
-void interp_hat(qlles_interp* it, int N, const void* u_hat, double* r, int np, void* u_r, double dt, hipStream_t s) {
+void interp_hat(qlles_interp* it, int N, const void* u_hat, int ld, double* r, int np, void* u_r, double dt,
+                hipStream_t s) {
   if (N < it->n || (N % 2) != 0) throw std::invalid_argument("solver grid N must be even and >= n");
-  const int nh = N / 2 + 1;
+  if (ld < N / 2 + 1) throw std::invalid_argument("spectrum row stride below N/2+1");
+  const int nh = ld;
   const double scale = 1.0 / (it->dim == 3 ? (double)N * N * N : (double)N * N);
   double* r_out = dt != 0.0 ? r : nullptr;
   if (it->dtype == QLLES_F32)

@@ -11,5 +11,7 @@ void interp_field(qlles_interp* it, const void* u, const double* r, int np, void
 int interp_dim(const qlles_interp* it);
 int interp_dtype(const qlles_interp* it);
 int interp_n(const qlles_interp* it);
-void interp_hat(qlles_interp* it, int N, const void* u_hat, double* r, int np, void* u_r, double dt, hipStream_t s);
+// u_hat: rfft-layout spectrum of the N-grid field, rows of ld >= N/2+1 complex
+void interp_hat(qlles_interp* it, int N, const void* u_hat, int ld, double* r, int np, void* u_r, double dt,
+                hipStream_t s);
 }  // namespace ql

@@ -102,7 +102,8 @@ struct NS3X {
   cx<T>* acc;        // RK accumulator (row stride ld)
   cx<T>* x5;         // [5][N][N][ld]
   cx<T>* rescale_dst;  // stage -1 + alpha: where masked modes are written back (row stride ld_un)
-  int ld_us, ld_un, ld_uout, ld;
+  cx<T>* copy_dst;     // stage -1: optional copy of the loaded state (row stride ld_copy)
+  int ld_us, ld_un, ld_uout, ld, ld_copy;
   int KZD, KZP, KYD;
   int do_inverse;
   const cx<T>* tw;
@@ -183,7 +184,16 @@ __device__ double forced_shell_energy3(const cx<T>* us, int ld, int kf2, double*
   return 0.5 * acc[0];
 }
 
-template <typename T, int N>
+// exp in the working precision (integrating-factor weights)
+__device__ __forceinline__ float tpexp(float x) { return expf(x); }
+__device__ __forceinline__ double tpexp(double x) { return exp(x); }
+
+// SG: -1 = load the state and inverse-transform only, 0..3 = RK4 stage,
+// 4 = write du (rhs); VIF: integrating-factor viscous term.  Both are
+// compile-time so the epilogue is straight-line code: all of a slot's loads
+// are issued unconditionally (clamped addresses, values selected) and only
+// the stores are predicated, which lets loads of later slots overlap.
+template <typename T, int N, int SG, bool VIF>
 __global__ void __launch_bounds__(XCfg<N>::THREADS) QL_ATTR_X
 k_ns3_x(NS3X<T> a) {
   using C = XCfg<N>;
@@ -214,15 +224,21 @@ k_ns3_x(NS3X<T> a) {
   }
   const int kz = nyq ? N / 2 : (int)blockIdx.x * C::TIX + p;
   const bool ok = kz < Nh;
+  const int kzs = ok ? kz : Nh - 1;  // in-bounds column for the unconditional loads
   const int kyi = nyq ? (int)blockIdx.y * C::TIX + p : (int)blockIdx.y;
   const int ky = kfreq(kyi, N);
   const int kzv = kz;  // rfft axis: index N/2 carries +N/2 (spectral_solver.py:34-36)
   FF::fill_tw(s_twf, a.tw, 1);
   IF::fill_tw(s_twi, a.tw, 1);
   __syncthreads();
+  const cx<T>* __restrict__ p_us = a.us;
+  const cx<T>* __restrict__ p_un = a.un;
+  cx<T>* __restrict__ p_acc = a.acc;
+  cx<T>* __restrict__ p_out = a.uout;
+  const cx<T> zero = {0, 0};
 
-  cx<T> w[3][NS] = {};  // new stage field (input of the inverse x-FFT)
-  if (c.stage >= 0) {
+  cx<T> w[3][NS] = {};  // N^ after the forward FFT, then the new stage field
+  if constexpr (SG >= 0) {
     // ---- forward x-FFT of the dealiased nonlinear term ----
     const int KZD = a.KZD, KYD = a.KYD;
     // N^ is zero outside |k| < kmax: tiles without a kept mode skip the
@@ -231,9 +247,7 @@ k_ns3_x(NS3X<T> a) {
     if (blk_fwd) {
       const int kyd = kyi < KZD ? kyi : kyi - (N - KYD);
       const bool fwd_ok = ok && kz < KZD;
-      // all three components are requested before the first transform so their
-      // HBM latency overlaps; w holds N^ after the forward FFT and is
-      // overwritten slot by slot with the new stage field
+      // all three components are requested before the first transform
       cx<T> pre[3][FF::NIN];
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc) {
@@ -241,7 +255,7 @@ k_ns3_x(NS3X<T> a) {
         for (int s = 0; s < FF::NIN; ++s) {
           pre[cc][s] = (fwd_ok && FF::in_valid(j, s))
                            ? a.ny[((size_t)(cc * N + FF::in_index(j, s)) * KYD + kyd) * a.KZP + kz]
-                           : cx<T>{0, 0};
+                           : zero;
         }
       }
 #pragma unroll
@@ -268,12 +282,9 @@ k_ns3_x(NS3X<T> a) {
     }
 
     // ---- pointwise: dealias kx, projection, viscous, forcing, RK update ----
-    // restrict-qualified views: the loads of later slots may be issued ahead
-    // of the stores of earlier slots (the arrays never overlap)
-    const cx<T>* __restrict__ p_us = a.us;
-    const cx<T>* __restrict__ p_un = a.un;
-    cx<T>* __restrict__ p_acc = a.acc;
-    cx<T>* __restrict__ p_out = a.uout;
+    const size_t cs_us = (size_t)N * N * a.ld_us, cs_un = (size_t)N * N * a.ld_un;
+    const size_t cs_ld = (size_t)N * N * a.ld, cs_out = (size_t)N * N * a.ld_uout;
+    const double wgt = (kz == 0 || kz == N / 2) ? 1.0 : 2.0;
     double epart[2] = {0.0, 0.0};
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -281,84 +292,78 @@ k_ns3_x(NS3X<T> a) {
       const int kx = kfreq(kxi, N);
       const bool valid = ok && FF::out_valid(j, s);
       const bool keep_x = keep_k(kxi, N, a.KZD);
-      cx<T> n0 = keep_x ? w[0][s] : cx<T>{0, 0};
-      cx<T> n1 = keep_x ? w[1][s] : cx<T>{0, 0};
-      cx<T> n2 = keep_x ? w[2][s] : cx<T>{0, 0};
+      const size_t row = (size_t)kxi * N + kyi;
+      // loads first, unconditionally (clamped column), for all components
+      cx<T> u3[3], n3[3], c3[3];
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        u3[cc] = p_us[cc * cs_us + row * a.ld_us + kzs];
+        if constexpr (SG == 1 || SG == 2) n3[cc] = p_un[cc * cs_un + row * a.ld_un + kzs];
+        if constexpr (SG >= 1 && SG <= 3) c3[cc] = p_acc[cc * cs_ld + row * a.ld + kzs];
+      }
+      const cx<T> n0 = keep_x ? w[0][s] : zero;
+      const cx<T> n1 = keep_x ? w[1][s] : zero;
+      const cx<T> n2 = keep_x ? w[2][s] : zero;
       const int k2 = kx * kx + ky * ky + kzv * kzv;
       const T inv = k2 == 0 ? (T)0 : (T)1 / (T)k2;
       // p^ = sum_c N^_c k_c / k^2 (spectral_solver.py:57)
       const cx<T> pr = {(n0.re * (T)kx + n1.re * (T)ky + n2.re * (T)kzv) * inv,
                         (n0.im * (T)kx + n1.im * (T)ky + n2.im * (T)kzv) * inv};
+      const T nuk2 = c.nu * (T)k2;
+      const T fsc = (k2 > 0 && k2 <= c.kf2) ? fscale : (T)0;
       cx<T> du[3];
       du[0] = csub(n0, cscale(pr, (T)kx));
       du[1] = csub(n1, cscale(pr, (T)ky));
       du[2] = csub(n2, cscale(pr, (T)kzv));
-      cx<T> us3[3] = {{0, 0}, {0, 0}, {0, 0}};
-      if (valid) {
-#pragma unroll
-        for (int cc = 0; cc < 3; ++cc) us3[cc] = p_us[((size_t)(cc * N + kxi) * N + kyi) * a.ld_us + kz];
-      }
-      const T nuk2 = c.nu * (T)k2;
-      const bool forced = fscale != (T)0 && k2 > 0 && k2 <= c.kf2;
-#pragma unroll
-      for (int cc = 0; cc < 3; ++cc) {
-        if (!c.visc_if) du[cc] = csub(du[cc], cscale(us3[cc], nuk2));  // spectral_solver.py:59
-        if (forced) du[cc] = cadd(du[cc], cscale(us3[cc], fscale));
-      }
-      if (!valid) continue;
-      const size_t i_ld = ((size_t)kxi * N + kyi) * a.ld + kz;
-      const size_t i_un = ((size_t)kxi * N + kyi) * a.ld_un + kz;
-      const size_t i_out = ((size_t)kxi * N + kyi) * a.ld_uout + kz;
-      const size_t cs_ld = (size_t)N * N * a.ld, cs_un = (size_t)N * N * a.ld_un, cs_out = (size_t)N * N * a.ld_uout;
-      double e_all = 0, e_low = 0;
       T eh = 1, e1 = 1;
-      if (c.visc_if) {  // integrating factor exp(-nu k^2 h), exact for the linear part
-        eh = (T)exp(-(double)c.nu * k2 * 0.5 * (double)c.dt);
-        e1 = (T)exp(-(double)c.nu * k2 * (double)c.dt);
+      if constexpr (VIF) {  // integrating factor exp(-nu k^2 h), exact for the linear part
+        eh = tpexp(-nuk2 * (T)0.5 * c.dt);
+        e1 = tpexp(-nuk2 * c.dt);
       }
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc) {
-        cx<T> out;
-        if (c.stage == 4) {
+        if constexpr (!VIF) du[cc] = csub(du[cc], cscale(u3[cc], nuk2));  // spectral_solver.py:59
+        du[cc] = cadd(du[cc], cscale(u3[cc], fsc));
+        cx<T> out, acc_new = zero;
+        if constexpr (SG == 4) {
           out = du[cc];
-        } else if (!c.visc_if) {
+        } else if constexpr (!VIF) {
           // spectral_solver.py:74-80
-          if (c.stage == 0) {
-            p_acc[cc * cs_ld + i_ld] = cadd(us3[cc], cscale(du[cc], c.ca));
-            out = cadd(us3[cc], cscale(du[cc], c.cb));
-          } else if (c.stage < 3) {
-            const cx<T> un = p_un[cc * cs_un + i_un];
-            p_acc[cc * cs_ld + i_ld] = cadd(p_acc[cc * cs_ld + i_ld], cscale(du[cc], c.ca));
-            out = cadd(un, cscale(du[cc], c.cb));
+          if constexpr (SG == 0) {
+            acc_new = cadd(u3[cc], cscale(du[cc], c.ca));
+            out = cadd(u3[cc], cscale(du[cc], c.cb));
+          } else if constexpr (SG < 3) {
+            acc_new = cadd(c3[cc], cscale(du[cc], c.ca));
+            out = cadd(n3[cc], cscale(du[cc], c.cb));
           } else {
-            out = cadd(p_acc[cc * cs_ld + i_ld], cscale(du[cc], c.ca));
+            out = cadd(c3[cc], cscale(du[cc], c.ca));
           }
         } else {
           const T dt = c.dt;
-          if (c.stage == 0) {
-            p_acc[cc * cs_ld + i_ld] = cadd(cscale(us3[cc], e1), cscale(du[cc], dt / (T)6 * e1));
-            out = cscale(cadd(us3[cc], cscale(du[cc], dt / (T)2)), eh);
-          } else if (c.stage == 1) {
-            const cx<T> un = p_un[cc * cs_un + i_un];
-            p_acc[cc * cs_ld + i_ld] = cadd(p_acc[cc * cs_ld + i_ld], cscale(du[cc], dt / (T)3 * eh));
-            out = cadd(cscale(un, eh), cscale(du[cc], dt / (T)2));
-          } else if (c.stage == 2) {
-            const cx<T> un = p_un[cc * cs_un + i_un];
-            p_acc[cc * cs_ld + i_ld] = cadd(p_acc[cc * cs_ld + i_ld], cscale(du[cc], dt / (T)3 * eh));
-            out = cadd(cscale(un, e1), cscale(du[cc], dt * eh));
+          if constexpr (SG == 0) {
+            acc_new = cadd(cscale(u3[cc], e1), cscale(du[cc], dt / (T)6 * e1));
+            out = cscale(cadd(u3[cc], cscale(du[cc], dt / (T)2)), eh);
+          } else if constexpr (SG == 1) {
+            acc_new = cadd(c3[cc], cscale(du[cc], dt / (T)3 * eh));
+            out = cadd(cscale(n3[cc], eh), cscale(du[cc], dt / (T)2));
+          } else if constexpr (SG == 2) {
+            acc_new = cadd(c3[cc], cscale(du[cc], dt / (T)3 * eh));
+            out = cadd(cscale(n3[cc], e1), cscale(du[cc], dt * eh));
           } else {
-            out = cadd(p_acc[cc * cs_ld + i_ld], cscale(du[cc], dt / (T)6));
+            out = cadd(c3[cc], cscale(du[cc], dt / (T)6));
           }
         }
-        p_out[cc * cs_out + i_out] = out;
-        w[cc][s] = out;
-        const double m2 = (double)out.re * out.re + (double)out.im * out.im;
-        e_all += m2;
-        if (k2 > 0 && k2 <= c.kf2) e_low += m2;
+        if (valid) {
+          if constexpr (SG >= 0 && SG < 3) p_acc[cc * cs_ld + row * a.ld + kz] = acc_new;
+          p_out[cc * cs_out + row * a.ld_uout + kz] = out;
+        }
+        w[cc][s] = valid ? out : zero;
+        if (c.partials != nullptr && valid) {
+          const double m2 = wgt * ((double)out.re * out.re + (double)out.im * out.im);
+          epart[0] += m2;
+          if (k2 > 0 && k2 <= c.kf2) epart[1] += m2;
+        }
       }
-      const double wgt = (kz == 0 || kz == N / 2) ? 1.0 : 2.0;
-      epart[0] += wgt * e_all;
-      epart[1] += wgt * e_low;
     }
     if (c.partials != nullptr) {
       block_sum<2>(epart, s_red);
@@ -375,21 +380,35 @@ k_ns3_x(NS3X<T> a) {
     for (int s = 0; s < NS; ++s) {
       const int kxi = IF::in_index(j, s);
       const bool valid = ok && IF::in_valid(j, s);
-      const int kx = kfreq(kxi, N);
-      const int k2 = kx * kx + ky * ky + kzv * kzv;
-      const bool masked = c.alpha && k2 > 0 && k2 <= c.kf2;
 #pragma unroll
-      for (int cc = 0; cc < 3; ++cc) {
-        cx<T> val = {0, 0};
-        if (valid) {
-          const size_t idx = ((size_t)(cc * N + kxi) * N + kyi) * a.ld_un + kz;
-          val = a.un[idx];
+      for (int cc = 0; cc < 3; ++cc) w[cc][s] = p_un[((size_t)(cc * N + kxi) * N + kyi) * a.ld_un + kzs];
+    }
+    if (c.alpha) {
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const int kxi = IF::in_index(j, s);
+        const int kx = kfreq(kxi, N);
+        const int k2 = kx * kx + ky * ky + kzv * kzv;
+        const bool masked = ok && IF::in_valid(j, s) && k2 > 0 && k2 <= c.kf2;
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
           if (masked) {
-            val = cscale(val, (T)alpha);  // spectral_solver.py:112
-            a.rescale_dst[idx] = val;
+            w[cc][s] = cscale(w[cc][s], (T)alpha);  // spectral_solver.py:112
+            a.rescale_dst[((size_t)(cc * N + kxi) * N + kyi) * a.ld_un + kz] = w[cc][s];
           }
         }
-        w[cc][s] = val;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const bool valid = ok && IF::in_valid(j, s);
+      if (!valid) {
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) w[cc][s] = zero;
+      } else if (a.copy_dst != nullptr) {
+        const int kxi = IF::in_index(j, s);
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) a.copy_dst[((size_t)(cc * N + kxi) * N + kyi) * a.ld_copy + kz] = w[cc][s];
       }
     }
   }

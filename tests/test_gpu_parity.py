@@ -320,3 +320,50 @@ def test_interp_hat_fused_and_advect(gpu, backend):
     r_new = host(rd)
     np.testing.assert_allclose(r_new, O.shift_fn(r, dt * host(ur)), atol=1e-12)
     assert (r_new >= 0).all() and (r_new < 2 * np.pi).all()
+
+
+@pytest.mark.parametrize("forcing", ["ekin_tot", "power"])
+def test_step_multi_forced_equals_repeated(gpu, forcing):
+    """Forced K-step call == K one-step calls, bit for bit (u_n kept in the padded buffer between steps)."""
+    from l3es_amd import set_up_solver
+
+    kw = dict(forcing_type=forcing, e_kin_target=0.5, power=0.2)
+    u, uh, _, _, step, _, _, _, _ = set_up_solver(32, 0.01, 3, "HIT", 0.005, 7, 32, 2, precision="float32",
+                                                  warmup=False, **kw)
+    e2 = torch.zeros(3, dtype=torch.float64, device=uh.device)
+    uh2, u2 = uh.clone(), torch.empty_like(u)
+    step.advance(u2, uh2, 3, u2, e_inj=e2, use_u=False)
+    uh3, u3 = uh.clone(), torch.empty_like(u)
+    e3 = []
+    for _ in range(3):
+        e = torch.zeros(1, dtype=torch.float64, device=uh.device)
+        step.advance(u3, uh3, 1, u3, e_inj=e, use_u=False)
+        e3.append(float(e[0]))
+    assert np.array_equal(host(uh2), host(uh3))
+    assert np.array_equal(host(u2), host(u3))
+    assert np.array_equal(host(e2), np.array(e3))
+
+
+def test_step_advect_multi_equals_repeated(gpu):
+    """qlles_step_advect(K) == K x (one step, then filter + interpolate + shift) on the same arrays."""
+    from l3es_amd import _lib, set_up_solver
+    from l3es_amd.integrator import spectral_interpolator_wrapper
+
+    N, n, dt = 32, 16, 0.005
+    u, uh, _, _, step, _, _, _, _ = set_up_solver(N, 0.01, 3, "HIT", dt, 11, n, 2, forcing_type="power",
+                                                  precision="float64", warmup=False, power=0.1)
+    it = spectral_interpolator_wrapper(n, (1, 2, 3), backend="nufft", precision="float64")
+    r0 = np.random.default_rng(9).random((300, 3)) * 2 * np.pi
+    lib = _lib.load()
+    uh_a, r_a = uh.clone(), dev(r0)
+    ur_a = torch.empty((300, 3), dtype=torch.float64, device=uh.device)
+    _lib.check(lib.qlles_step_advect(step.plan.handle, it.it.handle, None, _lib.ptr(uh_a), 3, _lib.ptr(r_a), 300,
+                                     _lib.ptr(ur_a), None, _lib.stream_ptr()))
+    uh_b, r_b = uh.clone(), dev(r0)
+    for _ in range(3):
+        step.advance(None, uh_b, 1, None, use_u=False)
+        ur_b = it.from_hat(uh_b, r_b, dt)
+    torch.cuda.synchronize()
+    assert np.array_equal(host(uh_a), host(uh_b))
+    assert np.array_equal(host(ur_a), host(ur_b))
+    assert np.array_equal(host(r_a), host(r_b))
