@@ -252,7 +252,23 @@ __device__ __forceinline__ int lds_at(int n) {
   else return n;
 }
 
-template <typename T, bool FWD, int M, int TP, typename RL, int PAD = 0>
+// LDS hand-off between passes.  SYNC_BLOCK: lanes of one pencil may sit in
+// different wavefronts (workgroup barrier).  SYNC_WAVE: every pencil lies in
+// one wavefront, whose LDS instructions execute in issue order, so only
+// compiler reordering has to be fenced off (no s_barrier).
+enum { SYNC_BLOCK = 0, SYNC_WAVE = 1 };
+template <int SYNC>
+__device__ __forceinline__ void lds_sync() {
+  if constexpr (SYNC == SYNC_BLOCK) {
+    __syncthreads();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+template <typename T, bool FWD, int M, int TP, typename RL, int PAD = 0, int SYNC = SYNC_BLOCK>
 struct PFFT {
   static constexpr int NP = RL::n;
   static_assert(RL::total() == M, "radix plan does not multiply to M");
@@ -370,9 +386,9 @@ struct PFFT {
   __device__ __forceinline__ static void pass_from(cx<T>* v, cx<T>* lds, int ls, int j, const cx<T>* tw, int tws) {
     if constexpr (p < NP) {
       write_lds<p - 1>(v, lds, ls, j);
-      __syncthreads();
+      lds_sync<SYNC>();
       read_lds<p>(v, lds, ls, j);
-      __syncthreads();
+      lds_sync<SYNC>();
       butterflies<p>(v, j, tw, tws);
       pass_from<p + 1>(v, lds, ls, j, tw, tws);
     }

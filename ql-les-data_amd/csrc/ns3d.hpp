@@ -616,10 +616,15 @@ struct ZState {
   using C = ZCfg<N>;
   static constexpr int M = C::M, TP = C::TP, P = C::P, Nh = M + 1, NT = C::THREADS;
   static constexpr int ZP = C::ZPAD;
-  using CI = PFFT<T, false, M, TP, Reverse<typename C::R>, ZP>;  // last pass has exactly TP butterflies
-  using CF = PFFT<T, true, M, TP, typename C::R, ZP>;           // pass 0 consumes CI's output order
+  // every pencil's TP lanes lie in one wavefront, and each wavefront loads
+  // and parks its own pencils: no workgroup barrier after the table set-up
+  static constexpr int WL = NT < 64 ? NT : 64;  // lanes of one wavefront of the block
+  static constexpr int PWV = WL / TP;           // pencils per wavefront
+  static_assert(WL % TP == 0 && P % PWV == 0, "z-pass pencils must not straddle wavefronts");
+  using CI = PFFT<T, false, M, TP, Reverse<typename C::R>, ZP, SYNC_WAVE>;  // last pass: TP butterflies
+  using CF = PFFT<T, true, M, TP, typename C::R, ZP, SYNC_WAVE>;           // pass 0 takes CI's output order
   static constexpr int NO = CI::NOUT;
-  static constexpr int NE = (P * Nh + NT - 1) / NT;  // prefetch elements per thread per field
+  static constexpr int NE = (PWV * Nh + WL - 1) / WL;  // prefetch elements per lane per field
   // fields in flight: two in FP32; one in FP64, whose larger registers would
   // otherwise push the ring into private memory
   static constexpr int DEPTH = sizeof(T) == 4 ? 2 : 1;
@@ -628,7 +633,7 @@ struct ZState {
   cx<T>* s_twi;  // per-plan table of the inverse half-length FFT
   cx<T>* s_twf;  // per-plan table of the forward half-length FFT
   cx<T>* s_buf;
-  int tid, p, j, row0, row;
+  int tid, p, j, row0, row, ln, wrow;
   size_t fs;
   cx<T> pfa[NE], pfb[NE];
   cx<T> uu[3][NO];
@@ -641,14 +646,16 @@ struct ZState {
     j = tid % TP;
     row0 = blockIdx.x * P;
     row = row0 + p;
+    ln = tid % WL;
+    wrow = (tid / WL) * PWV;  // first pencil of this wavefront within the block
     fs = (size_t)N * N * a.ld;
   }
   __device__ __forceinline__ void fetch(int f, cx<T>* dst) {
-    const cx<T>* src = a.in + f * fs + (size_t)row0 * a.ld;
+    const cx<T>* src = a.in + f * fs + (size_t)(row0 + wrow) * a.ld;
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
-      const int idx = tid + e * NT;
-      if (idx < P * Nh) {
+      const int idx = ln + e * WL;
+      if (idx < PWV * Nh) {
         const int q = idx / Nh, k = idx - q * Nh;
         dst[e] = src[(size_t)q * a.ld + k];
       }
@@ -657,10 +664,10 @@ struct ZState {
   __device__ __forceinline__ void park(const cx<T>* src) {
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
-      const int idx = tid + e * NT;
-      if (idx < P * Nh) {
+      const int idx = ln + e * WL;
+      if (idx < PWV * Nh) {
         const int q = idx / Nh, k = idx - q * Nh;
-        s_buf[q * C::LSR + lds_at<ZP>(k)] = src[e];
+        s_buf[(wrow + q) * C::LSR + lds_at<ZP>(k)] = src[e];
       }
     }
   }
@@ -673,13 +680,13 @@ struct ZState {
   template <int F>
   __device__ __forceinline__ void field() {
     cx<T>* pb = s_buf + p * C::LSR;
-    __syncthreads();  // previous field's FFT scratch is free
+    lds_sync<SYNC_WAVE>();  // previous field's FFT scratch is free
     park(slot<F>());
     if constexpr (F + DEPTH < 6) fetch(F + DEPTH, slot<F>());
-    __syncthreads();
+    lds_sync<SYNC_WAVE>();
     cx<T> v[CI::MAXE];
     c2r_prep<T, M, CI, ZP>(v, pb, s_tw, j);
-    __syncthreads();
+    lds_sync<SYNC_WAVE>();
     CI::run(v, pb, 1, j, s_twi, 0);
     const T scale = (T)1 / ((T)N * (T)N * (T)N);
 #pragma unroll
@@ -730,6 +737,7 @@ __device__ __forceinline__ void ns3_z_body(const NS3Z<T>& a, cx<T>* s_tw, cx<T>*
   stage_table(s_tw, a.tw, N);
   CI::fill_tw(s_twi, a.tw, 2);
   CF::fill_tw(s_twf, a.tw, 2);
+  __syncthreads();  // the only workgroup barrier: tables visible to every wavefront
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -760,12 +768,12 @@ __device__ __forceinline__ void ns3_z_body(const NS3Z<T>& a, cx<T>* s_tw, cx<T>*
     cx<T> v[CF::MAXE];
 #pragma unroll
     for (int s = 0; s < NO; ++s) v[s] = z.nl[cc][s];
-    __syncthreads();
+    lds_sync<SYNC_WAVE>();
     CF::run(v, pb, 1, z.j, s_twf, 0);
 #pragma unroll
     for (int s = 0; s < CF::NOUT; ++s)
       if (CF::out_valid(z.j, s)) pb[lds_at<St::ZP>(CF::out_index(z.j, s))] = v[s];
-    __syncthreads();
+    lds_sync<SYNC_WAVE>();
     r2c_post_store<T, M, St::ZP>(a.nz + ((size_t)cc * N * N + z.row) * a.KZP, pb, s_tw, z.j, TP, a.KZD);
   }
 }
