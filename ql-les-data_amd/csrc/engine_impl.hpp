@@ -98,10 +98,11 @@ class EngineT final : public Engine {
       const double nyv = (double)ncomp * N * (DIM == 3 ? KYD : 1) * KZD * 2.0 * sizeof(T);
       b += nyv;                                               // N^ (dealiased modes only)
       b += ncomp * cc;                                        // u_s
-      if (c.stage == 1 || c.stage == 2) b += 2 * ncomp * cc;  // u_n, acc read
-      if (c.stage == 3) b += ncomp * cc;                      // acc read
-      if (c.stage < 3) b += ncomp * cc;                       // acc write
-      if (c.stage == 4) b -= 0;
+      // the 3-D explicit scheme keeps no accumulator between stages 0 and 1
+      const bool full = DIM == 2 || c.visc_if != 0;
+      if (c.stage == 1 || c.stage == 2) b += ncomp * cc;                       // u_n read
+      if (c.stage == 2 || c.stage == 3 || (full && c.stage == 1)) b += ncomp * cc;  // acc read
+      if (c.stage == 1 || c.stage == 2 || (full && c.stage == 0)) b += ncomp * cc;  // acc write
       b += ncomp * cc;                                        // u_{s+1} / du write
     }
     if (inv) b += nx * cc;

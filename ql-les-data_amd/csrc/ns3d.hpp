@@ -188,6 +188,11 @@ __device__ double forced_shell_energy3(const cx<T>* us, int ld, int kf2, double*
 __device__ __forceinline__ float tpexp(float x) { return expf(x); }
 __device__ __forceinline__ double tpexp(double x) { return exp(x); }
 
+// which RK4 stages read / write the accumulator array (explicit viscous term:
+// stages 1-2 write, 2-3 read; integrating factor: stages 0-2 write, 1-3 read)
+template <int SG, bool VIF> struct AccIn { static constexpr bool value = (SG == 2 || SG == 3) || (VIF && SG == 1); };
+template <int SG, bool VIF> struct AccOut { static constexpr bool value = (SG == 1 || SG == 2) || (VIF && SG == 0); };
+
 // SG: -1 = load the state and inverse-transform only, 0..3 = RK4 stage,
 // 4 = write du (rhs); VIF: integrating-factor viscous term.  Both are
 // compile-time so the epilogue is straight-line code: all of a slot's loads
@@ -299,7 +304,7 @@ k_ns3_x(NS3X<T> a) {
       for (int cc = 0; cc < 3; ++cc) {
         u3[cc] = p_us[cc * cs_us + row * a.ld_us + kzs];
         if constexpr (SG == 1 || SG == 2) n3[cc] = p_un[cc * cs_un + row * a.ld_un + kzs];
-        if constexpr (SG >= 1 && SG <= 3) c3[cc] = p_acc[cc * cs_ld + row * a.ld + kzs];
+        if constexpr (AccIn<SG, VIF>::value) c3[cc] = p_acc[cc * cs_ld + row * a.ld + kzs];
       }
       const cx<T> n0 = keep_x ? w[0][s] : zero;
       const cx<T> n1 = keep_x ? w[1][s] : zero;
@@ -328,11 +333,17 @@ k_ns3_x(NS3X<T> a) {
         if constexpr (SG == 4) {
           out = du[cc];
         } else if constexpr (!VIF) {
-          // spectral_solver.py:74-80
+          // spectral_solver.py:74-80.  The stage-0 accumulator u_n + dt/6 k1
+          // equals u_n + (u_1 - u_n)/3, so it is formed at stage 1 from the two
+          // arrays that stage reads anyway instead of being stored at stage 0
+          // (the rounded 1/3 then scales only the increment, not u_n).
           if constexpr (SG == 0) {
-            acc_new = cadd(u3[cc], cscale(du[cc], c.ca));
             out = cadd(u3[cc], cscale(du[cc], c.cb));
-          } else if constexpr (SG < 3) {
+          } else if constexpr (SG == 1) {
+            const cx<T> acc1 = cadd(n3[cc], cscale(csub(u3[cc], n3[cc]), (T)(1.0 / 3.0)));
+            acc_new = cadd(acc1, cscale(du[cc], c.ca));
+            out = cadd(n3[cc], cscale(du[cc], c.cb));
+          } else if constexpr (SG == 2) {
             acc_new = cadd(c3[cc], cscale(du[cc], c.ca));
             out = cadd(n3[cc], cscale(du[cc], c.cb));
           } else {
@@ -354,7 +365,7 @@ k_ns3_x(NS3X<T> a) {
           }
         }
         if (valid) {
-          if constexpr (SG >= 0 && SG < 3) p_acc[cc * cs_ld + row * a.ld + kz] = acc_new;
+          if constexpr (AccOut<SG, VIF>::value) p_acc[cc * cs_ld + row * a.ld + kz] = acc_new;
           p_out[cc * cs_out + row * a.ld_uout + kz] = out;
         }
         w[cc][s] = valid ? out : zero;
