@@ -208,10 +208,18 @@ __global__ void k_curl_hat(const cx<T>* __restrict__ uh, cx<T>* __restrict__ out
 // full spectrum F(k) of u = irfftn(u^): the Hermitian extension of the
 // rfft half, with the rfft-axis DC and Nyquist planes symmetrized as the
 // c2r of irfftn does.  k in integer wavenumber units (any representative).
+// uq: the component's [N][N] Nyquist plane when kz = N/2 is stored apart
+// from the rows (null: column N/2 of the rows)
 template <typename T>
-__device__ __forceinline__ cx<T> full_spectrum3(const cx<T>* u, int N, int ld, int kx, int ky, int kz) {
+__device__ __forceinline__ cx<T> full_spectrum3(const cx<T>* u, int N, int ld, int kx, int ky, int kz,
+                                               const cx<T>* uq = nullptr) {
   const int ix = ((kx % N) + N) % N, iy = ((ky % N) + N) % N, iz = ((kz % N) + N) % N;
   const int jx = ((-kx % N) + N) % N, jy = ((-ky % N) + N) % N;
+  if (uq != nullptr && 2 * iz == N) {
+    const cx<T> a = uq[(size_t)ix * N + iy];
+    const cx<T> b = cconj(uq[(size_t)jx * N + jy]);
+    return cscale(cadd(a, b), (T)0.5);
+  }
   if (iz == 0 || 2 * iz == N) {
     const cx<T> a = u[((size_t)ix * N + iy) * ld + iz];
     const cx<T> b = cconj(u[((size_t)jx * N + jy) * ld + iz]);

@@ -109,8 +109,8 @@ int qlles_step_advect(qlles_plan* p, qlles_interp* it, void* u_out, void* u_hat,
   const int N = p->desc.N;
   const double dt = p->desc.dt;
   int rc = guarded([&] {
-    p->eng->after_step = [=](hipStream_t s, const void* uh, int ld) {
-      ql::interp_hat(it, N, uh, ld, r, np, u_r, dt, s);
+    p->eng->after_step = [=](hipStream_t s, const void* uh, int ld, size_t q) {
+      ql::interp_hat(it, N, uh, ld, q, r, np, u_r, dt, s);
     };
     p->eng->step(u_out, u_hat, nullptr, nsteps, e_inj, (hipStream_t)stream);
   });
@@ -192,7 +192,7 @@ int qlles_interp_hat(qlles_interp* it, int N, const void* u_hat, double* r, int 
                      void* stream) {
   if (!it || !u_hat || !r || !u_r) return fail(QLLES_ERR_ARG, "null argument");
   if (np < 0) return fail(QLLES_ERR_ARG, "np must be >= 0");
-  return guarded([&] { ql::interp_hat(it, N, u_hat, N / 2 + 1, r, np, u_r, dt, (hipStream_t)stream); });
+  return guarded([&] { ql::interp_hat(it, N, u_hat, N / 2 + 1, 0, r, np, u_r, dt, (hipStream_t)stream); });
 }
 
 }  // extern "C"

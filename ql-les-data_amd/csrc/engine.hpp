@@ -37,8 +37,9 @@ class Engine {
   enum Kind { K_XPASS = 0, K_YINV = 1, K_ZPASS = 2, K_YFWD = 3, K_OTHER = 4, NKIND = 5 };
   void profile_enable(bool on);
   // called on the step's stream after every completed step (particle advection)
-  // (stream, spectrum of u_{n+1}, its row stride in complex elements)
-  std::function<void(hipStream_t, const void*, int)> after_step;
+  // (stream, spectrum of u_{n+1}, its row stride in complex elements, offset
+  // of its separate Nyquist plane or 0 when kz = N/2 sits in the rows)
+  std::function<void(hipStream_t, const void*, int, size_t)> after_step;
   void profile_read(double* ms, int* count, double* bytes);
 
  protected:

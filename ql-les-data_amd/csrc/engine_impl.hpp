@@ -17,6 +17,11 @@ class EngineT final : public Engine {
   static constexpr int Nh = N / 2 + 1;
   Desc d_;
   int KZD, KZP, KYD, LD;
+  // 3-D internal spectra (P, S1a, S1b, S2, X5 and Y6 between the fused
+  // kernels): [nf][N][N][LDM] rows (LDM = N/2, cache-line aligned) followed by
+  // the [nf][N][N] plane of kz = N/2 entries, at element qs(nf)
+  int LDM;
+  size_t qs(int nf) const { return (size_t)nf * N * N * LDM; }
   size_t rows_;  // N*N (3-D) or N (2-D)
   size_t bytes_ = 0;
   std::vector<void*> allocs_;
@@ -117,7 +122,9 @@ class EngineT final : public Engine {
     prof_end(s, K_XPASS, xpass_bytes(a.c, a.do_inverse, 3, 5) + (a.copy_dst ? 3.0 * C() : 0.0), e0);
   }
   void yinv3(int nf, hipStream_t s) {
-    NS3Y<T> a{X5, Y6, LD, nf, twN_};
+    // Y6 is left in the split layout for the z-pass; the velocity-only form
+    // (nf = 3) feeds the generic c2r, which takes rows of LD with kz = N/2 inside
+    NS3Y<T> a{X5, Y6, LDM, nf == 6 ? LDM : LD, qs(5), nf == 6 ? qs(6) : 0, nf, twN_};
     dim3 g(xtiles(), N);
     hipEvent_t e0 = nullptr;
     prof_begin(s, &e0);
@@ -126,7 +133,7 @@ class EngineT final : public Engine {
     prof_end(s, K_YINV, (nf == 6 ? 11.0 : 6.0) * C(), e0);
   }
   void z3(const T* u_in, T* u_out, hipStream_t s) {
-    NS3Z<T> a{Y6, NZ, u_out, u_in, LD, KZD, KZP, twN_};
+    NS3Z<T> a{Y6, NZ, u_out, u_in, qs(6), KZD, KZP, twN_};
     hipEvent_t e0 = nullptr;
     prof_begin(s, &e0);
     hipLaunchKernelGGL((k_ns3_z<T, N>), dim3(N * N / ZCfg<N>::P), dim3(ZCfg<N>::THREADS), 0, s, a);
@@ -144,14 +151,17 @@ class EngineT final : public Engine {
     QL_LAUNCHED();
     prof_end(s, K_YFWD, 3.0 * N * (N + KYD) * KZD * 2.0 * sizeof(T), e0);
   }
-  NS3X<T> xargs3(cx<T>* un, int ld_un) const {
+  NS3X<T> xargs3(cx<T>* un, int ld_un, size_t q_un) const {
     NS3X<T> a{};
     a.ny = NY;
     a.un = un;
     a.ld_un = ld_un;
+    a.q_un = q_un;
     a.acc = S2;
     a.x5 = X5;
-    a.ld = LD;
+    a.ld = LDM;
+    a.q_acc = qs(3);
+    a.q_x5 = qs(5);
     a.KZD = KZD;
     a.KZP = KZP;
     a.KYD = KYD;
@@ -223,6 +233,7 @@ class EngineT final : public Engine {
     KZP = round_up(KZD, 16);
     KYD = 2 * KZD - 1;
     LD = round_up(Nh, 16);
+    LDM = N / 2;
     rows_ = DIM == 3 ? (size_t)N * N : (size_t)N;
     const size_t fsz = rows_ * LD;
     if (DIM == 3) {
@@ -262,29 +273,34 @@ class EngineT final : public Engine {
     const T* u_in = static_cast<const T*>(u_in_v);
     const bool ekin = d_.forcing == QLLES_FORCING_EKIN_TOT;
     if (DIM == 3) {
-      NS3X<T> a = xargs3(S0, Nh);
+      NS3X<T> a = xargs3(S0, Nh, 0);
       a.c = ctl(-1);
       a.us = S0;
       a.ld_us = Nh;
       a.copy_dst = P;
-      a.ld_copy = LD;
+      a.ld_copy = LDM;
+      a.q_copy = qs(3);
       a.do_inverse = 1;
       x3(a, s);
       for (int n = 0; n < nsteps; ++n) {
         const bool more = (n < nsteps - 1) || (u_out != nullptr);
         // u_{n+1} lands in P, except on the call's last step (caller's array)
-        cx<T>* nxt = n == nsteps - 1 ? S0 : P;
-        const int ld_nxt = n == nsteps - 1 ? Nh : LD;
+        const bool last = n == nsteps - 1;
+        cx<T>* nxt = last ? S0 : P;
+        const int ld_nxt = last ? Nh : LDM;
+        const size_t q_nxt = last ? 0 : qs(3);
         for (int st = 0; st < 4; ++st) {
           yinv3(6, s);
           z3((n == 0 && st == 0) ? u_in : nullptr, nullptr, s);
           yfwd3(s);
-          a = xargs3(P, LD);
+          a = xargs3(P, LDM, qs(3));
           a.c = ctl(st);
           a.us = st == 0 ? P : (st == 2 ? S1b : S1a);
-          a.ld_us = LD;
+          a.ld_us = LDM;
+          a.q_us = qs(3);
           a.uout = st == 0 ? S1a : (st == 1 ? S1b : (st == 2 ? S1a : nxt));
-          a.ld_uout = st == 3 ? ld_nxt : LD;
+          a.ld_uout = st == 3 ? ld_nxt : LDM;
+          a.q_uout = st == 3 ? q_nxt : qs(3);
           a.do_inverse = (st < 3) || (!ekin && more);
           if (st == 3 && ekin) a.c.partials = partials_;
           x3(a, s);
@@ -294,19 +310,20 @@ class EngineT final : public Engine {
           hipLaunchKernelGGL(k_ekin_alpha, dim3(1), dim3(RED_THREADS), 0, s, partials_, nxblk_, 1.0 / n6,
                              d_.e_kin_init, d_.dt, alpha_, e_inj ? e_inj + n : nullptr);
           QL_LAUNCHED();
-          a = xargs3(nxt, ld_nxt);
+          a = xargs3(nxt, ld_nxt, q_nxt);
           a.c = ctl(-1);
           a.c.alpha = alpha_;
           a.rescale_dst = nxt;
           a.us = nxt;
           a.ld_us = ld_nxt;
+          a.q_us = q_nxt;
           a.do_inverse = more ? 1 : 0;
           x3(a, s);
         } else if (e_inj) {
           hipLaunchKernelGGL(k_fill_double, dim3(1), dim3(32), 0, s, e_inj + n, 1, 0.0);
           QL_LAUNCHED();
         }
-        if (after_step) after_step(s, nxt, ld_nxt);
+        if (after_step) after_step(s, nxt, ld_nxt, q_nxt);
       }
       if (u_out) {
         yinv3(3, s);
@@ -337,7 +354,7 @@ class EngineT final : public Engine {
           hipLaunchKernelGGL(k_fill_double, dim3(1), dim3(32), 0, s, e_inj + n, 1, 0.0);
           QL_LAUNCHED();
         }
-        if (after_step) after_step(s, S0, Nh);
+        if (after_step) after_step(s, S0, Nh, 0);
       }
       if (u_out) {
         c2r_contig<T>(N, X5, u_out, 2 * N, LD, (T)1 / ((T)N * (T)N), twN_, s);
@@ -352,7 +369,7 @@ class EngineT final : public Engine {
     cx<T>* du = static_cast<cx<T>*>(du_v);
     const T* u = (rk == 0) ? static_cast<const T*>(u_v) : nullptr;
     if (DIM == 3) {
-      NS3X<T> a = xargs3(uh, Nh);
+      NS3X<T> a = xargs3(uh, Nh, 0);
       a.c = ctl(-1);
       a.us = uh;
       a.ld_us = Nh;
@@ -361,7 +378,7 @@ class EngineT final : public Engine {
       yinv3(6, s);
       z3(u, nullptr, s);
       yfwd3(s);
-      a = xargs3(uh, Nh);
+      a = xargs3(uh, Nh, 0);
       a.c = ctl(4);
       a.us = uh;
       a.ld_us = Nh;

@@ -88,7 +88,7 @@ __device__ __forceinline__ double wrap2pi(double v) {
 
 // coefficients G_k from a source rfft spectrum (grid Ns, row stride ld)
 template <typename T>
-__global__ void k_build_coef(const cx<T>* __restrict__ uh, int Ns, int ld, int dim, int n, double scale,
+__global__ void k_build_coef(const cx<T>* __restrict__ uh, int Ns, int ld, size_t qplane, int dim, int n, double scale,
                              const T* __restrict__ corr, cx<T>* __restrict__ out, int mode, int nf) {
   const int ncomp = dim;
   const size_t per = dim == 3 ? (size_t)n * n * n : (size_t)n * n;
@@ -98,6 +98,8 @@ __global__ void k_build_coef(const cx<T>* __restrict__ uh, int Ns, int ld, int d
     const int c = (int)(m / per);
     const size_t q = m % per;
     const cx<T>* u = uh + c * cs_in;
+    // 3-D source with a separate Nyquist plane at element qplane (0: in the rows)
+    const cx<T>* uq = (dim == 3 && qplane != 0) ? uh + qplane + (size_t)c * Ns * Ns : nullptr;
     cx<T> g;
     int ix, iy, iz = 0;
     if (dim == 3) {
@@ -106,7 +108,8 @@ __global__ void k_build_coef(const cx<T>* __restrict__ uh, int Ns, int ld, int d
       iz = (int)(q % n);
       const int kx = ix - h, ky = iy - h, kz = iz - h;
       const int sx = kx == -h ? h : kx, sy = ky == -h ? h : ky, sz = kz == -h ? h : kz;
-      g = cscale(cadd(full_spectrum3(u, Ns, ld, kx, ky, kz), full_spectrum3(u, Ns, ld, sx, sy, sz)), (T)(0.5 * scale));
+      g = cscale(cadd(full_spectrum3(u, Ns, ld, kx, ky, kz, uq), full_spectrum3(u, Ns, ld, sx, sy, sz, uq)),
+                 (T)(0.5 * scale));
       if (mode == 0) {
         g = cscale(g, corr[ix] * corr[iy] * corr[iz]);
         const size_t o = (((size_t)c * nf + (kx + nf) % nf) * nf + (ky + nf) % nf) * nf + (kz + nf) % nf;
@@ -312,13 +315,13 @@ void setup_t(qlles_interp* it) {
 }
 
 template <typename T>
-void run_t(qlles_interp* it, const cx<T>* uh, int Ns, int ld, double scale, const double* r, int np, T* ur,
-           double dt, double* r_out, hipStream_t s) {
+void run_t(qlles_interp* it, const cx<T>* uh, int Ns, int ld, size_t q, double scale, const double* r, int np,
+           T* ur, double dt, double* r_out, hipStream_t s) {
   const int n = it->n, d = it->dim;
   const size_t per_n = d == 3 ? (size_t)n * n * n : (size_t)n * n;
   const int nb = (int)std::min<size_t>((per_n * d + 255) / 256, 8192);
   const int mode = it->backend == QLLES_INTERP_NUFFT ? 0 : 1;
-  hipLaunchKernelGGL((k_build_coef<T>), dim3(nb), dim3(256), 0, s, uh, Ns, ld, d, n, scale,
+  hipLaunchKernelGGL((k_build_coef<T>), dim3(nb), dim3(256), 0, s, uh, Ns, ld, q, d, n, scale,
                      static_cast<const T*>(it->corr), static_cast<cx<T>*>(it->coef), mode, it->nf);
   QL_LAUNCHED();
   if (np == 0) return;
@@ -363,11 +366,11 @@ void field_t(qlles_interp* it, const T* u, const double* r, int np, T* ur, hipSt
     r2c_contig<T>(n, u, uh, 3 * n * n, nh, (T)1, twn, s);
     c2c_strided<T>(n, uh, uh, 3 * n, nh, true, (T)1, twn, s);
     c2c_strided<T>(n, uh, uh, 3, n * nh, true, (T)1, twn, s);
-    run_t<T>(it, uh, n, nh, 1.0 / ((double)n * n * n), r, np, ur, 0.0, nullptr, s);
+    run_t<T>(it, uh, n, nh, 0, 1.0 / ((double)n * n * n), r, np, ur, 0.0, nullptr, s);
   } else {
     r2c_contig<T>(n, u, uh, 2 * n, nh, (T)1, twn, s);
     c2c_strided<T>(n, uh, uh, 2, nh, true, (T)1, twn, s);
-    run_t<T>(it, uh, n, nh, 1.0 / ((double)n * n), r, np, ur, 0.0, nullptr, s);
+    run_t<T>(it, uh, n, nh, 0, 1.0 / ((double)n * n), r, np, ur, 0.0, nullptr, s);
   }
 }
 
@@ -437,17 +440,18 @@ void interp_field(qlles_interp* it, const void* u, const double* r, int np, void
     field_t<double>(it, static_cast<const double*>(u), r, np, static_cast<double*>(u_r), s);
 }
 
-void interp_hat(qlles_interp* it, int N, const void* u_hat, int ld, double* r, int np, void* u_r, double dt,
-                hipStream_t s) {
+void interp_hat(qlles_interp* it, int N, const void* u_hat, int ld, size_t q, double* r, int np, void* u_r,
+                double dt, hipStream_t s) {
   if (N < it->n || (N % 2) != 0) throw std::invalid_argument("solver grid N must be even and >= n");
-  if (ld < N / 2 + 1) throw std::invalid_argument("spectrum row stride below N/2+1");
-  const int nh = ld;
+  if (q == 0 ? ld < N / 2 + 1 : (ld < N / 2 || it->dim != 3))
+    throw std::invalid_argument("spectrum row stride too small for its layout");
   const double scale = 1.0 / (it->dim == 3 ? (double)N * N * N : (double)N * N);
   double* r_out = dt != 0.0 ? r : nullptr;
   if (it->dtype == QLLES_F32)
-    run_t<float>(it, static_cast<const cx<float>*>(u_hat), N, nh, scale, r, np, static_cast<float*>(u_r), dt, r_out, s);
+    run_t<float>(it, static_cast<const cx<float>*>(u_hat), N, ld, q, scale, r, np, static_cast<float*>(u_r), dt,
+                 r_out, s);
   else
-    run_t<double>(it, static_cast<const cx<double>*>(u_hat), N, nh, scale, r, np, static_cast<double*>(u_r), dt,
+    run_t<double>(it, static_cast<const cx<double>*>(u_hat), N, ld, q, scale, r, np, static_cast<double*>(u_r), dt,
                   r_out, s);
 }
 

@@ -11,7 +11,9 @@ void interp_field(qlles_interp* it, const void* u, const double* r, int np, void
 int interp_dim(const qlles_interp* it);
 int interp_dtype(const qlles_interp* it);
 int interp_n(const qlles_interp* it);
-// u_hat: rfft-layout spectrum of the N-grid field, rows of ld >= N/2+1 complex
-void interp_hat(qlles_interp* it, int N, const void* u_hat, int ld, double* r, int np, void* u_r, double dt,
-                hipStream_t s);
+// u_hat: rfft-layout spectrum of the N-grid field, rows of ld complex; the
+// kz = N/2 entries sit in column N/2 of the rows (q == 0, ld >= N/2+1) or,
+// for a 3-D field, in a separate [ncomp][N][N] plane at element q (ld >= N/2)
+void interp_hat(qlles_interp* it, int N, const void* u_hat, int ld, size_t q, double* r, int np, void* u_r,
+                double dt, hipStream_t s);
 }  // namespace ql

@@ -101,9 +101,11 @@ struct NS3X {
   cx<T>* uout;       // u_{s+1} / u_{n+1} / du (row stride ld_uout)
   cx<T>* acc;        // RK accumulator (row stride ld)
   cx<T>* x5;         // [5][N][N][ld]
-  cx<T>* rescale_dst;  // stage -1 + alpha: where masked modes are written back (row stride ld_un)
-  cx<T>* copy_dst;     // stage -1: optional copy of the loaded state (row stride ld_copy)
+  cx<T>* rescale_dst;  // stage -1 + alpha: where masked modes are written back (layout of un)
+  cx<T>* copy_dst;     // stage -1: optional copy of the loaded state
+  // row strides and Nyquist-plane offsets (see ColAt); acc and x5 share ld
   int ld_us, ld_un, ld_uout, ld, ld_copy;
+  size_t q_us, q_un, q_uout, q_acc, q_x5, q_copy;
   int KZD, KZP, KYD;
   int do_inverse;
   const cx<T>* tw;
@@ -135,6 +137,22 @@ template <typename T, int N> using XFwd = PFFT<T, true, N, XCfg<N>::TP, typename
 template <typename T, int N> using XInv = PFFT<T, false, N, XCfg<N>::TP, Reverse<typename XCfg<N>::R>>;
 
 __device__ __forceinline__ bool keep_k(int idx, int N, int KZD) { return idx < KZD || idx > N - KZD; }
+
+// Column kz of an [nf][N][N] (3-D) rfft-layout spectrum: element (c, row) is
+// at off + c*cs + row*rs.  Rows hold ld complex.  The kz = N/2 column is
+// either column N/2 of those rows (q == 0: the caller's layout, ld >= N/2+1)
+// or a separate [nf][N][N] plane starting at element q (the internal layout,
+// ld = N/2: rows start on cache-line boundaries and the Nyquist column is
+// contiguous across rows instead of one element in a line of its own).
+struct ColAt {
+  size_t off, rs, cs;
+  template <int N>
+  __device__ __forceinline__ static ColAt make(int ld, size_t q, int kz) {
+    if (q != 0 && kz == N / 2) return {q, 1, (size_t)N * N};
+    return {(size_t)kz, (size_t)ld, (size_t)N * N * ld};
+  }
+  __device__ __forceinline__ size_t at(int c, size_t row) const { return off + c * cs + row * rs; }
+};
 
 // deterministic block sum of NV doubles per thread (fixed tree order)
 template <int NV>
@@ -287,8 +305,8 @@ k_ns3_x(NS3X<T> a) {
     }
 
     // ---- pointwise: dealias kx, projection, viscous, forcing, RK update ----
-    const size_t cs_us = (size_t)N * N * a.ld_us, cs_un = (size_t)N * N * a.ld_un;
-    const size_t cs_ld = (size_t)N * N * a.ld, cs_out = (size_t)N * N * a.ld_uout;
+    const ColAt l_us = ColAt::make<N>(a.ld_us, a.q_us, kzs), l_un = ColAt::make<N>(a.ld_un, a.q_un, kzs);
+    const ColAt l_acc = ColAt::make<N>(a.ld, a.q_acc, kzs), l_out = ColAt::make<N>(a.ld_uout, a.q_uout, kzs);
     const double wgt = (kz == 0 || kz == N / 2) ? 1.0 : 2.0;
     double epart[2] = {0.0, 0.0};
 #pragma unroll
@@ -302,9 +320,9 @@ k_ns3_x(NS3X<T> a) {
       cx<T> u3[3], n3[3], c3[3];
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc) {
-        u3[cc] = p_us[cc * cs_us + row * a.ld_us + kzs];
-        if constexpr (SG == 1 || SG == 2) n3[cc] = p_un[cc * cs_un + row * a.ld_un + kzs];
-        if constexpr (AccIn<SG, VIF>::value) c3[cc] = p_acc[cc * cs_ld + row * a.ld + kzs];
+        u3[cc] = p_us[l_us.at(cc, row)];
+        if constexpr (SG == 1 || SG == 2) n3[cc] = p_un[l_un.at(cc, row)];
+        if constexpr (AccIn<SG, VIF>::value) c3[cc] = p_acc[l_acc.at(cc, row)];
       }
       const cx<T> n0 = keep_x ? w[0][s] : zero;
       const cx<T> n1 = keep_x ? w[1][s] : zero;
@@ -365,8 +383,8 @@ k_ns3_x(NS3X<T> a) {
           }
         }
         if (valid) {
-          if constexpr (AccOut<SG, VIF>::value) p_acc[cc * cs_ld + row * a.ld + kz] = acc_new;
-          p_out[cc * cs_out + row * a.ld_uout + kz] = out;
+          if constexpr (AccOut<SG, VIF>::value) p_acc[l_acc.at(cc, row)] = acc_new;
+          p_out[l_out.at(cc, row)] = out;
         }
         w[cc][s] = valid ? out : zero;
         if (c.partials != nullptr && valid) {
@@ -387,12 +405,12 @@ k_ns3_x(NS3X<T> a) {
   } else {
     // ---- load the state (optionally rescaling the forced shell) ----
     const double alpha = c.alpha ? *c.alpha : 1.0;
+    const ColAt l_un = ColAt::make<N>(a.ld_un, a.q_un, kzs);
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      const int kxi = IF::in_index(j, s);
-      const bool valid = ok && IF::in_valid(j, s);
+      const size_t row = (size_t)IF::in_index(j, s) * N + kyi;
 #pragma unroll
-      for (int cc = 0; cc < 3; ++cc) w[cc][s] = p_un[((size_t)(cc * N + kxi) * N + kyi) * a.ld_un + kzs];
+      for (int cc = 0; cc < 3; ++cc) w[cc][s] = p_un[l_un.at(cc, row)];
     }
     if (c.alpha) {
 #pragma unroll
@@ -405,7 +423,7 @@ k_ns3_x(NS3X<T> a) {
         for (int cc = 0; cc < 3; ++cc) {
           if (masked) {
             w[cc][s] = cscale(w[cc][s], (T)alpha);  // spectral_solver.py:112
-            a.rescale_dst[((size_t)(cc * N + kxi) * N + kyi) * a.ld_un + kz] = w[cc][s];
+            a.rescale_dst[l_un.at(cc, (size_t)kxi * N + kyi)] = w[cc][s];
           }
         }
       }
@@ -417,9 +435,10 @@ k_ns3_x(NS3X<T> a) {
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc) w[cc][s] = zero;
       } else if (a.copy_dst != nullptr) {
-        const int kxi = IF::in_index(j, s);
+        const ColAt l_cp = ColAt::make<N>(a.ld_copy, a.q_copy, kz);
+        const size_t row = (size_t)IF::in_index(j, s) * N + kyi;
 #pragma unroll
-        for (int cc = 0; cc < 3; ++cc) a.copy_dst[((size_t)(cc * N + kxi) * N + kyi) * a.ld_copy + kz] = w[cc][s];
+        for (int cc = 0; cc < 3; ++cc) a.copy_dst[l_cp.at(cc, row)] = w[cc][s];
       }
     }
   }
@@ -441,10 +460,10 @@ k_ns3_x(NS3X<T> a) {
       }
       IF::run(v, s_buf + p, C::LS, j, s_twi, 0);
       if (ok) {
+        const ColAt l_x5 = ColAt::make<N>(a.ld, a.q_x5, kz);
 #pragma unroll
         for (int s = 0; s < IF::NOUT; ++s)
-          if (IF::out_valid(j, s))
-            a.x5[((size_t)(f * N + IF::out_index(j, s)) * N + kyi) * a.ld + kz] = v[s];
+          if (IF::out_valid(j, s)) a.x5[l_x5.at(f, (size_t)IF::out_index(j, s) * N + kyi)] = v[s];
       }
     }
   }
@@ -455,9 +474,10 @@ k_ns3_x(NS3X<T> a) {
 // ---------------------------------------------------------------------------
 template <typename T>
 struct NS3Y {
-  const cx<T>* in;  // [5][N][N][ld]
-  cx<T>* out;       // [6][N][N][ld]
-  int ld;
+  const cx<T>* in;  // [5][N][N] rows (ld_in, Nyquist plane at q_in)
+  cx<T>* out;       // [6][N][N] rows (ld_out, Nyquist plane at q_out)
+  int ld_in, ld_out;
+  size_t q_in, q_out;
   int nfields;      // 6 (full RHS) or 3 (velocity only)
   const cx<T>* tw;
 };
@@ -482,18 +502,18 @@ k_ns3_yinv(NS3Y<T> a) {
   const int kzv = kz;
   F::fill_tw(s_tw, a.tw, 1);
   __syncthreads();
-  const size_t fs = (size_t)N * N * a.ld;
-  const size_t base = (size_t)xi * N * a.ld + kz;
+  const ColAt li = ColAt::make<N>(a.ld_in, a.q_in, kz), lo = ColAt::make<N>(a.ld_out, a.q_out, kz);
+  const size_t row0 = (size_t)xi * N;
   auto load = [&](int f, cx<T>* v) {
 #pragma unroll
     for (int s = 0; s < F::NIN; ++s)
-      v[s] = (ok && F::in_valid(j, s)) ? a.in[f * fs + base + (size_t)F::in_index(j, s) * a.ld] : cx<T>{0, 0};
+      v[s] = (ok && F::in_valid(j, s)) ? a.in[li.at(f, row0 + F::in_index(j, s))] : cx<T>{0, 0};
   };
   auto store = [&](int f, const cx<T>* v) {
     if (!ok) return;
 #pragma unroll
     for (int s = 0; s < F::NOUT; ++s)
-      if (F::out_valid(j, s)) a.out[f * fs + base + (size_t)F::out_index(j, s) * a.ld] = v[s];
+      if (F::out_valid(j, s)) a.out[lo.at(f, row0 + F::out_index(j, s))] = v[s];
   };
   // loads run ahead of the transforms: each field is requested one or two
   // transforms before it is consumed (software pipelining)
@@ -553,11 +573,12 @@ k_ns3_yinv(NS3Y<T> a) {
 // ---------------------------------------------------------------------------
 template <typename T>
 struct NS3Z {
-  const cx<T>* in;   // [6][N][N][ld]
+  const cx<T>* in;   // [6][N][N][N/2] rows + [6][N][N] Nyquist plane at element q
   cx<T>* nz;         // [3][N][N][KZP]
   T* u_out;          // optional physical velocity [3][N][N][N]
   const T* u_in;     // optional physical velocity replacing the c2r of fields 0..2
-  int ld, KZD, KZP;
+  size_t q;
+  int KZD, KZP;
   const cx<T>* tw;
 };
 
@@ -635,7 +656,11 @@ struct ZState {
   using CI = PFFT<T, false, M, TP, Reverse<typename C::R>, ZP, SYNC_WAVE>;  // last pass: TP butterflies
   using CF = PFFT<T, true, M, TP, typename C::R, ZP, SYNC_WAVE>;           // pass 0 takes CI's output order
   static constexpr int NO = CI::NOUT;
-  static constexpr int NE = (PWV * Nh + WL - 1) / WL;  // prefetch elements per lane per field
+  // prefetch slots per lane per field: the wavefront's PWV rows of M
+  // contiguous entries, plus their Nyquist entries (one per pencil, taken by
+  // lanes 0..PWV-1 from the contiguous Nyquist plane)
+  static constexpr int NEM = (PWV * M + WL - 1) / WL;
+  static constexpr int NE = NEM + 1;
   // fields in flight: two in FP32; one in FP64, whose larger registers would
   // otherwise push the ring into private memory
   static constexpr int DEPTH = sizeof(T) == 4 ? 2 : 1;
@@ -659,28 +684,27 @@ struct ZState {
     row = row0 + p;
     ln = tid % WL;
     wrow = (tid / WL) * PWV;  // first pencil of this wavefront within the block
-    fs = (size_t)N * N * a.ld;
+    fs = (size_t)N * N * M;
   }
   __device__ __forceinline__ void fetch(int f, cx<T>* dst) {
-    const cx<T>* src = a.in + f * fs + (size_t)(row0 + wrow) * a.ld;
+    const cx<T>* src = a.in + f * fs + (size_t)(row0 + wrow) * M;
 #pragma unroll
-    for (int e = 0; e < NE; ++e) {
+    for (int e = 0; e < NEM; ++e) {
       const int idx = ln + e * WL;
-      if (idx < PWV * Nh) {
-        const int q = idx / Nh, k = idx - q * Nh;
-        dst[e] = src[(size_t)q * a.ld + k];
-      }
+      if (idx < PWV * M) dst[e] = src[idx];
     }
+    if (ln < PWV) dst[NEM] = a.in[a.q + (size_t)f * N * N + row0 + wrow + ln];
   }
   __device__ __forceinline__ void park(const cx<T>* src) {
 #pragma unroll
-    for (int e = 0; e < NE; ++e) {
+    for (int e = 0; e < NEM; ++e) {
       const int idx = ln + e * WL;
-      if (idx < PWV * Nh) {
-        const int q = idx / Nh, k = idx - q * Nh;
+      if (idx < PWV * M) {
+        const int q = idx / M, k = idx - q * M;
         s_buf[(wrow + q) * C::LSR + lds_at<ZP>(k)] = src[e];
       }
     }
+    if (ln < PWV) s_buf[(wrow + ln) * C::LSR + lds_at<ZP>(M)] = src[NEM];
   }
   template <int F>
   __device__ __forceinline__ cx<T>* slot() {
