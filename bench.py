@@ -13,9 +13,13 @@ Prints ONE JSON line.  --gpus above 1 is refused: this build runs on one GPU.
 """
 
 import argparse
+import csv
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -86,6 +90,53 @@ def run_gpu(args, precision, steps, warmup, profile):
     return sps, e_fin, prof, finite
 
 
+def _pmc_pass(counter, child_args, timeout):
+    """One rocprofv3 counter pass over a child run of this script; {kernel name: [value per dispatch]}."""
+    out = tempfile.mkdtemp(prefix="qlles_pmc_", dir="/tmp")
+    try:
+        cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "pmc", "--",
+               sys.executable, os.path.abspath(__file__)] + child_args
+        env = dict(os.environ, TMPDIR="/tmp")
+        subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                       timeout=timeout, check=False)
+        found = {}
+        for root, _, files in os.walk(out):
+            for f in files:
+                if f.endswith("counter_collection.csv"):
+                    for r in csv.DictReader(open(os.path.join(root, f))):
+                        if r["Counter_Name"] == counter:
+                            found.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        return found
+    finally:
+        shutil.rmtree(out, ignore_errors=True)
+
+
+def hbm_traffic(args, kernel_prefix):
+    """HBM bytes per launch of the x-pass from FETCH_SIZE and WRITE_SIZE (KiB per dispatch).
+
+    Each counter is calibrated on a plain 1 GiB device copy: bytes = KiB * 1024 * (true / counted).
+    """
+    if shutil.which("rocprofv3") is None:
+        return None, "rocprofv3 not on PATH"
+    mib = 1024
+    probe = ["--copy-probe-mib", str(mib)]
+    child = ["--pmc-child", "--N", str(args.N), "--steps", "3", "--warmup", "1"]
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        cal = [v for k, vs in _pmc_pass(counter, probe, 180).items() if "copyBuffer" in k or "copy" in k.lower()
+               for v in vs]
+        runs = _pmc_pass(counter, child, 300)
+        # RK-stage launches only: the state-load launch (stage -1) runs once per call
+        vals = [v for k, vs in runs.items() if kernel_prefix in k and ", -1," not in k for v in vs]
+        if not cal or not vals:
+            return None, f"{counter}: no samples"
+        factor = mib * 1024.0 / max(cal)  # true KiB per counted KiB on the copy kernel
+        res[counter] = dict(kib_per_launch=sum(vals) / len(vals), factor=factor, launches=len(vals))
+    total = sum(v["kib_per_launch"] * v["factor"] * 1024.0 for v in res.values())
+    log(f"[bench] traffic: {res}")
+    return total, res
+
+
 def cpu_baseline(args):
     """The FP64 NumPy/SciPy restatement (oracle) timed on this host's cores, bounded sample."""
     import numpy as np
@@ -134,7 +185,16 @@ def main():
     ap.add_argument("--cpu-max-steps", type=int, default=4)
     ap.add_argument("--copy-probe-mib", dest="copy_probe", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (counter calibration)")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH/WRITE passes")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+
+    if args.pmc_child:  # the FP32 solver workload alone, run under rocprofv3 by hbm_traffic()
+        import torch
+
+        torch.cuda.set_device(0)
+        run_gpu(args, "float32", args.steps, args.warmup, profile=False)
+        return
 
     if args.copy_probe:
         import torch
@@ -184,8 +244,10 @@ def main():
             kernels[k]["share_of_time"] = v["ms"] / tot
         dom = max(prof, key=lambda k: prof[k]["ms"])
         a = kernels[dom]["achieved_GBs"]
+        traffic, tinfo = (None, "skipped") if args.no_traffic else hbm_traffic(args, "k_ns3_x<")
         roof = dict(bound="hbm", kernel=dom, achieved=a, peak=HBM_PEAK_GBS, unit="GB/s", frac=a / HBM_PEAK_GBS,
-                    traffic=None,
+                    traffic=traffic, traffic_unit="bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+                    traffic_detail=tinfo,
                     alg_bytes_per_launch=kernels[dom]["alg_bytes_per_launch"], avg_us=kernels[dom]["avg_us"])
     out = {
         "metric": METRIC,
