@@ -36,6 +36,17 @@
 #ifndef QL_WPE_X
 #define QL_WPE_X 4
 #endif
+// z-pass: threads per workgroup, LDS padding (one slot per 2^QL_ZPAD
+// entries), fields in flight in FP32
+#ifndef QL_ZTHREADS
+#define QL_ZTHREADS 256
+#endif
+#ifndef QL_ZPAD
+#define QL_ZPAD 3
+#endif
+#ifndef QL_ZDEPTH32
+#define QL_ZDEPTH32 2
+#endif
 #ifndef QL_WPE_YI
 #define QL_WPE_YI 0
 #endif
@@ -585,8 +596,11 @@ struct NS3Z {
 // Radix plan (forward order) of the half-length FFTs in the z-pass; the c2r
 // runs it reversed so its last pass has exactly TP butterflies.
 template <int M> struct ZPlan { using R = typename Plan<M>::R; };
-#if QL_ZPLAN_WIDE
+#if QL_ZPLAN_WIDE == 1
 template <> struct ZPlan<128> { using R = Radices<4, 4, 8>; };
+template <> struct ZPlan<64> { using R = Radices<4, 4, 4>; };
+#elif QL_ZPLAN_WIDE == 2
+template <> struct ZPlan<128> { using R = Radices<4, 4, 4, 2>; };
 template <> struct ZPlan<64> { using R = Radices<4, 4, 4>; };
 #endif
 
@@ -595,10 +609,10 @@ struct ZCfg {
   static constexpr int M = N / 2;
   using R = typename ZPlan<M>::R;
   static constexpr int TP = M / R::at(0);  // lanes per pencil = last-pass butterflies of the c2r
-  static constexpr int P0 = 256 / TP;
+  static constexpr int P0 = QL_ZTHREADS / TP;
   static constexpr int P = (N <= 64) ? (P0 > 8 ? 8 : P0) : (P0 > 16 ? 16 : P0);
   static constexpr int THREADS = P * TP;
-  static constexpr int ZPAD = 3;  // one LDS padding slot per 8 elements of a pencil
+  static constexpr int ZPAD = QL_ZPAD;  // one LDS padding slot per 2^ZPAD elements of a pencil
   static constexpr int LSR = M + (M >> ZPAD) + 1 + LPAD;
 };
 
@@ -663,7 +677,7 @@ struct ZState {
   static constexpr int NE = NEM + 1;
   // fields in flight: two in FP32; one in FP64, whose larger registers would
   // otherwise push the ring into private memory
-  static constexpr int DEPTH = sizeof(T) == 4 ? 2 : 1;
+  static constexpr int DEPTH = sizeof(T) == 4 ? QL_ZDEPTH32 : 1;
   const NS3Z<T>& a;
   cx<T>* s_tw;   // flat N-point table (c2r / r2c pre- and post-processing)
   cx<T>* s_twi;  // per-plan table of the inverse half-length FFT
@@ -671,7 +685,7 @@ struct ZState {
   cx<T>* s_buf;
   int tid, p, j, row0, row, ln, wrow;
   size_t fs;
-  cx<T> pfa[NE], pfb[NE];
+  cx<T> pf[DEPTH][NE];  // prefetch ring, indexed by compile-time field numbers only
   cx<T> uu[3][NO];
   cx<T> nl[3][NO];
 
@@ -680,20 +694,25 @@ struct ZState {
     tid = threadIdx.x;
     p = tid / TP;
     j = tid % TP;
-    row0 = blockIdx.x * P;
-    row = row0 + p;
+    set_block(blockIdx.x);
     ln = tid % WL;
     wrow = (tid / WL) * PWV;  // first pencil of this wavefront within the block
     fs = (size_t)N * N * M;
   }
-  __device__ __forceinline__ void fetch(int f, cx<T>* dst) {
-    const cx<T>* src = a.in + f * fs + (size_t)(row0 + wrow) * M;
+  // the block's P pencils are rows b*P .. b*P+P-1 of the [N*N] pencil index
+  __device__ __forceinline__ void set_block(int b) {
+    row0 = b * P;
+    row = row0 + p;
+  }
+  // request field f of the pencils whose block starts at row r0
+  __device__ __forceinline__ void fetch(int f, cx<T>* dst, int r0) {
+    const cx<T>* src = a.in + f * fs + (size_t)(r0 + wrow) * M;
 #pragma unroll
     for (int e = 0; e < NEM; ++e) {
       const int idx = ln + e * WL;
       if (idx < PWV * M) dst[e] = src[idx];
     }
-    if (ln < PWV) dst[NEM] = a.in[a.q + (size_t)f * N * N + row0 + wrow + ln];
+    if (ln < PWV) dst[NEM] = a.in[a.q + (size_t)f * N * N + r0 + wrow + ln];
   }
   __device__ __forceinline__ void park(const cx<T>* src) {
 #pragma unroll
@@ -708,16 +727,15 @@ struct ZState {
   }
   template <int F>
   __device__ __forceinline__ cx<T>* slot() {
-    if constexpr (DEPTH == 1 || (F & 1) == 0) return pfa;
-    else return pfb;
+    return pf[F % DEPTH];
   }
-  // c2r of field F from its parked rows; field F+2 is requested meanwhile
+  // c2r of field F from its parked rows; field F+DEPTH is requested meanwhile
   template <int F>
   __device__ __forceinline__ void field() {
     cx<T>* pb = s_buf + p * C::LSR;
     lds_sync<SYNC_WAVE>();  // previous field's FFT scratch is free
     park(slot<F>());
-    if constexpr (F + DEPTH < 6) fetch(F + DEPTH, slot<F>());
+    if constexpr (F + DEPTH < 6) fetch(F + DEPTH, slot<F>(), row0);
     lds_sync<SYNC_WAVE>();
     cx<T> v[CI::MAXE];
     c2r_prep<T, M, CI, ZP>(v, pb, s_tw, j);
@@ -786,8 +804,9 @@ __device__ __forceinline__ void ns3_z_body(const NS3Z<T>& a, cx<T>* s_tw, cx<T>*
       for (int s = 0; s < NO; ++s) z.uu[f][s] = src[CI::out_index(z.j, s)];
     }
   }
-  z.fetch(f0, z.template slot<f0>());
-  if constexpr (St::DEPTH == 2) z.fetch(f0 + 1, z.template slot<f0 + 1>());
+  z.fetch(f0, z.template slot<f0>(), z.row0);
+  if constexpr (St::DEPTH >= 2) z.fetch(f0 + 1, z.template slot<f0 + 1>(), z.row0);
+  if constexpr (St::DEPTH >= 3) z.fetch(f0 + 2, z.template slot<f0 + 2>(), z.row0);
   z.template fields<f0>();
   if (a.u_out != nullptr) {
 #pragma unroll
