@@ -162,6 +162,38 @@ int qlles_interp_field(qlles_interp* it, const void* u, const double* r, int np,
 int qlles_interp_hat(qlles_interp* it, int N, const void* u_hat, double* r, int np, void* u_r, double dt,
                      void* stream);
 
+/* ---- batched trajectories ------------------------------------------------
+ * Replaces the reference's one-trajectory-per-process SLURM arrays
+ * (slurm_*.sh:5; each array task runs main.py -> simulate / combined on its
+ * own GPU) for grids too small to fill an MI355X.  A batch holds members
+ * (plan, optional interpolator + particles, state arrays); qlles_batch_run
+ * advances member i by nsteps[i] steps (uneven lengths allowed), exactly as
+ *   qlles_step_advect(plan, it, NULL, u_hat, nsteps[i], r, np, u_r, NULL, s)
+ * (or qlles_step without particles) would alone: the same kernels in the
+ * same order with fixed-order reductions, so every trajectory is
+ * bit-identical to its solo run.  Whole chunks of steps replay a HIP graph
+ * captured once per member; members are spread over `lanes` streams.
+ * Members need their own plan and interpolator.  Not thread-safe. */
+typedef struct qlles_batch qlles_batch;
+int qlles_batch_create(qlles_batch** out);
+int qlles_batch_destroy(qlles_batch* b);
+/* r, u_r, np as in qlles_step_advect; it == NULL: no particles.  *index
+ * (optional) receives the member's position. */
+int qlles_batch_add(qlles_batch* b, qlles_plan* plan, qlles_interp* it, void* u_hat, double* r, int np, void* u_r,
+                    int* index);
+/* manual configuration: 1 <= lanes <= 16 streams, 1 <= chunk <= 1000 steps per graph */
+int qlles_batch_configure(qlles_batch* b, int lanes, int chunk);
+/* autotuner: times lanes in {1,2,4,8} x chunk in {1,4,16} over about
+ * trial_steps steps per member on scratch copies of the member states (the
+ * trajectories do not move) and keeps the fastest configuration */
+int qlles_batch_tune(qlles_batch* b, int trial_steps, void* stream);
+/* current configuration and the tuned time per trajectory-step (0 if untuned) */
+int qlles_batch_get_config(const qlles_batch* b, int* lanes, int* chunk, double* us_per_traj_step);
+/* nsteps: per-member step counts (NULL: every member takes nsteps_all).
+ * Stream-ordered with respect to `stream` (work forks onto the batch's
+ * streams and joins back). */
+int qlles_batch_run(qlles_batch* b, const int* nsteps, int nsteps_all, void* stream);
+
 /* message of the last failed call on this thread */
 const char* qlles_last_error(void);
 

@@ -1,0 +1,290 @@
+// Batched trajectories (many independent plans advanced together on one GPU).
+//
+// The reference runs one trajectory per process (a SLURM array, one GPU
+// each).  Small grids leave an MI355X mostly idle and launch-bound, so a batch
+// captures each member's chunk of steps into a HIP graph once and replays the
+// graphs on a few streams.  A member's kernels and their arguments are those
+// of qlles_step_advect / qlles_step on the same plan, and every reduction in
+// them runs in a fixed order, so each trajectory is bit-identical to its solo
+// run whatever the stream count, chunk length or interleaving.
+#include <algorithm>
+#include <chrono>
+#include <vector>
+
+#include "capi_common.hpp"
+
+using qlapi::fail;
+using qlapi::guarded;
+
+struct qlles_batch {
+  struct Member {
+    qlles_plan* p;
+    qlles_interp* it;
+    void* uh;
+    double* r;
+    int np;
+    void* ur;
+    size_t uh_bytes, r_bytes, ur_bytes;
+    hipGraphExec_t g = nullptr;  // `chunk` steps on this member's arrays
+  };
+  std::vector<Member> m;
+  int lanes = 1;  // streams the members are spread over (member i -> stream i % lanes)
+  int chunk = 1;  // steps per graph launch
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> join;
+  hipEvent_t fork = nullptr;
+  bool built = false;
+  double tuned_us_per_traj_step = 0.0;
+};
+
+namespace {
+
+void drop_graphs(qlles_batch* b) {
+  for (auto& x : b->m)
+    if (x.g) {
+      (void)hipGraphExecDestroy(x.g);
+      x.g = nullptr;
+    }
+  b->built = false;
+}
+
+void drop_streams(qlles_batch* b) {
+  for (hipStream_t s : b->streams) (void)hipStreamDestroy(s);
+  for (hipEvent_t e : b->join) (void)hipEventDestroy(e);
+  b->streams.clear();
+  b->join.clear();
+}
+
+void make_streams(qlles_batch* b) {
+  if ((int)b->streams.size() == b->lanes) return;
+  drop_streams(b);
+  for (int l = 0; l < b->lanes; ++l) {
+    hipStream_t s = nullptr;
+    QL_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));  // default priority
+    b->streams.push_back(s);
+    hipEvent_t e = nullptr;
+    QL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    b->join.push_back(e);
+  }
+  if (!b->fork) QL_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+}
+
+// capture `steps` steps of member x (on arrays uh/r/ur) into an executable graph
+hipGraphExec_t capture(const qlles_batch::Member& x, void* uh, double* r, void* ur, int steps, hipStream_t s) {
+  if (x.p->eng->profiling()) throw std::invalid_argument("disable qlles_profile on batch members");
+  QL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  hipGraph_t graph = nullptr;
+  try {
+    qlapi::run_steps(x.p, x.it, nullptr, uh, steps, r, x.np, ur, nullptr, s);
+  } catch (...) {
+    (void)hipStreamEndCapture(s, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  QL_HIP(hipStreamEndCapture(s, &graph));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  QL_HIP(e);
+  return exec;
+}
+
+void build(qlles_batch* b) {
+  if (b->built) return;
+  make_streams(b);
+  drop_graphs(b);
+  for (size_t i = 0; i < b->m.size(); ++i) {
+    auto& x = b->m[i];
+    x.g = capture(x, x.uh, x.r, x.ur, b->chunk, b->streams[i % b->lanes]);
+  }
+  b->built = true;
+}
+
+// advance member i by n_i steps: whole chunks by graph replay, the rest eagerly
+void run(qlles_batch* b, const std::vector<int>& n, hipStream_t caller) {
+  build(b);
+  QL_HIP(hipEventRecord(b->fork, caller));
+  for (hipStream_t s : b->streams) QL_HIP(hipStreamWaitEvent(s, b->fork, 0));
+  std::vector<int> left(n);
+  // round-robin over members so that every stream always has queued work
+  for (bool any = true; any;) {
+    any = false;
+    for (size_t i = 0; i < b->m.size(); ++i) {
+      if (left[i] >= b->chunk) {
+        QL_HIP(hipGraphLaunch(b->m[i].g, b->streams[i % b->lanes]));
+        left[i] -= b->chunk;
+        any = true;
+      }
+    }
+  }
+  for (size_t i = 0; i < b->m.size(); ++i) {
+    if (left[i] > 0) {
+      const auto& x = b->m[i];
+      qlapi::run_steps(x.p, x.it, nullptr, x.uh, left[i], x.r, x.np, x.ur, nullptr, b->streams[i % b->lanes]);
+    }
+  }
+  for (int l = 0; l < b->lanes; ++l) {
+    QL_HIP(hipEventRecord(b->join[l], b->streams[l]));
+    QL_HIP(hipStreamWaitEvent(caller, b->join[l], 0));
+  }
+}
+
+size_t elem_bytes(const qlles_plan* p) { return p->desc.dtype == QLLES_F32 ? 4 : 8; }
+
+}  // namespace
+
+extern "C" {
+
+int qlles_batch_create(qlles_batch** out) {
+  if (!out) return fail(QLLES_ERR_ARG, "null argument");
+  *out = nullptr;
+  return guarded([&] { *out = new qlles_batch(); });
+}
+
+int qlles_batch_destroy(qlles_batch* b) {
+  if (!b) return QLLES_OK;
+  return guarded([&] {
+    drop_graphs(b);
+    drop_streams(b);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    delete b;
+  });
+}
+
+int qlles_batch_add(qlles_batch* b, qlles_plan* p, qlles_interp* it, void* u_hat, double* r, int np, void* u_r,
+                    int* index) {
+  if (!b || !p || !u_hat) return fail(QLLES_ERR_ARG, "null argument");
+  if (it && (!r || !u_r || np < 0)) return fail(QLLES_ERR_ARG, "particles need r, u_r and np >= 0");
+  if (it && (ql::interp_dim(it) != p->desc.dim || ql::interp_dtype(it) != p->desc.dtype ||
+             ql::interp_n(it) > p->desc.N))
+    return fail(QLLES_ERR_ARG, "interpolator does not match the plan (dim, dtype, n <= N)");
+  for (const auto& x : b->m) {
+    if (x.p == p) return fail(QLLES_ERR_ARG, "each batch member needs its own plan (plans hold workspace)");
+    if (it && x.it == it) return fail(QLLES_ERR_ARG, "each batch member needs its own interpolator");
+  }
+  return guarded([&] {
+    const int N = p->desc.N, dim = p->desc.dim;
+    const size_t rows = dim == 3 ? (size_t)N * N : (size_t)N;
+    qlles_batch::Member x{p, it, u_hat, it ? r : nullptr, it ? np : 0, it ? u_r : nullptr,
+                          3 * rows * (N / 2 + 1) * 2 * elem_bytes(p),
+                          it ? (size_t)np * dim * sizeof(double) : 0, it ? (size_t)np * dim * elem_bytes(p) : 0};
+    b->m.push_back(x);
+    drop_graphs(b);
+    if (index) *index = (int)b->m.size() - 1;
+  });
+}
+
+int qlles_batch_configure(qlles_batch* b, int lanes, int chunk) {
+  if (!b) return fail(QLLES_ERR_ARG, "null batch");
+  if (lanes < 1 || lanes > 16 || chunk < 1 || chunk > 1000) return fail(QLLES_ERR_ARG, "need 1 <= lanes <= 16, 1 <= chunk <= 1000");
+  return guarded([&] {
+    drop_graphs(b);
+    b->lanes = lanes;
+    b->chunk = chunk;
+  });
+}
+
+int qlles_batch_get_config(const qlles_batch* b, int* lanes, int* chunk, double* us_per_traj_step) {
+  if (!b) return fail(QLLES_ERR_ARG, "null batch");
+  if (lanes) *lanes = b->lanes;
+  if (chunk) *chunk = b->chunk;
+  if (us_per_traj_step) *us_per_traj_step = b->tuned_us_per_traj_step;
+  return QLLES_OK;
+}
+
+int qlles_batch_run(qlles_batch* b, const int* nsteps, int nsteps_all, void* stream) {
+  if (!b) return fail(QLLES_ERR_ARG, "null batch");
+  if (!nsteps && nsteps_all < 0) return fail(QLLES_ERR_ARG, "step counts must be >= 0");
+  std::vector<int> n(b->m.size(), nsteps_all);
+  if (nsteps)
+    for (size_t i = 0; i < n.size(); ++i) n[i] = nsteps[i];
+  for (int v : n)
+    if (v < 0) return fail(QLLES_ERR_ARG, "step counts must be >= 0");
+  if (b->m.empty()) return QLLES_OK;
+  return guarded([&] { run(b, n, (hipStream_t)stream); });
+}
+
+int qlles_batch_tune(qlles_batch* b, int trial_steps, void* stream) {
+  if (!b) return fail(QLLES_ERR_ARG, "null batch");
+  if (trial_steps < 1) return fail(QLLES_ERR_ARG, "trial_steps must be >= 1");
+  if (b->m.empty()) return QLLES_OK;
+  const hipStream_t caller = (hipStream_t)stream;
+  std::vector<void*> scratch;
+  int rc = guarded([&] {
+    // trials run on copies of the member states, so tuning leaves every
+    // trajectory where it was
+    qlles_batch t;
+    for (const auto& x : b->m) {
+      qlles_batch::Member y = x;
+      y.g = nullptr;
+      void* p = nullptr;
+      QL_HIP(hipMalloc(&p, x.uh_bytes));
+      scratch.push_back(p);
+      QL_HIP(hipMemcpyAsync(p, x.uh, x.uh_bytes, hipMemcpyDeviceToDevice, caller));
+      y.uh = p;
+      if (x.it) {
+        QL_HIP(hipMalloc(&p, x.r_bytes));
+        scratch.push_back(p);
+        QL_HIP(hipMemcpyAsync(p, x.r, x.r_bytes, hipMemcpyDeviceToDevice, caller));
+        y.r = static_cast<double*>(p);
+        QL_HIP(hipMalloc(&p, std::max<size_t>(x.ur_bytes, 1)));
+        scratch.push_back(p);
+        y.ur = p;
+      }
+      t.m.push_back(y);
+    }
+    QL_HIP(hipStreamSynchronize(caller));
+    const int nm = (int)b->m.size();
+    double best = -1.0;
+    int best_l = 1, best_c = 1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    QL_HIP(hipEventCreate(&e0));
+    QL_HIP(hipEventCreate(&e1));
+    try {
+      for (int lanes : {1, 2, 4, 8}) {
+        if (lanes > nm && lanes > 1) break;
+        for (int chunk : {1, 4, 16}) {
+          if (chunk > trial_steps && chunk > 1) break;
+          t.lanes = lanes;
+          t.chunk = chunk;
+          drop_graphs(&t);
+          const int n = std::max(chunk, trial_steps / chunk * chunk);
+          run(&t, std::vector<int>(nm, chunk), caller);  // warm-up (also builds the graphs)
+          QL_HIP(hipEventRecord(e0, caller));
+          run(&t, std::vector<int>(nm, n), caller);
+          QL_HIP(hipEventRecord(e1, caller));
+          QL_HIP(hipEventSynchronize(e1));
+          float ms = 0.f;
+          QL_HIP(hipEventElapsedTime(&ms, e0, e1));
+          const double us = 1e3 * ms / ((double)n * nm);
+          if (best < 0 || us < best) {
+            best = us;
+            best_l = lanes;
+            best_c = chunk;
+          }
+        }
+      }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      drop_graphs(&t);
+      drop_streams(&t);
+      if (t.fork) (void)hipEventDestroy(t.fork);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    drop_graphs(&t);
+    drop_streams(&t);
+    if (t.fork) (void)hipEventDestroy(t.fork);
+    drop_graphs(b);
+    b->lanes = best_l;
+    b->chunk = best_c;
+    b->tuned_us_per_traj_step = best;
+  });
+  (void)hipStreamSynchronize(caller);
+  for (void* p : scratch) (void)hipFree(p);
+  return rc;
+}
+
+}  // extern "C"

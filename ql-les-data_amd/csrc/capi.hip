@@ -1,43 +1,12 @@
 // C ABI (include/qlles.h): argument validation, error codes, thread-local
 // error message.  No C++ exception crosses this boundary.
 #include <cstring>
-#include <new>
-#include <stdexcept>
-#include <string>
 
-#include "../../include/qlles.h"
-#include "engine.hpp"
-#include "fftlib.hpp"
-#include "interp.hpp"
-
-struct qlles_plan {
-  ql::Desc desc;
-  ql::Engine* eng;
-};
+#include "capi_common.hpp"
 
 namespace {
-thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-  g_err = msg;
-  return code;
-}
-
-template <class F>
-int guarded(F&& f) {
-  try {
-    f();
-    return QLLES_OK;
-  } catch (const ql::HipError& e) {
-    return fail(QLLES_ERR_HIP, e.what());
-  } catch (const std::invalid_argument& e) {
-    return fail(QLLES_ERR_ARG, e.what());
-  } catch (const std::bad_alloc&) {
-    return fail(QLLES_ERR_HIP, "host allocation failed");
-  } catch (const std::exception& e) {
-    return fail(QLLES_ERR_ARG, e.what());
-  }
-}
+using qlapi::fail;
+using qlapi::guarded;
 
 bool grid_ok(int dim, int N) {
   if (dim == 3) return N == 8 || N == 16 || N == 32 || N == 64 || N == 96 || N == 128 || N == 256;
@@ -48,7 +17,7 @@ bool grid_ok(int dim, int N) {
 
 extern "C" {
 
-const char* qlles_last_error(void) { return g_err.c_str(); }
+const char* qlles_last_error(void) { return qlapi::g_err.c_str(); }
 
 int qlles_plan_create(const qlles_desc* d, qlles_plan** out) {
   if (!d || !out) return fail(QLLES_ERR_ARG, "null argument");
@@ -106,16 +75,7 @@ int qlles_step_advect(qlles_plan* p, qlles_interp* it, void* u_out, void* u_hat,
   if (ql::interp_dim(it) != p->desc.dim || ql::interp_dtype(it) != p->desc.dtype || ql::interp_n(it) > p->desc.N)
     return fail(QLLES_ERR_ARG, "interpolator does not match the plan (dim, dtype, n <= N)");
   if (nsteps == 0) return QLLES_OK;
-  const int N = p->desc.N;
-  const double dt = p->desc.dt;
-  int rc = guarded([&] {
-    p->eng->after_step = [=](hipStream_t s, const void* uh, int ld, size_t q) {
-      ql::interp_hat(it, N, uh, ld, q, r, np, u_r, dt, s);
-    };
-    p->eng->step(u_out, u_hat, nullptr, nsteps, e_inj, (hipStream_t)stream);
-  });
-  p->eng->after_step = nullptr;
-  return rc;
+  return guarded([&] { qlapi::run_steps(p, it, u_out, u_hat, nsteps, r, np, u_r, e_inj, (hipStream_t)stream); });
 }
 
 int qlles_rhs(qlles_plan* p, const void* u, const void* u_hat, int rk, void* du_hat, void* stream) {

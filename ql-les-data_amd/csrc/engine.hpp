@@ -36,6 +36,7 @@ class Engine {
   // algorithmic bytes each launch must move (reads + writes of its arrays)
   enum Kind { K_XPASS = 0, K_YINV = 1, K_ZPASS = 2, K_YFWD = 3, K_OTHER = 4, NKIND = 5 };
   void profile_enable(bool on);
+  bool profiling() const { return prof_; }
   // called on the step's stream after every completed step (particle advection)
   // (stream, spectrum of u_{n+1}, its row stride in complex elements, offset
   // of its separate Nyquist plane or 0 when kz = N/2 sits in the rows)

@@ -312,6 +312,9 @@ void setup_t(qlles_interp* it) {
   }
   const size_t nrows = d == 3 ? (size_t)n * n : (size_t)n;
   QL_HIP(hipMalloc(&it->tmp, sizeof(cx<T>) * nrows * (n / 2 + 1) * d));
+  // twiddle tables are built now, so no later call allocates (graph capture)
+  (void)twiddles<T>(n);
+  if (it->backend == QLLES_INTERP_NUFFT) (void)twiddles<T>(it->nf);
 }
 
 template <typename T>

@@ -10,3 +10,4 @@ from . import _lib  # noqa: F401
 from .spectral_solver import (comp_dt, custom_jit_integrate, rhs_wrapper, rk4_wrapper,  # noqa: F401
                               set_up_solver, simulate)
 from .integrator import shift_fn, spectral_interpolator_wrapper  # noqa: F401
+from .batch import Batch  # noqa: F401

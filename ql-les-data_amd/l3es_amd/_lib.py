@@ -30,7 +30,8 @@ EXPORTS = (
     "qlles_rhs", "qlles_rfftn", "qlles_irfftn", "qlles_diagnostics", "qlles_dissipation",
     "qlles_energy_hat", "qlles_filter", "qlles_interp_create", "qlles_interp_destroy",
     "qlles_interp_field", "qlles_interp_hat", "qlles_last_error", "qlles_profile", "qlles_profile_read",
-    "qlles_step_advect",
+    "qlles_step_advect", "qlles_batch_create", "qlles_batch_destroy", "qlles_batch_add",
+    "qlles_batch_configure", "qlles_batch_tune", "qlles_batch_get_config", "qlles_batch_run",
 )
 KINDS = ("x-pass", "y-inverse", "z-pass", "y-forward", "other")
 
@@ -86,6 +87,13 @@ def load():
         "qlles_profile": (ci, [vp, ci]),
         "qlles_step_advect": (ci, [vp, vp, vp, vp, ci, vp, ci, vp, vp, vp]),
         "qlles_profile_read": (ci, [vp, vp, vp, vp]),
+        "qlles_batch_create": (ci, [ctypes.POINTER(vp)]),
+        "qlles_batch_destroy": (ci, [vp]),
+        "qlles_batch_add": (ci, [vp, vp, vp, vp, vp, ci, vp, ctypes.POINTER(ci)]),
+        "qlles_batch_configure": (ci, [vp, ci, ci]),
+        "qlles_batch_tune": (ci, [vp, ci, vp]),
+        "qlles_batch_get_config": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), pd]),
+        "qlles_batch_run": (ci, [vp, vp, ci, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -1,0 +1,72 @@
+"""Batched trajectories on one GPU (qlles_batch_* in include/qlles.h).
+
+The reference advances one trajectory per process and GPU (SLURM arrays,
+``slurm_*.sh:5``).  ``Batch`` advances many independent trajectories -- each
+with its own plan, state and optional particles -- together: whole chunks of
+steps replay a HIP graph captured once per member, spread over a few streams.
+Each trajectory is bit-identical to the same steps taken alone with
+``_Step.advance`` / ``qlles_step_advect``.
+"""
+
+import ctypes
+
+import torch
+
+from . import _lib
+
+
+class Batch:
+    def __init__(self):
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.qlles_batch_create(ctypes.byref(h)))
+        self.handle = h
+        self.members = []  # keeps the tensors and plans alive
+
+    def add(self, step, u_hat, interp=None, r=None, u_r=None):
+        """Add a trajectory: ``step`` from set_up_solver / rk4_wrapper, its state ``u_hat`` (updated in
+        place), optionally an interpolator with positions ``r`` (np, dim) FP64 and a (np, dim) output."""
+        np_ = 0
+        if interp is not None:
+            if r is None:
+                raise ValueError("particles need positions r")
+            np_ = int(r.shape[0])
+            if u_r is None:
+                u_r = torch.empty((np_, interp.dim), dtype=interp.dtype, device=r.device)
+        idx = ctypes.c_int(-1)
+        _lib.check(self._lib.qlles_batch_add(self.handle, step.plan.handle,
+                                             interp.it.handle if interp is not None else None,
+                                             _lib.ptr(u_hat), _lib.ptr(r) if interp is not None else None, np_,
+                                             _lib.ptr(u_r) if interp is not None else None, ctypes.byref(idx)))
+        self.members.append((step, u_hat, interp, r, u_r))
+        return idx.value
+
+    def configure(self, lanes, chunk):
+        _lib.check(self._lib.qlles_batch_configure(self.handle, int(lanes), int(chunk)))
+
+    def tune(self, trial_steps=16):
+        """Pick streams x steps-per-graph by timing on scratch copies (trajectories do not move)."""
+        _lib.check(self._lib.qlles_batch_tune(self.handle, int(trial_steps), _lib.stream_ptr()))
+        return self.config()
+
+    def config(self):
+        lanes, chunk, us = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _lib.check(self._lib.qlles_batch_get_config(self.handle, ctypes.byref(lanes), ctypes.byref(chunk),
+                                                    ctypes.byref(us)))
+        return dict(lanes=lanes.value, chunk=chunk.value, us_per_traj_step=us.value)
+
+    def run(self, nsteps):
+        """Advance every member: ``nsteps`` an int (all the same) or a list (uneven lengths)."""
+        if isinstance(nsteps, int):
+            _lib.check(self._lib.qlles_batch_run(self.handle, None, nsteps, _lib.stream_ptr()))
+        else:
+            arr = (ctypes.c_int * len(nsteps))(*[int(v) for v in nsteps])
+            _lib.check(self._lib.qlles_batch_run(self.handle, arr, 0, _lib.stream_ptr()))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                self._lib.qlles_batch_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass

@@ -1,0 +1,113 @@
+"""Batched trajectories (qlles_batch_*): every member bit-identical to its solo run.
+
+No reference counterpart: the reference runs one trajectory per process (slurm_*.sh:5).  The pin is
+exact equality with the same steps taken alone through qlles_step_advect / qlles_step.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+
+
+def has_lib():
+    from l3es_amd import _lib
+
+    import os
+    return os.path.exists(_lib.LIB_PATH)
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.mark.skipif(not has_lib(), reason="libqlles.so not built")
+def test_batch_validation_without_device():
+    from l3es_amd import _lib
+
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    assert lib.qlles_batch_create(ctypes.byref(h)) == 0
+    try:
+        assert lib.qlles_batch_add(h, None, None, None, None, 0, None, None) == _lib.QLLES_ERR_ARG
+        assert lib.qlles_batch_configure(h, 0, 1) == _lib.QLLES_ERR_ARG
+        assert lib.qlles_batch_configure(h, 2, 0) == _lib.QLLES_ERR_ARG
+        assert lib.qlles_batch_configure(h, 2, 4) == 0
+        lanes, chunk, us = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        assert lib.qlles_batch_get_config(h, ctypes.byref(lanes), ctypes.byref(chunk), ctypes.byref(us)) == 0
+        assert (lanes.value, chunk.value) == (2, 4)
+        assert lib.qlles_batch_run(h, None, -3, None) == _lib.QLLES_ERR_ARG
+        assert lib.qlles_batch_run(h, None, 5, None) == 0  # empty batch: nothing to do
+    finally:
+        assert lib.qlles_batch_destroy(h) == 0
+
+
+def _members():
+    from l3es_amd import set_up_solver
+    from l3es_amd.integrator import pos_init_cartesian, spectral_interpolator_wrapper
+
+    out = []
+    specs = [
+        # (N, dim, case, precision, kwargs, particle backend, n)
+        (32, 3, "HIT", "float32", dict(forcing_type="power", power=0.1, kf=2), "nufft", 16),
+        (16, 3, "TGV3D", "float64", dict(forcing_type="none", kf=None), None, 0),
+        (64, 2, "Kolm", "float32", dict(forcing_type="kolmogorov", drag=0.1, kolm_amp=1.0, kolm_k=4, kf=None),
+         "nufft", 32),
+        (32, 2, "TGV2D", "float64", dict(forcing_type="none", kf=None), "dft", 16),
+    ]
+    for i, (N, dim, case, prec, kw, be, n) in enumerate(specs):
+        kf = kw.pop("kf")
+        _, uh, _, _, step, _, _, _, _ = set_up_solver(N, 2e-3, dim, case, 2e-3, 10 + i, max(n, 8), kf,
+                                                      precision=prec, warmup=False, **kw)
+        it = r = None
+        if be is not None:
+            it = spectral_interpolator_wrapper(n, tuple(range(1, dim + 1)), backend=be, precision=prec)
+            r = torch.from_numpy(pos_init_cartesian(n, dim)).cuda()
+        out.append((step, uh, it, r))
+    return out
+
+
+def _solo(step, uh, it, r, nsteps):
+    from l3es_amd import _lib
+
+    lib = _lib.load()
+    if it is None:
+        step.advance(None, uh, nsteps, None, use_u=False)
+        return None
+    ur = torch.empty((r.shape[0], it.dim), dtype=it.dtype, device=r.device)
+    _lib.check(lib.qlles_step_advect(step.plan.handle, it.it.handle, None, _lib.ptr(uh), int(nsteps), _lib.ptr(r),
+                                     r.shape[0], _lib.ptr(ur), None, _lib.stream_ptr()))
+    return ur
+
+
+@pytest.mark.gpu
+def test_batch_bit_identical_to_solo(gpu):
+    from l3es_amd import Batch
+
+    mem = _members()
+    solo = [(uh.clone(), None if r is None else r.clone()) for (_, uh, _, r) in mem]
+    bat = [(uh.clone(), None if r is None else r.clone()) for (_, uh, _, r) in mem]
+    b = Batch()
+    urs = []
+    for (step, _, it, _), (uh, r) in zip(mem, bat):
+        ur = None if it is None else torch.empty((r.shape[0], it.dim), dtype=it.dtype, device=r.device)
+        urs.append(ur)
+        b.add(step, uh, it, r, ur)
+    lengths = [7, 5, 9, 3]  # uneven trajectories; chunk 2 leaves eager remainders
+    b.configure(lanes=2, chunk=2)
+    b.run(lengths)
+    cfg = b.tune(trial_steps=4)  # runs on scratch copies: states must not move
+    assert cfg["lanes"] in (1, 2, 4, 8) and cfg["chunk"] in (1, 4, 16) and cfg["us_per_traj_step"] > 0
+    b.run(5)
+    torch.cuda.synchronize()
+    for k, ((step, _, it, _), (uh_s, r_s), (uh_b, r_b), ur_b) in enumerate(zip(mem, solo, bat, urs)):
+        ur1 = _solo(step, uh_s, it, r_s, lengths[k])
+        ur2 = _solo(step, uh_s, it, r_s, 5)
+        torch.cuda.synchronize()
+        assert np.array_equal(host(uh_s), host(uh_b)), f"member {k}: spectrum differs from solo run"
+        if it is not None:
+            assert np.array_equal(host(r_s), host(r_b)), f"member {k}: positions differ"
+            assert np.array_equal(host(ur2), host(ur_b)), f"member {k}: particle velocities differ"
+        del ur1
