@@ -137,6 +137,83 @@ def hbm_traffic(args, kernel_prefix):
     return total, res
 
 
+BATCH_SIZES = [(2, 64), (2, 128), (2, 256), (2, 512), (3, 64), (3, 96), (3, 128)]
+
+
+def run_batch(args):
+    """24 uneven trajectories (2-D 64^2..512^2 Kolmogorov, 3-D 64^3..128^3 forced HIT, each with particles),
+    advanced together by qlles_batch (graph replay + autotuned streams) vs one after another (eager solo)."""
+    import torch
+
+    from l3es_amd import Batch, set_up_solver
+    from l3es_amd.integrator import pos_init_cartesian, spectral_interpolator_wrapper
+
+    torch.cuda.set_device(0)
+    mem = []
+    t0 = time.perf_counter()
+    for i in range(args.batch_members):
+        dim, N = BATCH_SIZES[i % len(BATCH_SIZES)]
+        if dim == 2:
+            n = 32
+            _, uh, _, _, step, _, _, _, _ = set_up_solver(N, 1e-3, 2, "Kolm", 1e-3, 100 + i, n, None,
+                                                          forcing_type="kolmogorov", drag=0.1, kolm_amp=1.0,
+                                                          kolm_k=4, precision="float32", warmup=False)
+        else:
+            n = 16
+            _, uh, _, _, step, _, _, _, _ = set_up_solver(N, 1e-3, 3, "HIT", 5e-4, 100 + i, n, 2,
+                                                          e_kin_target=0.5, forcing_type="power", power=0.1,
+                                                          spectrum="k4exp", precision="float32", warmup=False)
+        it = spectral_interpolator_wrapper(n, tuple(range(1, dim + 1)), backend="nufft", precision="float32")
+        r = torch.from_numpy(pos_init_cartesian(n, dim)).cuda()
+        mem.append((dim, N, step, uh, it, r))
+    torch.cuda.synchronize()
+    log(f"[bench] batch: {len(mem)} members set up in {time.perf_counter() - t0:.1f}s")
+    # uneven lengths: member i takes K + (i % 4) * K / 4 steps
+    K = args.steps
+    lengths = [K + (i % 4) * K // 4 for i in range(len(mem))]
+    total = sum(lengths)
+    # solo: one trajectory after the other, eager launches (qlles_step_advect on the caller stream)
+    from l3es_amd import _lib
+
+    lib = _lib.load()
+    copies = [(uh.clone(), r.clone()) for (_, _, _, uh, _, r) in mem]
+    urs = [torch.empty((r.shape[0], it.dim), dtype=it.dtype, device=r.device) for (_, _, _, _, it, r) in mem]
+
+    def solo(nsteps_of):
+        for (dim, N, step, _, it, _), (uh, r), ur, k in zip(mem, copies, urs, nsteps_of):
+            _lib.check(lib.qlles_step_advect(step.plan.handle, it.it.handle, None, _lib.ptr(uh), int(k), _lib.ptr(r),
+                                             r.shape[0], _lib.ptr(ur), None, _lib.stream_ptr()))
+
+    solo([args.warmup] * len(mem))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solo(lengths)
+    torch.cuda.synchronize()
+    t_solo = time.perf_counter() - t0
+    b = Batch()
+    for (dim, N, step, uh, it, r) in mem:
+        b.add(step, uh, it, r)
+    cfg = b.tune(trial_steps=64)
+    b.run(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.run(lengths)
+    torch.cuda.synchronize()
+    t_b = time.perf_counter() - t0
+    log(f"[bench] batch: tuned {cfg}; {total} trajectory-steps: batched {t_b:.3f}s, solo {t_solo:.3f}s")
+    out = {
+        "metric": "trajectory-steps/s, 24 uneven trajectories batched on one GPU",
+        "value": total / t_b, "unit": "trajectory-steps/s", "n_gpus": 1, "steps": total, "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_b / total, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "24 trajectories: 2-D Kolmogorov 64^2-512^2 (32^2 particles) and 3-D power-forced "
+                               "HIT 64^3-128^3 (16^3 particles), uneven lengths K..1.75K, NUFFT tol 1e-6",
+                   "sizes": [f"{d}D {N}" for (d, N, *_rest) in mem], "lengths": lengths},
+        "batched_tuned": cfg, "solo_sequential_traj_steps_per_s": total / t_solo, "speedup_vs_solo": t_solo / t_b,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(args):
     """The FP64 NumPy/SciPy restatement (oracle) timed on this host's cores, bounded sample."""
     import numpy as np
@@ -186,6 +263,9 @@ def main():
     ap.add_argument("--copy-probe-mib", dest="copy_probe", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (counter calibration)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH/WRITE passes")
+    ap.add_argument("--workload", choices=("hit256", "batch"), default="hit256",
+                    help="hit256: the headline 256^3 step (default); batch: 24 uneven batched trajectories")
+    ap.add_argument("--batch-members", dest="batch_members", type=int, default=24)
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
 
@@ -212,6 +292,10 @@ def main():
         print("bench.py runs on one GPU only (the solver grid fits one MI355X; "
               "multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch":
+        run_batch(args)
+        return
 
     import torch
 

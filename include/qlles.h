@@ -183,7 +183,7 @@ int qlles_batch_add(qlles_batch* b, qlles_plan* plan, qlles_interp* it, void* u_
                     int* index);
 /* manual configuration: 1 <= lanes <= 16 streams, 1 <= chunk <= 1000 steps per graph */
 int qlles_batch_configure(qlles_batch* b, int lanes, int chunk);
-/* autotuner: times lanes in {1,2,4,8} x chunk in {1,4,16} over about
+/* autotuner: times lanes in {1,2,4,8,16} x chunk in {1,4,16,64} over about
  * trial_steps steps per member on scratch copies of the member states (the
  * trajectories do not move) and keeps the fastest configuration */
 int qlles_batch_tune(qlles_batch* b, int trial_steps, void* stream);

@@ -241,9 +241,9 @@ int qlles_batch_tune(qlles_batch* b, int trial_steps, void* stream) {
     QL_HIP(hipEventCreate(&e0));
     QL_HIP(hipEventCreate(&e1));
     try {
-      for (int lanes : {1, 2, 4, 8}) {
+      for (int lanes : {1, 2, 4, 8, 16}) {
         if (lanes > nm && lanes > 1) break;
-        for (int chunk : {1, 4, 16}) {
+        for (int chunk : {1, 4, 16, 64}) {
           if (chunk > trial_steps && chunk > 1) break;
           t.lanes = lanes;
           t.chunk = chunk;
