@@ -214,6 +214,67 @@ def run_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def run_adjoint(args):
+    """Differentiable 3-D HIT: gradient of an MSE loss after K RK4 steps with respect to the initial
+    spectrum, through the checkpointed step adjoint (forward K fused steps, backward K step VJPs)."""
+    import torch
+
+    from l3es_amd import adjoint as A
+    from l3es_amd import set_up_solver
+
+    torch.cuda.set_device(0)
+    N, K, every = args.adj_N, args.adj_K, args.adj_every
+    _, uh0, _, _, step, _, _, _, _ = set_up_solver(N, args.nu, 3, "HIT", args.dt, args.seed, 16, None,
+                                                   e_kin_target=None, forcing_type="none", spectrum="k4exp",
+                                                   precision="float32", warmup=False)
+    g = torch.Generator(device="cpu").manual_seed(args.seed + 1)
+    with torch.no_grad():
+        pert = uh0 + 0.01 * (uh0.abs().max()) * torch.complex(
+            torch.randn(uh0.shape, generator=g), torch.randn(uh0.shape, generator=g)).to(uh0.device)
+        target = A.irfftn(A.integrate(step, pert, K, every), step.plan)
+
+    def grad_eval():
+        x = uh0.clone().requires_grad_(True)
+        t0 = time.perf_counter()
+        uK = A.integrate(step, x, K, every)
+        loss = torch.mean((A.irfftn(uK, step.plan) - target) ** 2)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return float(loss), t1 - t0, t2 - t1, x.grad
+
+    grad_eval()  # warm-up (workspace allocation)
+    reps = []
+    for _ in range(args.adj_reps):
+        reps.append(grad_eval())
+    fwd = min(r[1] for r in reps)
+    bwd = min(r[2] for r in reps)
+    grad = reps[-1][3]
+    # dot-product residual of the FP32 step maps at this size
+    v = torch.complex(torch.randn(uh0.shape, generator=g), torch.randn(uh0.shape, generator=g)).to(uh0.device)
+    w = torch.complex(torch.randn(uh0.shape, generator=g), torch.randn(uh0.shape, generator=g)).to(uh0.device)
+    jv = A.step_jvp(step, uh0, v)
+    lhs = float(torch.sum(jv.conj() * w).real)
+    rhs = float(torch.sum(v.conj() * A.step_vjp(step, uh0, w)).real)
+    dp = abs(lhs - rhs) / (float(torch.linalg.vector_norm(jv)) * float(torch.linalg.vector_norm(w)))
+    log(f"[bench] adjoint {N}^3 K={K}: forward {1e3 * fwd:.1f} ms, backward {1e3 * bwd:.1f} ms, dot-product {dp:.2e}")
+    out = {
+        "metric": f"gradient evaluations/s, differentiable HIT {N}^3 FP32, K={K} RK4 steps",
+        "value": 1.0 / (fwd + bwd), "unit": "gradients/s", "n_gpus": 1, "steps": K, "warmup": 1,
+        "ms_per_step": 1e3 * (fwd + bwd) / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"loss = mean((irfftn(u_hat_K) - target)^2), target from a perturbed initial "
+                               f"spectrum; checkpoints every {every} steps; nu={args.nu}, dt={args.dt}, "
+                               f"Rogallo k^4 exp(-2(k/4)^2) init, forcing none",
+                   "N": N, "K": K, "checkpoint_every": every},
+        "forward_ms": 1e3 * fwd, "backward_ms": 1e3 * bwd, "backward_over_forward": bwd / fwd,
+        "loss": reps[-1][0], "grad_finite": bool(torch.isfinite(grad).all()), "dot_product_rel_residual": dp,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(args):
     """The FP64 NumPy/SciPy restatement (oracle) timed on this host's cores, bounded sample."""
     import numpy as np
@@ -263,8 +324,13 @@ def main():
     ap.add_argument("--copy-probe-mib", dest="copy_probe", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (counter calibration)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH/WRITE passes")
-    ap.add_argument("--workload", choices=("hit256", "batch"), default="hit256",
-                    help="hit256: the headline 256^3 step (default); batch: 24 uneven batched trajectories")
+    ap.add_argument("--workload", choices=("hit256", "batch", "adjoint"), default="hit256",
+                    help="hit256: the headline 256^3 step (default); batch: 24 uneven batched trajectories; "
+                         "adjoint: gradient through K checkpointed steps")
+    ap.add_argument("--adj-N", dest="adj_N", type=int, default=128)
+    ap.add_argument("--adj-K", dest="adj_K", type=int, default=50)
+    ap.add_argument("--adj-every", dest="adj_every", type=int, default=10)
+    ap.add_argument("--adj-reps", dest="adj_reps", type=int, default=3)
     ap.add_argument("--batch-members", dest="batch_members", type=int, default=24)
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -295,6 +361,9 @@ def main():
 
     if args.workload == "batch":
         run_batch(args)
+        return
+    if args.workload == "adjoint":
+        run_adjoint(args)
         return
 
     import torch

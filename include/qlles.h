@@ -122,6 +122,27 @@ int qlles_energy_hat(qlles_plan* plan, const void* u_hat, double* out, void* str
  * u_lres: (3,n,n,n) / (2,n,n) real, n = ckp_N <= N, n even. */
 int qlles_filter(qlles_plan* plan, const void* u_hat, int ckp_N, void* u_lres, void* stream);
 
+/* ---- tangent-linear and adjoint maps -----------------------------------------
+ * No reference counterpart (the reference computes no gradients); they make
+ * the step differentiable.  Defined for 3-D plans with forcing none and the
+ * explicit viscous term.  Inner products: sum_x a.b on physical fields and
+ * Re sum_k conj(a_k) b_k over the stored half-spectrum (torch's convention
+ * for complex gradients), so that for the map y -> f(y):
+ *   <jvp(v), w> = <v, vjp(w)>.
+ * All spectra are (3,N,N,N/2+1); `out` may not alias an input.
+ *   qlles_rhs_jvp / _vjp : R'(u_hat) v and R'(u_hat)^T lam for
+ *                          rhs_fn(u, u_hat, rk>0) (spectral_solver.py:50-60)
+ *   qlles_step_jvp / _vjp: the same for one classical RK4 step
+ *                          (spectral_solver.py:73-80) taken from u_hat
+ *   qlles_irfftn_adjoint : adjoint of irfftn: out (spectrum) from g (physical)
+ *   qlles_rfftn_adjoint  : adjoint of rfftn: out (physical) from g_hat (spectrum) */
+int qlles_rhs_jvp(qlles_plan* plan, const void* u_hat, const void* v, void* out, void* stream);
+int qlles_rhs_vjp(qlles_plan* plan, const void* u_hat, const void* lam, void* out, void* stream);
+int qlles_step_jvp(qlles_plan* plan, const void* u_hat, const void* v, void* out, void* stream);
+int qlles_step_vjp(qlles_plan* plan, const void* u_hat, const void* lam, void* out, void* stream);
+int qlles_irfftn_adjoint(qlles_plan* plan, const void* g, void* out, void* stream);
+int qlles_rfftn_adjoint(qlles_plan* plan, const void* g_hat, void* out, void* stream);
+
 /* ---- measurement ----------------------------------------------------------
  * No reference counterpart (the reference times whole steps with wall-clock
  * around block_until_ready, spectral_solver.py:310-313).  When enabled, every

@@ -116,6 +116,40 @@ int qlles_filter(qlles_plan* p, const void* u_hat, int ckp_N, void* u_lres, void
   return guarded([&] { p->eng->filter(u_hat, ckp_N, u_lres, (hipStream_t)stream); });
 }
 
+int qlles_rhs_jvp(qlles_plan* p, const void* u_hat, const void* v, void* out, void* stream) {
+  if (!p || !u_hat || !v || !out) return fail(QLLES_ERR_ARG, "null argument");
+  if (out == u_hat || out == v) return fail(QLLES_ERR_ARG, "out may not alias an input");
+  return guarded([&] { p->eng->rhs_jvp(u_hat, v, out, (hipStream_t)stream); });
+}
+
+int qlles_rhs_vjp(qlles_plan* p, const void* u_hat, const void* lam, void* out, void* stream) {
+  if (!p || !u_hat || !lam || !out) return fail(QLLES_ERR_ARG, "null argument");
+  if (out == u_hat || out == lam) return fail(QLLES_ERR_ARG, "out may not alias an input");
+  return guarded([&] { p->eng->rhs_vjp(u_hat, lam, out, (hipStream_t)stream); });
+}
+
+int qlles_step_jvp(qlles_plan* p, const void* u_hat, const void* v, void* out, void* stream) {
+  if (!p || !u_hat || !v || !out) return fail(QLLES_ERR_ARG, "null argument");
+  if (out == u_hat || out == v) return fail(QLLES_ERR_ARG, "out may not alias an input");
+  return guarded([&] { p->eng->step_jvp(u_hat, v, out, (hipStream_t)stream); });
+}
+
+int qlles_step_vjp(qlles_plan* p, const void* u_hat, const void* lam, void* out, void* stream) {
+  if (!p || !u_hat || !lam || !out) return fail(QLLES_ERR_ARG, "null argument");
+  if (out == u_hat || out == lam) return fail(QLLES_ERR_ARG, "out may not alias an input");
+  return guarded([&] { p->eng->step_vjp(u_hat, lam, out, (hipStream_t)stream); });
+}
+
+int qlles_irfftn_adjoint(qlles_plan* p, const void* g, void* out, void* stream) {
+  if (!p || !g || !out) return fail(QLLES_ERR_ARG, "null argument");
+  return guarded([&] { p->eng->irfftn_adjoint(g, out, (hipStream_t)stream); });
+}
+
+int qlles_rfftn_adjoint(qlles_plan* p, const void* g_hat, void* out, void* stream) {
+  if (!p || !g_hat || !out) return fail(QLLES_ERR_ARG, "null argument");
+  return guarded([&] { p->eng->rfftn_adjoint(g_hat, out, (hipStream_t)stream); });
+}
+
 int qlles_profile(qlles_plan* p, int enable) {
   if (!p) return fail(QLLES_ERR_ARG, "null plan");
   return guarded([&] { p->eng->profile_enable(enable != 0); });

@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "../../include/qlles.h"
+#include "adjoint_kernels.hpp"
 #include "aux_kernels.hpp"
 #include "engine.hpp"
 #include "fftlib.hpp"
@@ -30,6 +31,11 @@ class EngineT final : public Engine {
   // each multi-step call and written back to the caller's array on the last step
   cx<T>* P = nullptr;
   cx<T>*S1a = nullptr, *S1b = nullptr, *S2 = nullptr, *X5 = nullptr, *Y6 = nullptr, *NZ = nullptr, *NY = nullptr;
+  // tangent / adjoint workspace, allocated on first use: spectra AS_[i]
+  // ([3][N][N][Nh]) and physical fields AP_[i] ([3][N][N][N])
+  static constexpr int NAS = 10, NAP = 5;
+  cx<T>* AS_[NAS] = {};
+  T* AP_[NAP] = {};
   double* partials_ = nullptr;
   double* alpha_ = nullptr;
   double* dscr_ = nullptr;
@@ -225,6 +231,82 @@ class EngineT final : public Engine {
       c2c_strided<T>(N, src, tmp, ncomp, ld_in, false, (T)1, twN_, s);
       c2r_contig<T>(N, tmp, dst, ncomp * N, ld_in, scale, twN_, s);
     }
+  }
+
+  // ---------------- tangent-linear / adjoint (3-D) ----------------
+  size_t spec_elems() const { return (size_t)3 * rows_ * Nh; }
+  size_t phys_elems() const { return (size_t)3 * N * N * (DIM == 3 ? N : 1); }
+  void ensure_adj() {
+    if (AS_[0]) return;
+    for (int i = 0; i < NAS; ++i) AS_[i] = alloc<cx<T>>(spec_elems());
+    for (int i = 0; i < NAP; ++i) AP_[i] = alloc<T>(phys_elems());
+  }
+  void check_adj() const {
+    if (DIM != 3) throw std::invalid_argument("tangent / adjoint maps are implemented for 3-D grids");
+    if (d_.forcing != QLLES_FORCING_NONE) throw std::invalid_argument("tangent / adjoint maps need forcing none");
+    if (d_.viscous != QLLES_VISC_EXPLICIT) throw std::invalid_argument("tangent / adjoint maps need the explicit viscous term");
+  }
+  static int nblk(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 8192); }
+  void to_phys(const cx<T>* src, T* dst, hipStream_t s) { irfftn_into(src, Nh, AS_[9], dst, 3, s); }
+  void to_spec(const T* src, cx<T>* dst, hipStream_t s) {
+    r2c_contig<T>(N, src, dst, 3 * N * N, Nh, (T)1, twN_, s);
+    c2c_strided<T>(N, dst, dst, 3 * N, Nh, true, (T)1, twN_, s);
+    c2c_strided<T>(N, dst, dst, 3, N * Nh, true, (T)1, twN_, s);
+  }
+  void curl(const cx<T>* src, cx<T>* dst, hipStream_t s) {
+    hipLaunchKernelGGL((k_curl_hat<T>), dim3(nblk(rows_ * Nh)), dim3(256), 0, s, src, dst, N, 3, Nh, Nh);
+    QL_LAUNCHED();
+  }
+  void axpby(const cx<T>* x, const cx<T>* y, cx<T>* out, T a, T b, hipStream_t s) {
+    hipLaunchKernelGGL((k_axpby<T>), dim3(nblk(spec_elems())), dim3(256), 0, s, x, y, out, a, b, spec_elems());
+    QL_LAUNCHED();
+  }
+  void cross2(const T* a, const T* b, const T* c, const T* d, T* out, hipStream_t s) {
+    const size_t n = (size_t)N * N * N;
+    hipLaunchKernelGGL((k_cross2<T>), dim3(nblk(n)), dim3(256), 0, s, a, b, c, d, out, n);
+    QL_LAUNCHED();
+  }
+  // u = irfftn(y) -> AP_[0], w = irfftn(K y) -> AP_[1]
+  void state_fields(const cx<T>* y, hipStream_t s) {
+    to_phys(y, AP_[0], s);
+    curl(y, AS_[8], s);
+    to_phys(AS_[8], AP_[1], s);
+  }
+  // out = R'(y) d  (temporaries AS_[7..9], AP_[0..4])
+  void rhs_jvp_at(const cx<T>* y, const cx<T>* d, cx<T>* out, hipStream_t s) {
+    state_fields(y, s);
+    to_phys(d, AP_[2], s);
+    curl(d, AS_[8], s);
+    to_phys(AS_[8], AP_[3], s);
+    cross2(AP_[2], AP_[1], AP_[0], AP_[3], AP_[4], s);  // du x w + u x dw
+    to_spec(AP_[4], AS_[7], s);
+    hipLaunchKernelGGL((k_jvp_combine<T>), dim3(nblk(rows_ * Nh)), dim3(256), 0, s, AS_[7], d, out, N, KZD,
+                       (T)d_.nu);
+    QL_LAUNCHED();
+  }
+  // out = R'(y)^T l  (temporaries AS_[7..9], AP_[0..4])
+  void rhs_vjp_at(const cx<T>* y, const cx<T>* l, cx<T>* out, hipStream_t s) {
+    state_fields(y, s);
+    hipLaunchKernelGGL((k_pd<T>), dim3(nblk(rows_ * Nh)), dim3(256), 0, s, l, AS_[8], N, KZD, 1);
+    QL_LAUNCHED();
+    to_phys(AS_[8], AP_[2], s);                           // psi = irfftn(P D l / c)
+    cross2(AP_[1], AP_[2], nullptr, nullptr, AP_[3], s);  // w x psi
+    to_spec(AP_[3], AS_[7], s);
+    cross2(AP_[2], AP_[0], nullptr, nullptr, AP_[3], s);  // psi x u
+    to_spec(AP_[3], AS_[8], s);
+    hipLaunchKernelGGL((k_vjp_combine<T>), dim3(nblk(rows_ * Nh)), dim3(256), 0, s, AS_[7], AS_[8], l, out, N, KZD,
+                       (T)d_.nu);
+    QL_LAUNCHED();
+  }
+  // RK4 stage states y1, y2, y3 of the step from y0 (spectral_solver.py:74-80) -> AS_[0..2]
+  void stage_states(const cx<T>* y0, hipStream_t s) {
+    const T h = (T)d_.dt;
+    rhs(nullptr, y0, 1, AS_[3], s);
+    axpby(y0, AS_[3], AS_[0], (T)1, h / 2, s);
+    rhs(nullptr, AS_[0], 1, AS_[3], s);
+    axpby(y0, AS_[3], AS_[1], (T)1, h / 2, s);
+    rhs(nullptr, AS_[1], 1, AS_[3], s);
+    axpby(y0, AS_[3], AS_[2], (T)1, h, s);
   }
 
  public:
@@ -448,6 +530,76 @@ class EngineT final : public Engine {
     hipLaunchKernelGGL(k_energy_hat_final, dim3(1), dim3(RED_THREADS), 0, s, partials_, RED_BLOCKS, 1.0 / (nd * nd),
                        out);
     QL_LAUNCHED();
+  }
+
+  void rhs_jvp(const void* uhat, const void* v, void* out, hipStream_t s) override {
+    check_adj();
+    ensure_adj();
+    rhs_jvp_at(static_cast<const cx<T>*>(uhat), static_cast<const cx<T>*>(v), static_cast<cx<T>*>(out), s);
+  }
+  void rhs_vjp(const void* uhat, const void* lam, void* out, hipStream_t s) override {
+    check_adj();
+    ensure_adj();
+    rhs_vjp_at(static_cast<const cx<T>*>(uhat), static_cast<const cx<T>*>(lam), static_cast<cx<T>*>(out), s);
+  }
+  // tangent of one classical RK4 step: dy' = v + h/6 (dk1 + 2 dk2 + 2 dk3 + dk4)
+  void step_jvp(const void* uhat, const void* v_in, void* out_v, hipStream_t s) override {
+    check_adj();
+    ensure_adj();
+    const T h = (T)d_.dt;
+    const cx<T>* y0 = static_cast<const cx<T>*>(uhat);
+    cx<T>* out = static_cast<cx<T>*>(out_v);
+    cx<T>* v = AS_[6];
+    QL_HIP(hipMemcpyAsync(v, v_in, spec_elems() * sizeof(cx<T>), hipMemcpyDeviceToDevice, s));
+    stage_states(y0, s);
+    cx<T>*dk = AS_[3], *dy = AS_[4];
+    const cx<T>* ys[4] = {y0, AS_[0], AS_[1], AS_[2]};
+    const T wk[4] = {h / 6, h / 3, h / 3, h / 6};
+    const T wy[3] = {h / 2, h / 2, h};
+    for (int st = 0; st < 4; ++st) {
+      rhs_jvp_at(ys[st], st == 0 ? v : dy, dk, s);
+      axpby(st == 0 ? v : out, dk, out, (T)1, wk[st], s);
+      if (st < 3) axpby(v, dk, dy, (T)1, wy[st], s);
+    }
+  }
+  // adjoint of one classical RK4 step, backward through the stages
+  void step_vjp(const void* uhat, const void* lam_in, void* out_v, hipStream_t s) override {
+    check_adj();
+    ensure_adj();
+    const T h = (T)d_.dt;
+    const cx<T>* y0 = static_cast<const cx<T>*>(uhat);
+    cx<T>* out = static_cast<cx<T>*>(out_v);
+    cx<T>* lam = AS_[6];
+    QL_HIP(hipMemcpyAsync(lam, lam_in, spec_elems() * sizeof(cx<T>), hipMemcpyDeviceToDevice, s));
+    stage_states(y0, s);
+    cx<T>*kb = AS_[3], *yb = AS_[4], *acc = AS_[5];
+    const cx<T>* ys[4] = {y0, AS_[0], AS_[1], AS_[2]};
+    axpby(lam, nullptr, acc, (T)1, (T)0, s);
+    // kbar_4 = h/6 l; kbar_3 = h/3 l + h ybar_3; kbar_2 = h/3 l + h/2 ybar_2; kbar_1 = h/6 l + h/2 ybar_1
+    const T wl[4] = {h / 6, h / 3, h / 3, h / 6};  // for stages 3, 2, 1, 0
+    const T wb[4] = {0, h, h / 2, h / 2};
+    for (int q = 0; q < 4; ++q) {
+      const int st = 3 - q;
+      axpby(lam, q == 0 ? nullptr : yb, kb, wl[q], wb[q], s);
+      rhs_vjp_at(ys[st], kb, yb, s);
+      axpby(acc, yb, st == 0 ? out : acc, (T)1, (T)1, s);
+    }
+  }
+  void irfftn_adjoint(const void* g, void* out_v, hipStream_t s) override {
+    cx<T>* out = static_cast<cx<T>*>(out_v);
+    rfftn(g, out, s);
+    const double nd = DIM == 3 ? (double)N * N * N : (double)N * N;
+    hipLaunchKernelGGL((k_cweight<T>), dim3(nblk(spec_elems())), dim3(256), 0, s, out, out, N, rows_, 3,
+                       (T)(1.0 / nd), 0);
+    QL_LAUNCHED();
+  }
+  void rfftn_adjoint(const void* gh, void* out, hipStream_t s) override {
+    ensure_adj();
+    const double nd = DIM == 3 ? (double)N * N * N : (double)N * N;
+    hipLaunchKernelGGL((k_cweight<T>), dim3(nblk(spec_elems())), dim3(256), 0, s, static_cast<const cx<T>*>(gh),
+                       AS_[8], N, rows_, 3, (T)nd, 1);
+    QL_LAUNCHED();
+    irfftn_into(AS_[8], Nh, AS_[9], static_cast<T*>(out), 3, s);
   }
 
   void filter(const void* uhat_v, int n, void* ulres_v, hipStream_t s) override {

@@ -32,6 +32,8 @@ EXPORTS = (
     "qlles_interp_field", "qlles_interp_hat", "qlles_last_error", "qlles_profile", "qlles_profile_read",
     "qlles_step_advect", "qlles_batch_create", "qlles_batch_destroy", "qlles_batch_add",
     "qlles_batch_configure", "qlles_batch_tune", "qlles_batch_get_config", "qlles_batch_run",
+    "qlles_rhs_jvp", "qlles_rhs_vjp", "qlles_step_jvp", "qlles_step_vjp", "qlles_irfftn_adjoint",
+    "qlles_rfftn_adjoint",
 )
 KINDS = ("x-pass", "y-inverse", "z-pass", "y-forward", "other")
 
@@ -94,6 +96,12 @@ def load():
         "qlles_batch_tune": (ci, [vp, ci, vp]),
         "qlles_batch_get_config": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), pd]),
         "qlles_batch_run": (ci, [vp, vp, ci, vp]),
+        "qlles_rhs_jvp": (ci, [vp, vp, vp, vp, vp]),
+        "qlles_rhs_vjp": (ci, [vp, vp, vp, vp, vp]),
+        "qlles_step_jvp": (ci, [vp, vp, vp, vp, vp]),
+        "qlles_step_vjp": (ci, [vp, vp, vp, vp, vp]),
+        "qlles_irfftn_adjoint": (ci, [vp, vp, vp, vp]),
+        "qlles_rfftn_adjoint": (ci, [vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

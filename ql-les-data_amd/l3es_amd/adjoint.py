@@ -1,0 +1,110 @@
+"""Tangent-linear / adjoint maps of the RK4 step and a differentiable, checkpointed integrator.
+
+The reference computes no gradients (its solver is a plain JAX forward map, spectral_solver.py:73-118).
+The maps here are the exact tangent-linear model and adjoint of the classical RK4 step
+(spectral_solver.py:70-80) with the rotational-form RHS (:50-60): explicit viscous term, no forcing,
+3-D.  They run in libqlles.so (qlles_{rhs,step}_{jvp,vjp}).  Inner products: Re sum_k conj(a_k) b_k
+over the stored half-spectrum, which is torch's convention for gradients of complex tensors, and
+sum_x a.b for physical fields.
+
+``integrate(step, u_hat0, nsteps, every)`` is a torch.autograd.Function: its forward takes nsteps fused
+RK4 steps and keeps a checkpoint every ``every`` steps; its backward recomputes each segment from its
+checkpoint and applies the step adjoint in reverse.  ``irfftn(u_hat, plan)`` is differentiable too.
+"""
+
+import torch
+
+from . import _lib
+
+
+def _run(name, plan, a, b, out):
+    _lib.check(getattr(_lib.load(), name)(plan.handle, _lib.ptr(a), _lib.ptr(b), _lib.ptr(out), _lib.stream_ptr()))
+    return out
+
+
+def rhs_jvp(step, u_hat, v):
+    """R'(u_hat) v for rhs_fn(u, u_hat, rk > 0)."""
+    return _run("qlles_rhs_jvp", step.plan, u_hat.contiguous(), v.contiguous(), torch.empty_like(u_hat))
+
+
+def rhs_vjp(step, u_hat, lam):
+    """R'(u_hat)^T lam."""
+    return _run("qlles_rhs_vjp", step.plan, u_hat.contiguous(), lam.contiguous(), torch.empty_like(u_hat))
+
+
+def step_jvp(step, u_hat, v):
+    """Tangent of one RK4 step taken from u_hat, applied to v."""
+    return _run("qlles_step_jvp", step.plan, u_hat.contiguous(), v.contiguous(), torch.empty_like(u_hat))
+
+
+def step_vjp(step, u_hat, lam):
+    """Adjoint of one RK4 step taken from u_hat, applied to lam."""
+    return _run("qlles_step_vjp", step.plan, u_hat.contiguous(), lam.contiguous(), torch.empty_like(u_hat))
+
+
+def _advance(step, uh):
+    step.advance(None, uh, 1, None, use_u=False)
+
+
+class _Integrate(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, u_hat0, step, nsteps, every):
+        uh = u_hat0.detach().contiguous().clone()
+        ckpts = []
+        for n in range(nsteps):
+            if n % every == 0:
+                ckpts.append(uh.clone())
+            _advance(step, uh)
+        ctx.step, ctx.nsteps, ctx.every, ctx.ckpts = step, nsteps, every, ckpts
+        return uh
+
+    @staticmethod
+    def backward(ctx, grad):
+        step, nsteps, every = ctx.step, ctx.nsteps, ctx.every
+        lam = grad.contiguous()
+        for seg in reversed(range(len(ctx.ckpts))):
+            start = seg * every
+            stop = min(start + every, nsteps)
+            states = [ctx.ckpts[seg]]
+            uh = ctx.ckpts[seg].clone()
+            for _ in range(start, stop - 1):  # recompute the segment's step inputs
+                _advance(step, uh)
+                states.append(uh.clone())
+            for y in reversed(states):
+                lam = step_vjp(step, y, lam)
+        ctx.ckpts = None
+        return lam, None, None, None
+
+
+def integrate(step, u_hat0, nsteps, every=10):
+    """nsteps RK4 steps of u_hat0, differentiable with checkpoints every ``every`` steps."""
+    if nsteps < 1 or every < 1:
+        raise ValueError("need nsteps >= 1 and every >= 1")
+    return _Integrate.apply(u_hat0, step, int(nsteps), int(every))
+
+
+class _Irfftn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, u_hat, plan, shape):
+        ctx.plan = plan
+        u = torch.empty(shape, dtype=torch.float32 if u_hat.dtype == torch.complex64 else torch.float64,
+                        device=u_hat.device)
+        _lib.check(_lib.load().qlles_irfftn(plan.handle, _lib.ptr(u_hat.detach().contiguous()), _lib.ptr(u),
+                                            _lib.stream_ptr()))
+        ctx.hat_shape = u_hat.shape
+        return u
+
+    @staticmethod
+    def backward(ctx, g):
+        out = torch.empty(ctx.hat_shape, dtype=torch.complex64 if g.dtype == torch.float32 else torch.complex128,
+                          device=g.device)
+        _lib.check(_lib.load().qlles_irfftn_adjoint(ctx.plan.handle, _lib.ptr(g.contiguous()), _lib.ptr(out),
+                                                    _lib.stream_ptr()))
+        return out, None, None
+
+
+def irfftn(u_hat, plan):
+    """irfftn(u_hat) over the plan's axes (spectral_solver.py:114), differentiable."""
+    N = u_hat.shape[1]
+    shape = (3, N, N, N) if u_hat.dim() == 4 else (3, N, N)
+    return _Irfftn.apply(u_hat, plan, shape)

@@ -1,0 +1,139 @@
+"""Tangent-linear and adjoint maps (qlles_*_jvp / _vjp) and the checkpointed differentiable integrator.
+
+No reference counterpart (the reference computes no gradient).  Pins, without the reference:
+  - dot-product identity <J v, w> = <v, J^T w> (1e-10 relative in FP64, 1e-5 in FP32);
+  - the tangent map equals the central difference of the forward step (which runs the fused kernels);
+  - the gradient of an MSE loss after K checkpointed steps matches a central finite difference
+    (eps = 1e-3, within 1e-4 relative).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+
+def _inner(a, b):
+    return float(torch.sum(a.conj() * b).real)
+
+
+def _setup(N, prec, seed=3):
+    from l3es_amd import set_up_solver
+
+    _, uh, _, _, step, _, _, _, _ = set_up_solver(N, 0.02, 3, "HIT", 5e-3, seed, 8, None, forcing_type="none",
+                                                  precision=prec, warmup=False)
+    return uh, step
+
+
+def _rand_hat(uh, seed):
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    re = torch.randn(uh.shape, generator=g, dtype=torch.float64)
+    im = torch.randn(uh.shape, generator=g, dtype=torch.float64)
+    return torch.complex(re, im).to(uh.dtype).to(uh.device)
+
+
+def _valid_dir(uh, step, seed):
+    """A direction that is the rfftn of a real field, scaled to the norm of u_hat."""
+    from l3es_amd import _lib
+
+    N = uh.shape[1]
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    real = torch.randn((3, N, N, N), generator=g, dtype=torch.float64)
+    real = real.to(torch.float32 if uh.dtype == torch.complex64 else torch.float64).cuda()
+    v = torch.empty_like(uh)
+    _lib.check(_lib.load().qlles_rfftn(step.plan.handle, _lib.ptr(real), _lib.ptr(v), _lib.stream_ptr()))
+    return v * (torch.linalg.vector_norm(uh) / torch.linalg.vector_norm(v))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,tol", [("float64", 1e-10), ("float32", 1e-5)])
+@pytest.mark.parametrize("which", ["rhs", "step"])
+def test_dot_product(gpu, prec, tol, which):
+    from l3es_amd import adjoint as A
+
+    uh, step = _setup(16, prec)
+    v, w = _rand_hat(uh, 1), _rand_hat(uh, 2)
+    jvp, vjp = (A.rhs_jvp, A.rhs_vjp) if which == "rhs" else (A.step_jvp, A.step_vjp)
+    lhs = _inner(jvp(step, uh, v), w)
+    rhs = _inner(v, vjp(step, uh, w))
+    scale = float(torch.linalg.vector_norm(jvp(step, uh, v))) * float(torch.linalg.vector_norm(w))
+    assert abs(lhs - rhs) <= tol * scale, (lhs, rhs)
+
+
+@pytest.mark.gpu
+def test_step_jvp_matches_central_difference(gpu):
+    from l3es_amd import adjoint as A
+
+    uh, step = _setup(16, "float64")
+    v = _valid_dir(uh, step, 5)
+    eps = 1e-4
+
+    def fwd(x):
+        y = x.clone()
+        step.advance(None, y, 1, None, use_u=False)
+        return y
+
+    fd = (fwd(uh + eps * v) - fwd(uh - eps * v)) / (2 * eps)
+    jv = A.step_jvp(step, uh, v)
+    err = float(torch.linalg.vector_norm(fd - jv) / torch.linalg.vector_norm(jv))
+    assert err < 1e-6, err
+
+
+@pytest.mark.gpu
+def test_rhs_vjp_rejects_forced_plans(gpu):
+    from l3es_amd import _lib, set_up_solver
+    from l3es_amd import adjoint as A
+
+    _, uh, _, _, step, _, _, _, _ = set_up_solver(16, 0.02, 3, "HIT", 5e-3, 3, 8, 2, forcing_type="power",
+                                                  precision="float64", warmup=False)
+    with pytest.raises(_lib.QllesError):
+        A.rhs_vjp(step, uh, uh.clone())
+
+
+@pytest.mark.gpu
+def test_checkpointed_gradient_matches_central_difference(gpu):
+    from l3es_amd import adjoint as A
+
+    uh0, step = _setup(16, "float64", seed=7)
+    K, every = 7, 3
+    g = torch.Generator(device="cpu").manual_seed(11)
+    with torch.no_grad():
+        target = A.irfftn(A.integrate(step, uh0, K, every), step.plan)
+        target = target + 0.05 * torch.randn(target.shape, generator=g, dtype=torch.float64).cuda()
+
+    def loss(x):
+        return torch.mean((A.irfftn(A.integrate(step, x, K, every), step.plan) - target) ** 2)
+
+    x = uh0.clone().requires_grad_(True)
+    L = loss(x)
+    L.backward()
+    grad = x.grad.detach()
+    v = _valid_dir(uh0, step, 13)
+    eps = 1e-3
+    with torch.no_grad():
+        fd = (float(loss(uh0 + eps * v)) - float(loss(uh0 - eps * v))) / (2 * eps)
+    an = _inner(grad, v)
+    assert abs(fd - an) <= 1e-4 * abs(an), (fd, an)
+
+
+@pytest.mark.gpu
+def test_irfftn_adjoint_pairs(gpu):
+    """<irfftn(a), g> = <a, irfftn^T g> and <rfftn(x), b> = <x, rfftn^T b>."""
+    from l3es_amd import _lib
+
+    uh, step = _setup(16, "float64")
+    lib = _lib.load()
+    a, b = _rand_hat(uh, 21), _rand_hat(uh, 22)
+    gen = torch.Generator(device="cpu").manual_seed(23)
+    x = torch.randn((3, 16, 16, 16), generator=gen, dtype=torch.float64).cuda()
+    g = torch.randn((3, 16, 16, 16), generator=gen, dtype=torch.float64).cuda()
+    ia, gt, rx, bt = torch.empty_like(x), torch.empty_like(uh), torch.empty_like(uh), torch.empty_like(x)
+    s = _lib.stream_ptr()
+    _lib.check(lib.qlles_irfftn(step.plan.handle, _lib.ptr(a), _lib.ptr(ia), s))
+    _lib.check(lib.qlles_irfftn_adjoint(step.plan.handle, _lib.ptr(g), _lib.ptr(gt), s))
+    _lib.check(lib.qlles_rfftn(step.plan.handle, _lib.ptr(x), _lib.ptr(rx), s))
+    _lib.check(lib.qlles_rfftn_adjoint(step.plan.handle, _lib.ptr(b), _lib.ptr(bt), s))
+    l1, r1 = float(torch.sum(ia * g)), _inner(a, gt)
+    l2, r2 = _inner(rx, b), float(torch.sum(x * bt))
+    assert abs(l1 - r1) <= 1e-12 * max(1.0, abs(l1))
+    assert abs(l2 - r2) <= 1e-12 * max(1.0, abs(l2)) * 16 ** 3
+    assert np.isfinite(l1) and np.isfinite(l2)
