@@ -124,8 +124,9 @@ int qlles_filter(qlles_plan* plan, const void* u_hat, int ckp_N, void* u_lres, v
 
 /* ---- tangent-linear and adjoint maps -----------------------------------------
  * No reference counterpart (the reference computes no gradients); they make
- * the step differentiable.  Defined for 3-D plans with forcing none and the
- * explicit viscous term.  Inner products: sum_x a.b on physical fields and
+ * the step differentiable.  Defined for 3-D plans with the explicit viscous
+ * term and forcing none or fixed power (the power term a(u) M u with
+ * a = P / (2 E_f(u) + eps) is differentiated through E_f).  Inner products: sum_x a.b on physical fields and
  * Re sum_k conj(a_k) b_k over the stored half-spectrum (torch's convention
  * for complex gradients), so that for the map y -> f(y):
  *   <jvp(v), w> = <v, vjp(w)>.
@@ -137,9 +138,11 @@ int qlles_filter(qlles_plan* plan, const void* u_hat, int ckp_N, void* u_lres, v
  *   qlles_irfftn_adjoint : adjoint of irfftn: out (spectrum) from g (physical)
  *   qlles_rfftn_adjoint  : adjoint of rfftn: out (physical) from g_hat (spectrum) */
 int qlles_rhs_jvp(qlles_plan* plan, const void* u_hat, const void* v, void* out, void* stream);
-int qlles_rhs_vjp(qlles_plan* plan, const void* u_hat, const void* lam, void* out, void* stream);
+/* dpower: optional device double; the VJPs ADD <lam, d(map)/dP> to it (the
+ * cotangent of the injection power of a fixed-power plan) */
+int qlles_rhs_vjp(qlles_plan* plan, const void* u_hat, const void* lam, void* out, double* dpower, void* stream);
 int qlles_step_jvp(qlles_plan* plan, const void* u_hat, const void* v, void* out, void* stream);
-int qlles_step_vjp(qlles_plan* plan, const void* u_hat, const void* lam, void* out, void* stream);
+int qlles_step_vjp(qlles_plan* plan, const void* u_hat, const void* lam, void* out, double* dpower, void* stream);
 int qlles_irfftn_adjoint(qlles_plan* plan, const void* g, void* out, void* stream);
 int qlles_rfftn_adjoint(qlles_plan* plan, const void* g_hat, void* out, void* stream);
 

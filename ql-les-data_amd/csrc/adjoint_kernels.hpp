@@ -128,6 +128,58 @@ __global__ void k_cross2(const T* __restrict__ a, const T* __restrict__ b, const
   }
 }
 
+// Fixed-power forcing term F(y) = a(y) M y, a = P / (2 E_f(y) + eps),
+// E_f = 0.5 sum_S w |y_k|^2 / N^6 over the shell S = {0 < k^2 <= kf2} (w = 1 on
+// the kz = 0 plane, 2 elsewhere), as the fused x-pass evaluates it in FP64.
+//   mode 0 (JVP, x = d):   out_S += a d - (2P / (2E+eps)^2) <g_E, d> y,   <g_E, d> = sum_S w Re(conj(y) d) / N^6
+//   mode 1 (VJP, x = l):   out_S += a l - (2P / (2E+eps)^2) (sum_S Re(conj(y) l)) w y / N^6
+// and, in mode 1 with dpow != nullptr, *dpow += (1 / (2E+eps)) sum_S Re(conj(l) y),
+// the cotangent of the injection power P (dF/dP = M y / (2E+eps)).
+// One workgroup: the shell holds at most 31 x 31 x 16 modes (kf <= 15) and
+// the sums run in a fixed order.
+template <typename T>
+__global__ void __launch_bounds__(256) k_force_tangent(const cx<T>* __restrict__ y, const cx<T>* __restrict__ x,
+                                                       cx<T>* __restrict__ out, int N, int kf2, double power,
+                                                       int mode, double* dpow) {
+  __shared__ double s_red[3 * 256];
+  const int Nh = N / 2 + 1;
+  const size_t cs = (size_t)N * N * Nh;
+  int kf = 0;
+  while ((kf + 1) * (kf + 1) <= kf2) ++kf;
+  const int nx = 2 * kf + 1, nz = kf + 1, cnt = nx * nx * nz;
+  const double n6 = (double)N * N * N * (double)N * N * N;
+  double acc[3] = {0.0, 0.0, 0.0};  // E (unnormalised, x0.5 later), weighted dot, plain dot
+  for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+    const int kx = q / (nx * nz) - kf, ky = (q / nz) % nx - kf, kz = q % nz;
+    const int k2 = kx * kx + ky * ky + kz * kz;
+    if (k2 == 0 || k2 > kf2) continue;
+    const size_t m = ((size_t)((kx + N) % N) * N + (ky + N) % N) * Nh + kz;
+    const double w = kz == 0 ? 1.0 : 2.0;
+    for (int c = 0; c < 3; ++c) {
+      const cx<T> a = y[c * cs + m], b = x[c * cs + m];
+      const double re = (double)a.re * b.re + (double)a.im * b.im;
+      acc[0] += w * ((double)a.re * a.re + (double)a.im * a.im);
+      acc[1] += w * re;
+      acc[2] += re;
+    }
+  }
+  block_sum<3>(acc, s_red);
+  const double E = 0.5 * acc[0] / n6;
+  const double den = 2.0 * E + 2.220446049250313e-16;
+  const double a = power / den, b = 2.0 * power / (den * den);
+  if (mode == 1 && dpow != nullptr && threadIdx.x == 0) *dpow += acc[2] / den;
+  for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+    const int kx = q / (nx * nz) - kf, ky = (q / nz) % nx - kf, kz = q % nz;
+    const int k2 = kx * kx + ky * ky + kz * kz;
+    if (k2 == 0 || k2 > kf2) continue;
+    const size_t m = ((size_t)((kx + N) % N) * N + (ky + N) % N) * Nh + kz;
+    const double w = kz == 0 ? 1.0 : 2.0;
+    const T cy = mode == 0 ? (T)(-b * acc[1] / n6) : (T)(-b * acc[2] * w / n6);
+    for (int c = 0; c < 3; ++c)
+      out[c * cs + m] = cadd(out[c * cs + m], cadd(cscale(x[c * cs + m], (T)a), cscale(y[c * cs + m], cy)));
+  }
+}
+
 // out = alpha x + beta y (complex arrays of n entries; y may be nullptr)
 template <typename T>
 __global__ void k_axpby(const cx<T>* __restrict__ x, const cx<T>* __restrict__ y, cx<T>* __restrict__ out, T alpha,

@@ -122,10 +122,10 @@ int qlles_rhs_jvp(qlles_plan* p, const void* u_hat, const void* v, void* out, vo
   return guarded([&] { p->eng->rhs_jvp(u_hat, v, out, (hipStream_t)stream); });
 }
 
-int qlles_rhs_vjp(qlles_plan* p, const void* u_hat, const void* lam, void* out, void* stream) {
+int qlles_rhs_vjp(qlles_plan* p, const void* u_hat, const void* lam, void* out, double* dpower, void* stream) {
   if (!p || !u_hat || !lam || !out) return fail(QLLES_ERR_ARG, "null argument");
   if (out == u_hat || out == lam) return fail(QLLES_ERR_ARG, "out may not alias an input");
-  return guarded([&] { p->eng->rhs_vjp(u_hat, lam, out, (hipStream_t)stream); });
+  return guarded([&] { p->eng->rhs_vjp(u_hat, lam, out, dpower, (hipStream_t)stream); });
 }
 
 int qlles_step_jvp(qlles_plan* p, const void* u_hat, const void* v, void* out, void* stream) {
@@ -134,10 +134,10 @@ int qlles_step_jvp(qlles_plan* p, const void* u_hat, const void* v, void* out, v
   return guarded([&] { p->eng->step_jvp(u_hat, v, out, (hipStream_t)stream); });
 }
 
-int qlles_step_vjp(qlles_plan* p, const void* u_hat, const void* lam, void* out, void* stream) {
+int qlles_step_vjp(qlles_plan* p, const void* u_hat, const void* lam, void* out, double* dpower, void* stream) {
   if (!p || !u_hat || !lam || !out) return fail(QLLES_ERR_ARG, "null argument");
   if (out == u_hat || out == lam) return fail(QLLES_ERR_ARG, "out may not alias an input");
-  return guarded([&] { p->eng->step_vjp(u_hat, lam, out, (hipStream_t)stream); });
+  return guarded([&] { p->eng->step_vjp(u_hat, lam, out, dpower, (hipStream_t)stream); });
 }
 
 int qlles_irfftn_adjoint(qlles_plan* p, const void* g, void* out, void* stream) {

@@ -32,12 +32,13 @@ class Engine {
   virtual void energy_hat(const void* uhat, double* out, hipStream_t s) = 0;
   virtual void filter(const void* uhat, int n, void* ulres, hipStream_t s) = 0;
   // tangent-linear and adjoint maps of the RHS and of one RK4 step at u_hat
-  // (3-D, explicit viscous term, no forcing); inputs and outputs are
-  // [3][N][N][N/2+1] spectra and may not alias each other
+  // (3-D, explicit viscous term, forcing none or fixed power); inputs and
+  // outputs are [3][N][N][N/2+1] spectra and may not alias each other.
+  // dpow (device double, optional) accumulates the cotangent of the power.
   virtual void rhs_jvp(const void* uhat, const void* v, void* out, hipStream_t s) = 0;
-  virtual void rhs_vjp(const void* uhat, const void* lam, void* out, hipStream_t s) = 0;
+  virtual void rhs_vjp(const void* uhat, const void* lam, void* out, double* dpow, hipStream_t s) = 0;
   virtual void step_jvp(const void* uhat, const void* v, void* out, hipStream_t s) = 0;
-  virtual void step_vjp(const void* uhat, const void* lam, void* out, hipStream_t s) = 0;
+  virtual void step_vjp(const void* uhat, const void* lam, void* out, double* dpow, hipStream_t s) = 0;
   // adjoints of irfftn (c/N^d rfftn) and rfftn (N^d irfftn(. / c))
   virtual void irfftn_adjoint(const void* g, void* out, hipStream_t s) = 0;
   virtual void rfftn_adjoint(const void* gh, void* out, hipStream_t s) = 0;

@@ -243,7 +243,8 @@ class EngineT final : public Engine {
   }
   void check_adj() const {
     if (DIM != 3) throw std::invalid_argument("tangent / adjoint maps are implemented for 3-D grids");
-    if (d_.forcing != QLLES_FORCING_NONE) throw std::invalid_argument("tangent / adjoint maps need forcing none");
+    if (d_.forcing != QLLES_FORCING_NONE && d_.forcing != QLLES_FORCING_POWER)
+      throw std::invalid_argument("tangent / adjoint maps need forcing none or power");
     if (d_.viscous != QLLES_VISC_EXPLICIT) throw std::invalid_argument("tangent / adjoint maps need the explicit viscous term");
   }
   static int nblk(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 8192); }
@@ -283,9 +284,15 @@ class EngineT final : public Engine {
     hipLaunchKernelGGL((k_jvp_combine<T>), dim3(nblk(rows_ * Nh)), dim3(256), 0, s, AS_[7], d, out, N, KZD,
                        (T)d_.nu);
     QL_LAUNCHED();
+    if (d_.forcing == QLLES_FORCING_POWER) {
+      hipLaunchKernelGGL((k_force_tangent<T>), dim3(1), dim3(256), 0, s, y, d, out, N, d_.kf * d_.kf, d_.power, 0,
+                         nullptr);
+      QL_LAUNCHED();
+    }
   }
-  // out = R'(y)^T l  (temporaries AS_[7..9], AP_[0..4])
-  void rhs_vjp_at(const cx<T>* y, const cx<T>* l, cx<T>* out, hipStream_t s) {
+  // out = R'(y)^T l  (temporaries AS_[7..9], AP_[0..4]); with fixed-power
+  // forcing, *dpow (device, optional) accumulates <l, dR/dP>
+  void rhs_vjp_at(const cx<T>* y, const cx<T>* l, cx<T>* out, hipStream_t s, double* dpow = nullptr) {
     state_fields(y, s);
     hipLaunchKernelGGL((k_pd<T>), dim3(nblk(rows_ * Nh)), dim3(256), 0, s, l, AS_[8], N, KZD, 1);
     QL_LAUNCHED();
@@ -297,6 +304,11 @@ class EngineT final : public Engine {
     hipLaunchKernelGGL((k_vjp_combine<T>), dim3(nblk(rows_ * Nh)), dim3(256), 0, s, AS_[7], AS_[8], l, out, N, KZD,
                        (T)d_.nu);
     QL_LAUNCHED();
+    if (d_.forcing == QLLES_FORCING_POWER) {
+      hipLaunchKernelGGL((k_force_tangent<T>), dim3(1), dim3(256), 0, s, y, l, out, N, d_.kf * d_.kf, d_.power, 1,
+                         dpow);
+      QL_LAUNCHED();
+    }
   }
   // RK4 stage states y1, y2, y3 of the step from y0 (spectral_solver.py:74-80) -> AS_[0..2]
   void stage_states(const cx<T>* y0, hipStream_t s) {
@@ -537,10 +549,10 @@ class EngineT final : public Engine {
     ensure_adj();
     rhs_jvp_at(static_cast<const cx<T>*>(uhat), static_cast<const cx<T>*>(v), static_cast<cx<T>*>(out), s);
   }
-  void rhs_vjp(const void* uhat, const void* lam, void* out, hipStream_t s) override {
+  void rhs_vjp(const void* uhat, const void* lam, void* out, double* dpow, hipStream_t s) override {
     check_adj();
     ensure_adj();
-    rhs_vjp_at(static_cast<const cx<T>*>(uhat), static_cast<const cx<T>*>(lam), static_cast<cx<T>*>(out), s);
+    rhs_vjp_at(static_cast<const cx<T>*>(uhat), static_cast<const cx<T>*>(lam), static_cast<cx<T>*>(out), s, dpow);
   }
   // tangent of one classical RK4 step: dy' = v + h/6 (dk1 + 2 dk2 + 2 dk3 + dk4)
   void step_jvp(const void* uhat, const void* v_in, void* out_v, hipStream_t s) override {
@@ -563,7 +575,7 @@ class EngineT final : public Engine {
     }
   }
   // adjoint of one classical RK4 step, backward through the stages
-  void step_vjp(const void* uhat, const void* lam_in, void* out_v, hipStream_t s) override {
+  void step_vjp(const void* uhat, const void* lam_in, void* out_v, double* dpow, hipStream_t s) override {
     check_adj();
     ensure_adj();
     const T h = (T)d_.dt;
@@ -581,7 +593,7 @@ class EngineT final : public Engine {
     for (int q = 0; q < 4; ++q) {
       const int st = 3 - q;
       axpby(lam, q == 0 ? nullptr : yb, kb, wl[q], wb[q], s);
-      rhs_vjp_at(ys[st], kb, yb, s);
+      rhs_vjp_at(ys[st], kb, yb, s, dpow);
       axpby(acc, yb, st == 0 ? out : acc, (T)1, (T)1, s);
     }
   }

@@ -97,9 +97,9 @@ def load():
         "qlles_batch_get_config": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), pd]),
         "qlles_batch_run": (ci, [vp, vp, ci, vp]),
         "qlles_rhs_jvp": (ci, [vp, vp, vp, vp, vp]),
-        "qlles_rhs_vjp": (ci, [vp, vp, vp, vp, vp]),
+        "qlles_rhs_vjp": (ci, [vp, vp, vp, vp, vp, vp]),
         "qlles_step_jvp": (ci, [vp, vp, vp, vp, vp]),
-        "qlles_step_vjp": (ci, [vp, vp, vp, vp, vp]),
+        "qlles_step_vjp": (ci, [vp, vp, vp, vp, vp, vp]),
         "qlles_irfftn_adjoint": (ci, [vp, vp, vp, vp]),
         "qlles_rfftn_adjoint": (ci, [vp, vp, vp, vp]),
     }

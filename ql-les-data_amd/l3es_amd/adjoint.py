@@ -2,8 +2,9 @@
 
 The reference computes no gradients (its solver is a plain JAX forward map, spectral_solver.py:73-118).
 The maps here are the exact tangent-linear model and adjoint of the classical RK4 step
-(spectral_solver.py:70-80) with the rotational-form RHS (:50-60): explicit viscous term, no forcing,
-3-D.  They run in libqlles.so (qlles_{rhs,step}_{jvp,vjp}).  Inner products: Re sum_k conj(a_k) b_k
+(spectral_solver.py:70-80) with the rotational-form RHS (:50-60): 3-D, explicit viscous term, forcing
+none or fixed power (the power term is differentiated through its energy normalisation, and the
+adjoint also returns the cotangent of the injection power).  They run in libqlles.so (qlles_{rhs,step}_{jvp,vjp}).  Inner products: Re sum_k conj(a_k) b_k
 over the stored half-spectrum, which is torch's convention for gradients of complex tensors, and
 sum_x a.b for physical fields.
 
@@ -17,8 +18,12 @@ import torch
 from . import _lib
 
 
-def _run(name, plan, a, b, out):
-    _lib.check(getattr(_lib.load(), name)(plan.handle, _lib.ptr(a), _lib.ptr(b), _lib.ptr(out), _lib.stream_ptr()))
+def _run(name, plan, a, b, out, dpow=None):
+    lib = getattr(_lib.load(), name)
+    if name.endswith("_vjp"):
+        _lib.check(lib(plan.handle, _lib.ptr(a), _lib.ptr(b), _lib.ptr(out), _lib.ptr(dpow), _lib.stream_ptr()))
+    else:
+        _lib.check(lib(plan.handle, _lib.ptr(a), _lib.ptr(b), _lib.ptr(out), _lib.stream_ptr()))
     return out
 
 
@@ -27,9 +32,9 @@ def rhs_jvp(step, u_hat, v):
     return _run("qlles_rhs_jvp", step.plan, u_hat.contiguous(), v.contiguous(), torch.empty_like(u_hat))
 
 
-def rhs_vjp(step, u_hat, lam):
-    """R'(u_hat)^T lam."""
-    return _run("qlles_rhs_vjp", step.plan, u_hat.contiguous(), lam.contiguous(), torch.empty_like(u_hat))
+def rhs_vjp(step, u_hat, lam, dpower=None):
+    """R'(u_hat)^T lam; ``dpower`` (float64 device tensor of one element) accumulates <lam, dR/dP>."""
+    return _run("qlles_rhs_vjp", step.plan, u_hat.contiguous(), lam.contiguous(), torch.empty_like(u_hat), dpower)
 
 
 def step_jvp(step, u_hat, v):
@@ -37,9 +42,9 @@ def step_jvp(step, u_hat, v):
     return _run("qlles_step_jvp", step.plan, u_hat.contiguous(), v.contiguous(), torch.empty_like(u_hat))
 
 
-def step_vjp(step, u_hat, lam):
-    """Adjoint of one RK4 step taken from u_hat, applied to lam."""
-    return _run("qlles_step_vjp", step.plan, u_hat.contiguous(), lam.contiguous(), torch.empty_like(u_hat))
+def step_vjp(step, u_hat, lam, dpower=None):
+    """Adjoint of one RK4 step taken from u_hat, applied to lam (``dpower`` as in rhs_vjp)."""
+    return _run("qlles_step_vjp", step.plan, u_hat.contiguous(), lam.contiguous(), torch.empty_like(u_hat), dpower)
 
 
 def _advance(step, uh):
@@ -48,7 +53,7 @@ def _advance(step, uh):
 
 class _Integrate(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, u_hat0, step, nsteps, every):
+    def forward(ctx, u_hat0, power, step, nsteps, every):
         uh = u_hat0.detach().contiguous().clone()
         ckpts = []
         for n in range(nsteps):
@@ -62,6 +67,7 @@ class _Integrate(torch.autograd.Function):
     def backward(ctx, grad):
         step, nsteps, every = ctx.step, ctx.nsteps, ctx.every
         lam = grad.contiguous()
+        dpow = torch.zeros(1, dtype=torch.float64, device=grad.device) if ctx.needs_input_grad[1] else None
         for seg in reversed(range(len(ctx.ckpts))):
             start = seg * every
             stop = min(start + every, nsteps)
@@ -71,16 +77,24 @@ class _Integrate(torch.autograd.Function):
                 _advance(step, uh)
                 states.append(uh.clone())
             for y in reversed(states):
-                lam = step_vjp(step, y, lam)
+                lam = step_vjp(step, y, lam, dpow)
         ctx.ckpts = None
-        return lam, None, None, None
+        gp = None if dpow is None else dpow.reshape(()).to(torch.float64)
+        return lam, gp, None, None, None
 
 
-def integrate(step, u_hat0, nsteps, every=10):
-    """nsteps RK4 steps of u_hat0, differentiable with checkpoints every ``every`` steps."""
+def integrate(step, u_hat0, nsteps, every=10, power=None):
+    """nsteps RK4 steps of u_hat0, differentiable with checkpoints every ``every`` steps.
+
+    ``power``: optional 0-d float64 tensor equal to the plan's injection power; when it requires grad,
+    backward returns dL/dP for a fixed-power plan (the forward uses the plan's own value)."""
     if nsteps < 1 or every < 1:
         raise ValueError("need nsteps >= 1 and every >= 1")
-    return _Integrate.apply(u_hat0, step, int(nsteps), int(every))
+    if power is None:
+        power = torch.zeros((), dtype=torch.float64, device=u_hat0.device)
+    elif abs(float(power) - float(step.plan.desc.power)) > 1e-12 * max(1.0, abs(float(power))):
+        raise ValueError("power must equal the plan's injection power (create a plan per value)")
+    return _Integrate.apply(u_hat0, power, step, int(nsteps), int(every))
 
 
 class _Irfftn(torch.autograd.Function):

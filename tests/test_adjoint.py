@@ -16,11 +16,17 @@ def _inner(a, b):
     return float(torch.sum(a.conj() * b).real)
 
 
-def _setup(N, prec, seed=3):
+def _setup(N, prec, seed=3, power=None):
     from l3es_amd import set_up_solver
 
-    _, uh, _, _, step, _, _, _, _ = set_up_solver(N, 0.02, 3, "HIT", 5e-3, seed, 8, None, forcing_type="none",
-                                                  precision=prec, warmup=False)
+    if power is None:
+        kw = dict(forcing_type="none")
+        kf = None
+    else:
+        kw = dict(forcing_type="power", power=power)
+        kf = 2
+    _, uh, _, _, step, _, _, _, _ = set_up_solver(N, 0.02, 3, "HIT", 5e-3, seed, 8, kf, precision=prec,
+                                                  warmup=False, **kw)
     return uh, step
 
 
@@ -47,10 +53,11 @@ def _valid_dir(uh, step, seed):
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec,tol", [("float64", 1e-10), ("float32", 1e-5)])
 @pytest.mark.parametrize("which", ["rhs", "step"])
-def test_dot_product(gpu, prec, tol, which):
+@pytest.mark.parametrize("power", [None, 0.3])
+def test_dot_product(gpu, prec, tol, which, power):
     from l3es_amd import adjoint as A
 
-    uh, step = _setup(16, prec)
+    uh, step = _setup(16, prec, power=power)
     v, w = _rand_hat(uh, 1), _rand_hat(uh, 2)
     jvp, vjp = (A.rhs_jvp, A.rhs_vjp) if which == "rhs" else (A.step_jvp, A.step_vjp)
     lhs = _inner(jvp(step, uh, v), w)
@@ -60,10 +67,11 @@ def test_dot_product(gpu, prec, tol, which):
 
 
 @pytest.mark.gpu
-def test_step_jvp_matches_central_difference(gpu):
+@pytest.mark.parametrize("power", [None, 0.3])
+def test_step_jvp_matches_central_difference(gpu, power):
     from l3es_amd import adjoint as A
 
-    uh, step = _setup(16, "float64")
+    uh, step = _setup(16, "float64", power=power)
     v = _valid_dir(uh, step, 5)
     eps = 1e-4
 
@@ -79,12 +87,12 @@ def test_step_jvp_matches_central_difference(gpu):
 
 
 @pytest.mark.gpu
-def test_rhs_vjp_rejects_forced_plans(gpu):
+def test_rhs_vjp_rejects_ekin_tot_plans(gpu):
     from l3es_amd import _lib, set_up_solver
     from l3es_amd import adjoint as A
 
-    _, uh, _, _, step, _, _, _, _ = set_up_solver(16, 0.02, 3, "HIT", 5e-3, 3, 8, 2, forcing_type="power",
-                                                  precision="float64", warmup=False)
+    _, uh, _, _, step, _, _, _, _ = set_up_solver(16, 0.02, 3, "HIT", 5e-3, 3, 8, 2, forcing_type="ekin_tot",
+                                                  e_kin_target=0.5, precision="float64", warmup=False)
     with pytest.raises(_lib.QllesError):
         A.rhs_vjp(step, uh, uh.clone())
 
@@ -113,6 +121,32 @@ def test_checkpointed_gradient_matches_central_difference(gpu):
         fd = (float(loss(uh0 + eps * v)) - float(loss(uh0 - eps * v))) / (2 * eps)
     an = _inner(grad, v)
     assert abs(fd - an) <= 1e-4 * abs(an), (fd, an)
+
+
+@pytest.mark.gpu
+def test_power_gradient_matches_central_difference(gpu):
+    """dL/dP of a fixed-power plan (the forcing-amplitude gradient) against plans at P +- eps."""
+    from l3es_amd import adjoint as A
+
+    P, eps, K, every = 0.3, 1e-3, 6, 4
+    uh0, step = _setup(16, "float64", seed=9, power=P)
+    _, step_p = _setup(16, "float64", seed=9, power=P + eps)
+    _, step_m = _setup(16, "float64", seed=9, power=P - eps)
+    g = torch.Generator(device="cpu").manual_seed(17)
+    with torch.no_grad():
+        target = A.irfftn(A.integrate(step, uh0, K, every), step.plan)
+        target = target + 0.05 * torch.randn(target.shape, generator=g, dtype=torch.float64).cuda()
+
+    def loss(st, p=None):
+        return torch.mean((A.irfftn(A.integrate(st, uh0, K, every, power=p), st.plan) - target) ** 2)
+
+    p = torch.tensor(P, dtype=torch.float64, device=uh0.device, requires_grad=True)
+    L = loss(step, p)
+    L.backward()
+    with torch.no_grad():
+        fd = (float(loss(step_p)) - float(loss(step_m))) / (2 * eps)
+    an = float(p.grad)
+    assert abs(fd - an) <= 1e-4 * abs(an) + 1e-12, (fd, an)
 
 
 @pytest.mark.gpu
