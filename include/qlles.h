@@ -218,6 +218,17 @@ int qlles_batch_get_config(const qlles_batch* b, int* lanes, int* chunk, double*
  * streams and joins back). */
 int qlles_batch_run(qlles_batch* b, const int* nsteps, int nsteps_all, void* stream);
 
+/* Adjoint of qlles_interp_hat's interpolation u_r(u_hat, r) (dt = 0; 3-D,
+ * NUFFT backend; no reference counterpart).  For the cotangent u_bar (np, 3)
+ * in the plan dtype it ADDS
+ *   lam_hat += (d u_r / d u_hat)^T u_bar   ((3,N,N,N/2+1) spectrum, the
+ *              adjoint of the filter + type-2 NUFFT: type-1 spreading with
+ *              the same kernel, forward fine-grid FFT, window adjoint)
+ *   r_bar   += (d u_r / d r)^T u_bar       ((np, 3) FP64, optional)
+ * with the inner products of qlles_step_vjp. */
+int qlles_interp_hat_vjp(qlles_interp* it, int N, const void* u_hat, const double* r, int np, const void* u_bar,
+                         void* lam_hat, double* r_bar, void* stream);
+
 /* message of the last failed call on this thread */
 const char* qlles_last_error(void);
 

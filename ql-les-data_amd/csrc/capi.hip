@@ -189,4 +189,12 @@ int qlles_interp_hat(qlles_interp* it, int N, const void* u_hat, double* r, int 
   return guarded([&] { ql::interp_hat(it, N, u_hat, N / 2 + 1, 0, r, np, u_r, dt, (hipStream_t)stream); });
 }
 
+int qlles_interp_hat_vjp(qlles_interp* it, int N, const void* u_hat, const double* r, int np, const void* u_bar,
+                         void* lam_hat, double* r_bar, void* stream) {
+  if (!it || !u_hat || !r || !u_bar || !lam_hat) return fail(QLLES_ERR_ARG, "null argument");
+  if (np < 0) return fail(QLLES_ERR_ARG, "np must be >= 0");
+  if (lam_hat == u_hat) return fail(QLLES_ERR_ARG, "lam_hat may not alias u_hat");
+  return guarded([&] { ql::interp_hat_vjp(it, N, u_hat, r, np, u_bar, lam_hat, r_bar, (hipStream_t)stream); });
+}
+
 }  // extern "C"

@@ -37,6 +37,8 @@ struct qlles_interp {
   void* coef;   // NUFFT: [ncomp][nf^d] zero-padded coefficients; DFT: [ncomp][n^d] shifted coefficients
   void* fine;   // NUFFT: [ncomp][nf^d] fine-grid values
   void* tmp;    // [ncomp][n^(d-1)][n/2+1] rfft of a physical n-grid field
+  void* adj_a;  // adjoint (3-D NUFFT, allocated on first use): [3][nf^3] complex
+  void* adj_b;  //   second [3][nf^3] complex buffer
 };
 
 namespace ql {
@@ -276,6 +278,174 @@ k_dft(const cx<T>* __restrict__ G, int n, const double* __restrict__ r, int np, 
   }
 }
 
+// ---- adjoint of the fused filter + type-2 NUFFT interpolation (3-D) ----
+//
+// Forward: u_r(p) = Re sum_{k in W} G_k e^{i k.r_p}, W = [-h, h)^3, h = n/2,
+//   G = H_n(crop(FS(u_hat))) / N^3  (FS: Hermitian full spectrum of the
+//   N-grid rfft state; H_n: the n-grid Hermitian projection that the
+//   reference's .real implies, pairing -h with -h),
+// evaluated as gather(ES weights) of ifft(pad(corr * G)) on the fine grid.
+// Adjoint for the cotangent ub (np x 3):
+//   fbar = spread(ub) (same ES weights), cbar = fft(fbar),
+//   Gbar_k = corr(k) cbar_k (k in W),
+//   lam_k += c_kz / N^3 * 0.5 (1[k in W] H_n(Gbar)_k + 1[-k in W] conj(H_n(Gbar)_{-k}))
+// for every stored half-spectrum entry k (c = 1 on the kz = 0 / N/2 planes,
+// 2 elsewhere: the adjoint of irfftn).  Position cotangent:
+//   rbar_{p,j} += sum_c ub_{p,c} Re sum_W (i k_j) G_k e^{i k.r_p}.
+
+// fine-grid spreading: one wavefront per particle, atomics into the real parts
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_spread3(cx<T>* __restrict__ f, int nf, int w, T beta, const double* __restrict__ r, const T* __restrict__ ub,
+          int np) {
+  const int lane = threadIdx.x & 63;
+  const int pid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (pid >= np) return;
+  const double h = 6.283185307179586476925286766559 / nf;
+  const double halfw = 0.5 * w;
+  const T two_over_w = (T)(2.0 / w);
+  double t[3];
+  int i0[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    t[d] = r[(size_t)pid * 3 + d] / h;
+    i0[d] = (int)ceil(t[d] - halfw);
+  }
+  const T u0 = ub[(size_t)pid * 3], u1 = ub[(size_t)pid * 3 + 1], u2 = ub[(size_t)pid * 3 + 2];
+  const size_t cs = (size_t)nf * nf * nf;
+  for (int q = lane; q < w * w; q += 64) {
+    const int dy = q / w, dz = q % w;
+    const int iy = i0[1] + dy, iz = i0[2] + dz;
+    const T wyz = es_kernel((T)(t[1] - iy) * two_over_w, beta) * es_kernel((T)(t[2] - iz) * two_over_w, beta);
+    const int jy = ((iy % nf) + nf) % nf, jz = ((iz % nf) + nf) % nf;
+    for (int dx = 0; dx < w; ++dx) {
+      const int ix = i0[0] + dx;
+      const T wx = es_kernel((T)(t[0] - ix) * two_over_w, beta) * wyz;
+      const int jx = ((ix % nf) + nf) % nf;
+      const size_t o = ((size_t)jx * nf + jy) * nf + jz;
+      atomicAdd(&f[o].re, wx * u0);
+      atomicAdd(&f[cs + o].re, wx * u1);
+      atomicAdd(&f[2 * cs + o].re, wx * u2);
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> gbar_at(const cx<T>* cb, const T* corr, int n, int nf, size_t cs, int c, int kx,
+                                          int ky, int kz) {
+  const int h = n / 2;
+  const size_t o = ((size_t)((kx + nf) % nf) * nf + (ky + nf) % nf) * nf + (kz + nf) % nf;
+  return cscale(cb[c * cs + o], corr[kx + h] * corr[ky + h] * corr[kz + h]);
+}
+// H_n(Gbar)_k: pairs k with -k, where -(-h) wraps to -h
+template <typename T>
+__device__ __forceinline__ cx<T> hn_gbar(const cx<T>* cb, const T* corr, int n, int nf, size_t cs, int c, int kx,
+                                          int ky, int kz) {
+  const int h = n / 2;
+  auto neg = [h](int v) { return v == -h ? -h : -v; };
+  const cx<T> a = gbar_at(cb, corr, n, nf, cs, c, kx, ky, kz);
+  const cx<T> b = cconj(gbar_at(cb, corr, n, nf, cs, c, neg(kx), neg(ky), neg(kz)));
+  return cscale(cadd(a, b), (T)0.5);
+}
+
+// one thread per stored entry (kx, ky in (-h-1, h], kz in [0, h]) x component
+template <typename T>
+__global__ void k_window_adjoint3(const cx<T>* __restrict__ cb, const T* __restrict__ corr, int n, int nf, int N,
+                                  cx<T>* __restrict__ lam) {
+  const int h = n / 2, Nh = N / 2 + 1;
+  const int ax = n + 1;  // kx, ky in [-h, h]
+  const size_t total = (size_t)3 * ax * ax * (h + 1);
+  const size_t cs = (size_t)nf * nf * nf, csl = (size_t)N * N * Nh;
+  const double inv_n3 = 1.0 / ((double)N * N * N);
+  for (size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x; m < total; m += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(m / ((size_t)ax * ax * (h + 1)));
+    const size_t q = m % ((size_t)ax * ax * (h + 1));
+    const int kx = (int)(q / ((size_t)ax * (h + 1))) - h, ky = (int)((q / (h + 1)) % ax) - h, kz = (int)(q % (h + 1));
+    // each stored entry once: its signed row indices are the fftfreq values
+    // (when n == N, +N/2 is the entry of -N/2)
+    const int ix = (kx + N) % N, iy = (ky + N) % N;
+    if (kfreq(ix, N) != kx || kfreq(iy, N) != ky) continue;
+    // -k on the N-periodic grid, as signed fftfreq values (the rfft axis: kz = N/2 -> -N/2)
+    const int nx = kfreq((N - ix) % N, N), ny = kfreq((N - iy) % N, N), nz = -kz;
+    // the rfft-axis entry kz = N/2 is the periodic index -N/2 (in W when n == N)
+    const int kzc = 2 * kz == N ? -kz : kz;
+    const bool in_w = kx < h && ky < h && kzc >= -h && kzc < h;  // k in W (kx, ky >= -h by construction)
+    const bool neg_w = nx >= -h && nx < h && ny >= -h && ny < h && nz >= -h && nz < h;
+    if (!in_w && !neg_w) continue;
+    cx<T> v = {0, 0};
+    if (in_w) v = cadd(v, hn_gbar(cb, corr, n, nf, cs, c, kx, ky, kzc));
+    if (neg_w) v = cadd(v, cconj(hn_gbar(cb, corr, n, nf, cs, c, nx, ny, nz)));
+    const double cw = (kz == 0 || 2 * kz == N) ? 1.0 : 2.0;
+    const size_t o = (size_t)c * csl + ((size_t)ix * N + iy) * Nh + kz;
+    lam[o] = cadd(lam[o], cscale(v, (T)(0.5 * cw * inv_n3)));
+  }
+}
+
+// coefficients of d/dr_j: out = i k_j coef on the padded fine-grid window
+template <typename T>
+__global__ void k_coef_deriv(const cx<T>* __restrict__ co, cx<T>* __restrict__ out, int nf, int j) {
+  const size_t cs = (size_t)nf * nf * nf, total = 3 * cs;
+  for (size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x; m < total; m += (size_t)gridDim.x * blockDim.x) {
+    const size_t o = m % cs;
+    const int i = j == 0 ? (int)(o / ((size_t)nf * nf)) : (j == 1 ? (int)((o / nf) % nf) : (int)(o % nf));
+    const int k = i < nf / 2 ? i : i - nf;
+    out[m] = cmuli(cscale(co[m], (T)k));
+  }
+}
+
+// rbar_{p,j} += sum_c ub_{p,c} (gathered derivative field c)
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_gather_dot3(const cx<T>* __restrict__ f, int nf, int w, T beta, const double* __restrict__ r, int np,
+              const T* __restrict__ ub, double* __restrict__ rbar, int j) {
+  const int lane = threadIdx.x & 63;
+  const int pid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (pid >= np) return;
+  const double h = 6.283185307179586476925286766559 / nf;
+  const double halfw = 0.5 * w;
+  const T two_over_w = (T)(2.0 / w);
+  double t[3];
+  int i0[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    t[d] = r[(size_t)pid * 3 + d] / h;
+    i0[d] = (int)ceil(t[d] - halfw);
+  }
+  const size_t cs = (size_t)nf * nf * nf;
+  T acc[3] = {0, 0, 0};
+  for (int q = lane; q < w * w; q += 64) {
+    const int dy = q / w, dz = q % w;
+    const int iy = i0[1] + dy, iz = i0[2] + dz;
+    const T wyz = es_kernel((T)(t[1] - iy) * two_over_w, beta) * es_kernel((T)(t[2] - iz) * two_over_w, beta);
+    const int jy = ((iy % nf) + nf) % nf, jz = ((iz % nf) + nf) % nf;
+    for (int dx = 0; dx < w; ++dx) {
+      const int ix = i0[0] + dx;
+      const T wx = es_kernel((T)(t[0] - ix) * two_over_w, beta) * wyz;
+      const int jx = ((ix % nf) + nf) % nf;
+      const size_t o = ((size_t)jx * nf + jy) * nf + jz;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) acc[c] += wx * f[c * cs + o].re;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
+  }
+  if (lane == 0) {
+    double dot = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dot += (double)ub[(size_t)pid * 3 + c] * (double)acc[c];
+    rbar[(size_t)pid * 3 + j] += dot;
+  }
+}
+
+template <typename T>
+__global__ void k_zero_cx(cx<T>* p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = cx<T>{0, 0};
+}
+
 size_t tsize(int dtype) { return dtype == QLLES_F32 ? sizeof(float) : sizeof(double); }
 
 template <typename T>
@@ -429,6 +599,8 @@ int interp_n(const qlles_interp* it) { return it->n; }
 
 void destroy_interp(qlles_interp* it) {
   if (!it) return;
+  if (it->adj_a) (void)hipFree(it->adj_a);
+  if (it->adj_b) (void)hipFree(it->adj_b);
   if (it->corr) (void)hipFree(it->corr);
   if (it->coef) (void)hipFree(it->coef);
   if (it->fine) (void)hipFree(it->fine);
@@ -441,6 +613,65 @@ void interp_field(qlles_interp* it, const void* u, const double* r, int np, void
     field_t<float>(it, static_cast<const float*>(u), r, np, static_cast<float*>(u_r), s);
   else
     field_t<double>(it, static_cast<const double*>(u), r, np, static_cast<double*>(u_r), s);
+}
+
+template <typename T>
+void hat_vjp_t(qlles_interp* it, int N, const cx<T>* uh, const double* r, int np, const T* ub, cx<T>* lam,
+               double* rbar, hipStream_t s) {
+  const int n = it->n, nf = it->nf;
+  const size_t cs = (size_t)nf * nf * nf;
+  if (!it->adj_a) {
+    QL_HIP(hipMalloc(&it->adj_a, sizeof(cx<T>) * 3 * cs));
+    QL_HIP(hipMalloc(&it->adj_b, sizeof(cx<T>) * 3 * cs));
+  }
+  cx<T>* fa = static_cast<cx<T>*>(it->adj_a);
+  cx<T>* fb = static_cast<cx<T>*>(it->adj_b);
+  const cx<T>* twf = twiddles<T>(nf);
+  const int nbz = (int)std::min<size_t>((3 * cs + 255) / 256, 8192);
+  // spectrum cotangent: spread, forward fine-grid FFT, window adjoint
+  hipLaunchKernelGGL((k_zero_cx<T>), dim3(nbz), dim3(256), 0, s, fa, 3 * cs);
+  QL_LAUNCHED();
+  if (np > 0) {
+    hipLaunchKernelGGL((k_spread3<T>), dim3((np + 3) / 4), dim3(256), 0, s, fa, nf, it->w, (T)it->beta, r, ub, np);
+    QL_LAUNCHED();
+  }
+  c2c_strided<T>(nf, fa, fa, 3, nf * nf, true, (T)1, twf, s);
+  c2c_strided<T>(nf, fa, fa, 3 * nf, nf, true, (T)1, twf, s);
+  c2c_contig<T>(nf, fa, fa, 3 * nf * nf, true, (T)1, twf, s);
+  const size_t nw = (size_t)3 * (n + 1) * (n + 1) * (n / 2 + 1);
+  hipLaunchKernelGGL((k_window_adjoint3<T>), dim3((int)std::min<size_t>((nw + 255) / 256, 8192)), dim3(256), 0, s,
+                     fa, static_cast<const T*>(it->corr), n, nf, N, lam);
+  QL_LAUNCHED();
+  if (rbar == nullptr || np == 0) return;
+  // position cotangent: derivative coefficients of the forward interpolant
+  const size_t per_n = (size_t)n * n * n;
+  hipLaunchKernelGGL((k_build_coef<T>), dim3((int)std::min<size_t>((per_n * 3 + 255) / 256, 8192)), dim3(256), 0,
+                     s, uh, N, N / 2 + 1, (size_t)0, 3, n, 1.0 / ((double)N * N * N),
+                     static_cast<const T*>(it->corr), static_cast<cx<T>*>(it->coef), 0, nf);
+  QL_LAUNCHED();
+  for (int j = 0; j < 3; ++j) {
+    hipLaunchKernelGGL((k_coef_deriv<T>), dim3(nbz), dim3(256), 0, s, static_cast<const cx<T>*>(it->coef), fb, nf, j);
+    QL_LAUNCHED();
+    c2c_strided<T>(nf, fb, fb, 3, nf * nf, false, (T)1, twf, s);
+    c2c_strided<T>(nf, fb, fb, 3 * nf, nf, false, (T)1, twf, s);
+    c2c_contig<T>(nf, fb, fb, 3 * nf * nf, false, (T)1, twf, s);
+    hipLaunchKernelGGL((k_gather_dot3<T>), dim3((np + 3) / 4), dim3(256), 0, s, fb, nf, it->w, (T)it->beta, r, np,
+                       ub, rbar, j);
+    QL_LAUNCHED();
+  }
+}
+
+void interp_hat_vjp(qlles_interp* it, int N, const void* u_hat, const double* r, int np, const void* u_bar,
+                    void* lam_hat, double* r_bar, hipStream_t s) {
+  if (it->dim != 3 || it->backend != QLLES_INTERP_NUFFT)
+    throw std::invalid_argument("the interpolation adjoint is implemented for the 3-D NUFFT backend");
+  if (N < it->n || (N % 2) != 0) throw std::invalid_argument("solver grid N must be even and >= n");
+  if (it->dtype == QLLES_F32)
+    hat_vjp_t<float>(it, N, static_cast<const cx<float>*>(u_hat), r, np, static_cast<const float*>(u_bar),
+                     static_cast<cx<float>*>(lam_hat), r_bar, s);
+  else
+    hat_vjp_t<double>(it, N, static_cast<const cx<double>*>(u_hat), r, np, static_cast<const double*>(u_bar),
+                      static_cast<cx<double>*>(lam_hat), r_bar, s);
 }
 
 void interp_hat(qlles_interp* it, int N, const void* u_hat, int ld, size_t q, double* r, int np, void* u_r,

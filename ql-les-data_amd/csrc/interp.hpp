@@ -16,4 +16,8 @@ int interp_n(const qlles_interp* it);
 // for a 3-D field, in a separate [ncomp][N][N] plane at element q (ld >= N/2)
 void interp_hat(qlles_interp* it, int N, const void* u_hat, int ld, size_t q, double* r, int np, void* u_r,
                 double dt, hipStream_t s);
+// adjoint of u_r = interp_hat(u_hat, r) (3-D NUFFT): lam_hat += (du_r/du_hat)^T u_bar,
+// r_bar (optional) += (du_r/dr)^T u_bar
+void interp_hat_vjp(qlles_interp* it, int N, const void* u_hat, const double* r, int np, const void* u_bar,
+                    void* lam_hat, double* r_bar, hipStream_t s);
 }  // namespace ql

@@ -33,7 +33,7 @@ EXPORTS = (
     "qlles_step_advect", "qlles_batch_create", "qlles_batch_destroy", "qlles_batch_add",
     "qlles_batch_configure", "qlles_batch_tune", "qlles_batch_get_config", "qlles_batch_run",
     "qlles_rhs_jvp", "qlles_rhs_vjp", "qlles_step_jvp", "qlles_step_vjp", "qlles_irfftn_adjoint",
-    "qlles_rfftn_adjoint",
+    "qlles_rfftn_adjoint", "qlles_interp_hat_vjp",
 )
 KINDS = ("x-pass", "y-inverse", "z-pass", "y-forward", "other")
 
@@ -102,6 +102,7 @@ def load():
         "qlles_step_vjp": (ci, [vp, vp, vp, vp, vp, vp]),
         "qlles_irfftn_adjoint": (ci, [vp, vp, vp, vp]),
         "qlles_rfftn_adjoint": (ci, [vp, vp, vp, vp]),
+        "qlles_interp_hat_vjp": (ci, [vp, ci, vp, vp, ci, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -122,3 +122,80 @@ def irfftn(u_hat, plan):
     N = u_hat.shape[1]
     shape = (3, N, N, N) if u_hat.dim() == 4 else (3, N, N)
     return _Irfftn.apply(u_hat, plan, shape)
+
+
+# ---------------------------------------------------------------------------
+# particles: combined-mode loop (combined.py:97,127,130,199)
+#   u_hat_{n+1} = step(u_hat_n);  r_{n+1} = (r_n + dt interp(filter(u_hat_{n+1}), r_n)) mod 2 pi
+# ---------------------------------------------------------------------------
+def interp_vjp(interp, u_hat, r, u_bar, lam=None, r_bar=None):
+    """Adjoint of u_r = interpolate(filter(irfftn(u_hat)), r): adds (du_r/du_hat)^T u_bar to ``lam`` and
+    (du_r/dr)^T u_bar to ``r_bar`` (both created when None); returns (lam, r_bar)."""
+    if lam is None:
+        lam = torch.zeros_like(u_hat)
+    if r_bar is None:
+        r_bar = torch.zeros_like(r, dtype=torch.float64)
+    _lib.check(_lib.load().qlles_interp_hat_vjp(interp.it.handle, u_hat.shape[1], _lib.ptr(u_hat.contiguous()),
+                                                _lib.ptr(r.contiguous()), r.shape[0],
+                                                _lib.ptr(u_bar.to(interp.dtype).contiguous()), _lib.ptr(lam),
+                                                _lib.ptr(r_bar), _lib.stream_ptr()))
+    return lam, r_bar
+
+
+class _IntegrateParticles(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, u_hat0, r0, power, step, interp, nsteps, every):
+        dt = float(step.plan.desc.dt)
+        uh = u_hat0.detach().contiguous().clone()
+        r = r0.detach().to(torch.float64).contiguous().clone()
+        ckpts, rs = [], [r.clone()]
+        for n in range(nsteps):
+            if n % every == 0:
+                ckpts.append(uh.clone())
+            _advance(step, uh)
+            interp.from_hat(uh, r, dt)  # r <- (r + dt u_r) mod 2 pi, in place
+            rs.append(r.clone())
+        ctx.step, ctx.interp, ctx.nsteps, ctx.every, ctx.dt = step, interp, nsteps, every, dt
+        ctx.ckpts, ctx.rs = ckpts, rs
+        return uh, r
+
+    @staticmethod
+    def backward(ctx, g_uh, g_r):
+        step, interp, nsteps, every, dt = ctx.step, ctx.interp, ctx.nsteps, ctx.every, ctx.dt
+        lam = torch.zeros_like(ctx.ckpts[0]) if g_uh is None else g_uh.contiguous().clone()
+        lr = torch.zeros_like(ctx.rs[0]) if g_r is None else g_r.to(torch.float64).contiguous().clone()
+        dpow = torch.zeros(1, dtype=torch.float64, device=lam.device) if ctx.needs_input_grad[2] else None
+        for seg in reversed(range(len(ctx.ckpts))):
+            start = seg * every
+            stop = min(start + every, nsteps)
+            states = [ctx.ckpts[seg]]
+            uh = ctx.ckpts[seg].clone()
+            for _ in range(start, stop):  # u_hat_start .. u_hat_stop
+                _advance(step, uh)
+                states.append(uh.clone())
+            for n in reversed(range(start, stop)):
+                y_next, y = states[n - start + 1], states[n - start]
+                # r_{n+1} = r_n + dt u_r(u_hat_{n+1}, r_n)
+                lam, lr = interp_vjp(interp, y_next, ctx.rs[n], dt * lr, lam, lr.clone())
+                lam = step_vjp(step, y, lam, dpow)
+        ctx.ckpts = ctx.rs = None
+        gp = None if dpow is None else dpow.reshape(()).to(torch.float64)
+        return lam, lr, gp, None, None, None, None
+
+
+def integrate_particles(step, interp, u_hat0, r0, nsteps, every=10, power=None):
+    """The combined-mode loop for nsteps: returns (u_hat_K, r_K), differentiable with respect to u_hat0,
+    r0 and (fixed-power plans) the injection power, with checkpoints every ``every`` steps."""
+    if nsteps < 1 or every < 1:
+        raise ValueError("need nsteps >= 1 and every >= 1")
+    if power is None:
+        power = torch.zeros((), dtype=torch.float64, device=u_hat0.device)
+    elif abs(float(power) - float(step.plan.desc.power)) > 1e-12 * max(1.0, abs(float(power))):
+        raise ValueError("power must equal the plan's injection power (create a plan per value)")
+    return _IntegrateParticles.apply(u_hat0, r0, power, step, interp, int(nsteps), int(every))
+
+
+def periodic_diff(a, b):
+    """a - b wrapped to [-pi, pi) (positions live on the 2 pi torus)."""
+    two_pi = 2 * torch.pi
+    return torch.remainder(a - b + torch.pi, two_pi) - torch.pi

@@ -171,3 +171,88 @@ def test_irfftn_adjoint_pairs(gpu):
     assert abs(l1 - r1) <= 1e-12 * max(1.0, abs(l1))
     assert abs(l2 - r2) <= 1e-12 * max(1.0, abs(l2)) * 16 ** 3
     assert np.isfinite(l1) and np.isfinite(l2)
+
+
+def _interp(n, prec, tol=1e-9):
+    from l3es_amd.integrator import spectral_interpolator_wrapper
+
+    return spectral_interpolator_wrapper(n, (1, 2, 3), backend="nufft", precision=prec, tol=tol)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,tol", [("float64", 1e-10), ("float32", 1e-5)])
+@pytest.mark.parametrize("N,n", [(16, 8), (16, 16)])
+def test_interp_vjp_dot_product(gpu, prec, tol, N, n):
+    """<I v, ub> = <v, I^T ub> for the filter + NUFFT interpolation, as a linear map of the spectrum."""
+    from l3es_amd import adjoint as A
+
+    uh, step = _setup(N, prec)
+    it = _interp(n, prec, 1e-9 if prec == "float64" else 1e-6)
+    g = torch.Generator(device="cpu").manual_seed(31)
+    r = (torch.rand((200, 3), generator=g, dtype=torch.float64) * 2 * np.pi).cuda()
+    v = _rand_hat(uh, 32)
+    ub = torch.randn((200, 3), generator=g, dtype=torch.float64).to(it.dtype).cuda()
+    iv = it.from_hat(v, r.clone())
+    lam, _ = A.interp_vjp(it, uh, r, ub)
+    lhs = float(torch.sum(iv.to(torch.float64) * ub.to(torch.float64)))
+    rhs = _inner(v, lam)
+    scale = float(torch.linalg.vector_norm(iv)) * float(torch.linalg.vector_norm(ub))
+    assert abs(lhs - rhs) <= tol * scale, (lhs, rhs)
+
+
+@pytest.mark.gpu
+def test_interp_position_gradient_matches_central_difference(gpu):
+    from l3es_amd import adjoint as A
+
+    uh, step = _setup(16, "float64")
+    it = _interp(8, "float64")
+    g = torch.Generator(device="cpu").manual_seed(41)
+    r = (torch.rand((64, 3), generator=g, dtype=torch.float64) * 2 * np.pi).cuda()
+    ub = torch.randn((64, 3), generator=g, dtype=torch.float64).cuda()
+    _, rbar = A.interp_vjp(it, uh, r, ub)
+    eps = 1e-5
+    for j in range(3):
+        e = torch.zeros_like(r)
+        e[:, j] = eps
+        fp = it.from_hat(uh, (r + e).clone())
+        fm = it.from_hat(uh, (r - e).clone())
+        fd = torch.sum((fp - fm) * ub, dim=1) / (2 * eps)
+        err = float(torch.max(torch.abs(fd - rbar[:, j])) / torch.max(torch.abs(rbar[:, j])))
+        assert err < 1e-5, (j, err)
+
+
+@pytest.mark.gpu
+def test_particle_loss_gradients_match_central_difference(gpu):
+    """Loss = MSE of final particle positions vs a perturbed target after K combined-mode steps:
+    gradients in u_hat0 and in the injection power against central differences (eps = 1e-3)."""
+    from l3es_amd import adjoint as A
+    from l3es_amd.integrator import pos_init_cartesian
+
+    P, K, every = 0.3, 5, 2
+    uh0, step = _setup(16, "float64", seed=5, power=P)
+    _, step_p = _setup(16, "float64", seed=5, power=P + 1e-3)
+    _, step_m = _setup(16, "float64", seed=5, power=P - 1e-3)
+    it = _interp(8, "float64")
+    r0 = torch.from_numpy(pos_init_cartesian(8, 3)).cuda()
+    g = torch.Generator(device="cpu").manual_seed(43)
+    with torch.no_grad():
+        _, rK = A.integrate_particles(step, it, uh0, r0, K, every)
+        target = rK + 0.01 * torch.randn(rK.shape, generator=g, dtype=torch.float64).cuda()
+
+    def loss(st, x, p=None):
+        _, r = A.integrate_particles(st, it, x, r0, K, every, power=p)
+        return torch.mean(A.periodic_diff(r, target) ** 2)
+
+    x = uh0.clone().requires_grad_(True)
+    p = torch.tensor(P, dtype=torch.float64, device=uh0.device, requires_grad=True)
+    L = loss(step, x, p)
+    L.backward()
+    v = _valid_dir(uh0, step, 47)
+    eps = 1e-3
+    with torch.no_grad():
+        fd_u = (float(loss(step, uh0 + eps * v)) - float(loss(step, uh0 - eps * v))) / (2 * eps)
+        fd_p = (float(loss(step_p, uh0)) - float(loss(step_m, uh0))) / (2 * 1e-3)
+    an_u = _inner(x.grad, v)
+    an_p = float(p.grad)
+    assert abs(fd_u - an_u) <= 1e-4 * abs(an_u), (fd_u, an_u)
+    assert abs(fd_p - an_p) <= 1e-4 * abs(an_p) + 1e-14, (fd_p, an_p)
