@@ -215,43 +215,47 @@ def run_batch(args):
 
 
 def run_adjoint(args):
-    """Differentiable 3-D HIT: gradient of an MSE loss after K RK4 steps with respect to the initial
-    spectrum, through the checkpointed step adjoint (forward K fused steps, backward K step VJPs)."""
+    """Differentiable forced HIT with particles: gradient of the MSE between final particle positions and a
+    seeded perturbed target, with respect to the initial spectrum and the forcing amplitude, through the
+    checkpointed adjoint of the combined-mode loop (K fused steps + NUFFT advection, then K step and
+    interpolation adjoints)."""
     import torch
 
     from l3es_amd import adjoint as A
     from l3es_amd import set_up_solver
+    from l3es_amd.integrator import pos_init_cartesian, spectral_interpolator_wrapper
 
     torch.cuda.set_device(0)
-    N, K, every = args.adj_N, args.adj_K, args.adj_every
-    _, uh0, _, _, step, _, _, _, _ = set_up_solver(N, args.nu, 3, "HIT", args.dt, args.seed, 16, None,
-                                                   e_kin_target=None, forcing_type="none", spectrum="k4exp",
-                                                   precision="float32", warmup=False)
+    N, K, every, n = args.adj_N, args.adj_K, args.adj_every, args.adj_n
+    _, uh0, _, _, step, _, _, _, _ = set_up_solver(N, args.nu, 3, "HIT", args.dt, args.seed, n, args.kf,
+                                                   e_kin_target=args.e_kin, forcing_type="power",
+                                                   power=args.power, spectrum="k4exp", precision="float32",
+                                                   warmup=False)
+    it = spectral_interpolator_wrapper(n, (1, 2, 3), backend="nufft", precision="float32", tol=args.tol)
+    r0 = torch.from_numpy(pos_init_cartesian(n, 3)).cuda()
     g = torch.Generator(device="cpu").manual_seed(args.seed + 1)
     with torch.no_grad():
-        pert = uh0 + 0.01 * (uh0.abs().max()) * torch.complex(
-            torch.randn(uh0.shape, generator=g), torch.randn(uh0.shape, generator=g)).to(uh0.device)
-        target = A.irfftn(A.integrate(step, pert, K, every), step.plan)
+        _, rK = A.integrate_particles(step, it, uh0, r0, K, every)
+        target = rK + 0.01 * torch.randn(rK.shape, generator=g, dtype=torch.float64).cuda()
 
     def grad_eval():
         x = uh0.clone().requires_grad_(True)
+        p = torch.tensor(args.power, dtype=torch.float64, device=uh0.device, requires_grad=True)
         t0 = time.perf_counter()
-        uK = A.integrate(step, x, K, every)
-        loss = torch.mean((A.irfftn(uK, step.plan) - target) ** 2)
+        _, r = A.integrate_particles(step, it, x, r0, K, every, power=p)
+        loss = torch.mean(A.periodic_diff(r, target) ** 2)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         loss.backward()
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        return float(loss), t1 - t0, t2 - t1, x.grad
+        return float(loss.detach()), t1 - t0, t2 - t1, x.grad, float(p.grad)
 
     grad_eval()  # warm-up (workspace allocation)
-    reps = []
-    for _ in range(args.adj_reps):
-        reps.append(grad_eval())
+    reps = [grad_eval() for _ in range(args.adj_reps)]
     fwd = min(r[1] for r in reps)
     bwd = min(r[2] for r in reps)
-    grad = reps[-1][3]
+    grad, gpow = reps[-1][3], reps[-1][4]
     # dot-product residual of the FP32 step maps at this size
     v = torch.complex(torch.randn(uh0.shape, generator=g), torch.randn(uh0.shape, generator=g)).to(uh0.device)
     w = torch.complex(torch.randn(uh0.shape, generator=g), torch.randn(uh0.shape, generator=g)).to(uh0.device)
@@ -259,18 +263,21 @@ def run_adjoint(args):
     lhs = float(torch.sum(jv.conj() * w).real)
     rhs = float(torch.sum(v.conj() * A.step_vjp(step, uh0, w)).real)
     dp = abs(lhs - rhs) / (float(torch.linalg.vector_norm(jv)) * float(torch.linalg.vector_norm(w)))
-    log(f"[bench] adjoint {N}^3 K={K}: forward {1e3 * fwd:.1f} ms, backward {1e3 * bwd:.1f} ms, dot-product {dp:.2e}")
+    log(f"[bench] adjoint {N}^3 K={K}: forward {1e3 * fwd:.1f} ms, backward {1e3 * bwd:.1f} ms, "
+        f"dL/dP={gpow:.3e}, dot-product {dp:.2e}")
     out = {
-        "metric": f"gradient evaluations/s, differentiable HIT {N}^3 FP32, K={K} RK4 steps",
+        "metric": f"gradient evaluations/s, differentiable forced HIT {N}^3 FP32 with {n}^3 particles, K={K}",
         "value": 1.0 / (fwd + bwd), "unit": "gradients/s", "n_gpus": 1, "steps": K, "warmup": 1,
         "ms_per_step": 1e3 * (fwd + bwd) / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"loss = mean((irfftn(u_hat_K) - target)^2), target from a perturbed initial "
-                               f"spectrum; checkpoints every {every} steps; nu={args.nu}, dt={args.dt}, "
-                               f"Rogallo k^4 exp(-2(k/4)^2) init, forcing none",
-                   "N": N, "K": K, "checkpoint_every": every},
+        "config": {"workload": f"loss = mean(periodic(r_K - target)^2) over {n}^3 particles advected by NUFFT "
+                               f"(tol {args.tol}) through K RK4 steps of power-forced HIT (P={args.power}, "
+                               f"kf={args.kf}); target = seeded perturbation of r_K; gradients with respect to "
+                               f"the initial spectrum and P; checkpoints every {every} steps",
+                   "N": N, "K": K, "checkpoint_every": every, "particles": n ** 3},
         "forward_ms": 1e3 * fwd, "backward_ms": 1e3 * bwd, "backward_over_forward": bwd / fwd,
-        "loss": reps[-1][0], "grad_finite": bool(torch.isfinite(grad).all()), "dot_product_rel_residual": dp,
+        "loss": reps[-1][0], "dL_dpower": gpow, "grad_finite": bool(torch.isfinite(grad).all()),
+        "dot_product_rel_residual": dp,
     }
     print(json.dumps(out), flush=True)
 
@@ -331,6 +338,7 @@ def main():
     ap.add_argument("--adj-K", dest="adj_K", type=int, default=50)
     ap.add_argument("--adj-every", dest="adj_every", type=int, default=10)
     ap.add_argument("--adj-reps", dest="adj_reps", type=int, default=3)
+    ap.add_argument("--adj-n", dest="adj_n", type=int, default=16)
     ap.add_argument("--batch-members", dest="batch_members", type=int, default=24)
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()

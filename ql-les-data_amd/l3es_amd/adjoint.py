@@ -92,7 +92,7 @@ def integrate(step, u_hat0, nsteps, every=10, power=None):
         raise ValueError("need nsteps >= 1 and every >= 1")
     if power is None:
         power = torch.zeros((), dtype=torch.float64, device=u_hat0.device)
-    elif abs(float(power) - float(step.plan.desc.power)) > 1e-12 * max(1.0, abs(float(power))):
+    elif abs(float(power.detach()) - float(step.plan.desc.power)) > 1e-12 * max(1.0, abs(float(power.detach()))):
         raise ValueError("power must equal the plan's injection power (create a plan per value)")
     return _Integrate.apply(u_hat0, power, step, int(nsteps), int(every))
 
@@ -190,7 +190,7 @@ def integrate_particles(step, interp, u_hat0, r0, nsteps, every=10, power=None):
         raise ValueError("need nsteps >= 1 and every >= 1")
     if power is None:
         power = torch.zeros((), dtype=torch.float64, device=u_hat0.device)
-    elif abs(float(power) - float(step.plan.desc.power)) > 1e-12 * max(1.0, abs(float(power))):
+    elif abs(float(power.detach()) - float(step.plan.desc.power)) > 1e-12 * max(1.0, abs(float(power.detach()))):
         raise ValueError("power must equal the plan's injection power (create a plan per value)")
     return _IntegrateParticles.apply(u_hat0, r0, power, step, interp, int(nsteps), int(every))
 
