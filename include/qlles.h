@@ -76,6 +76,17 @@ int qlles_plan_destroy(qlles_plan* plan);
 /* bytes of device workspace held by the plan */
 size_t qlles_plan_workspace_bytes(const qlles_plan* plan);
 
+/* Kernel variants (no reference counterpart; the reference leaves FFT
+ * planning to XLA): xv picks the x / y-pass tile width (16 or 8 columns) and
+ * zv the radix plan of the z-pass (2-D: y-pass) half-length FFTs.  Grids up
+ * to 128 have 2 choices each, larger grids 1 (*nvar).  Variants change the
+ * rounding of the FFTs, not the algorithm.  qlles_plan_tune times every
+ * (xv, zv) over trial_steps steps on a scratch copy of u_hat (u_hat does not
+ * move) and keeps the fastest; *best_us (optional) = its time per step. */
+int qlles_plan_set_variant(qlles_plan* plan, int xv, int zv);
+int qlles_plan_get_variant(const qlles_plan* plan, int* xv, int* zv, int* nvar);
+int qlles_plan_tune(qlles_plan* plan, const void* u_hat, int trial_steps, double* best_us, void* stream);
+
 /* `nsteps` applications of step_fn(u, u_hat) -> (u, u_hat, e_inj)
  * (spectral_solver.py:73-118).  u_hat is updated in place.
  *   u_in   : optional physical field used by the first RK stage (the
@@ -205,14 +216,18 @@ int qlles_batch_destroy(qlles_batch* b);
  * (optional) receives the member's position. */
 int qlles_batch_add(qlles_batch* b, qlles_plan* plan, qlles_interp* it, void* u_hat, double* r, int np, void* u_r,
                     int* index);
-/* manual configuration: 1 <= lanes <= 16 streams, 1 <= chunk <= 1000 steps per graph */
-int qlles_batch_configure(qlles_batch* b, int lanes, int chunk);
-/* autotuner: times lanes in {1,2,4,8,16} x chunk in {1,4,16,64} over about
- * trial_steps steps per member on scratch copies of the member states (the
- * trajectories do not move) and keeps the fastest configuration */
+/* manual configuration: 1 <= lanes <= 16 streams, 1 <= chunk <= 1000 steps
+ * per graph; single_graph != 0 captures every member's chunk into ONE graph
+ * (parallel branches over the lanes), launched once per chunk while every
+ * member still needs a whole chunk */
+int qlles_batch_configure(qlles_batch* b, int lanes, int chunk, int single_graph);
+/* autotuner: for each member, its plan's kernel variants (qlles_plan_tune);
+ * then single_graph in {0,1} x lanes in {1,2,4,8,16} x chunk in {1,4,16,64},
+ * timed over about trial_steps steps per member on scratch copies of the
+ * member states (the trajectories do not move); keeps the fastest */
 int qlles_batch_tune(qlles_batch* b, int trial_steps, void* stream);
 /* current configuration and the tuned time per trajectory-step (0 if untuned) */
-int qlles_batch_get_config(const qlles_batch* b, int* lanes, int* chunk, double* us_per_traj_step);
+int qlles_batch_get_config(const qlles_batch* b, int* lanes, int* chunk, int* single_graph, double* us_per_traj_step);
 /* nsteps: per-member step counts (NULL: every member takes nsteps_all).
  * Stream-ordered with respect to `stream` (work forks onto the batch's
  * streams and joins back). */

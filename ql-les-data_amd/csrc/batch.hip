@@ -30,6 +30,8 @@ struct qlles_batch {
   std::vector<Member> m;
   int lanes = 1;  // streams the members are spread over (member i -> stream i % lanes)
   int chunk = 1;  // steps per graph launch
+  int single = 0;  // 1: one graph holds every member's chunk as parallel branches
+  hipGraphExec_t all = nullptr;
   std::vector<hipStream_t> streams;
   std::vector<hipEvent_t> join;
   hipEvent_t fork = nullptr;
@@ -45,6 +47,10 @@ void drop_graphs(qlles_batch* b) {
       (void)hipGraphExecDestroy(x.g);
       x.g = nullptr;
     }
+  if (b->all) {
+    (void)hipGraphExecDestroy(b->all);
+    b->all = nullptr;
+  }
   b->built = false;
 }
 
@@ -89,6 +95,38 @@ hipGraphExec_t capture(const qlles_batch::Member& x, void* uh, double* r, void* 
   return exec;
 }
 
+// one graph: the first stream forks to the others, every member's chunk is
+// captured on its stream, and the streams join back
+hipGraphExec_t capture_all(qlles_batch* b) {
+  for (const auto& x : b->m)
+    if (x.p->eng->profiling()) throw std::invalid_argument("disable qlles_profile on batch members");
+  hipStream_t o = b->streams[0];
+  QL_HIP(hipStreamBeginCapture(o, hipStreamCaptureModeThreadLocal));
+  hipGraph_t graph = nullptr;
+  try {
+    QL_HIP(hipEventRecord(b->fork, o));
+    for (int l = 1; l < b->lanes; ++l) QL_HIP(hipStreamWaitEvent(b->streams[l], b->fork, 0));
+    for (size_t i = 0; i < b->m.size(); ++i) {
+      const auto& x = b->m[i];
+      qlapi::run_steps(x.p, x.it, nullptr, x.uh, b->chunk, x.r, x.np, x.ur, nullptr, b->streams[i % b->lanes]);
+    }
+    for (int l = 1; l < b->lanes; ++l) {
+      QL_HIP(hipEventRecord(b->join[l], b->streams[l]));
+      QL_HIP(hipStreamWaitEvent(o, b->join[l], 0));
+    }
+  } catch (...) {
+    (void)hipStreamEndCapture(o, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  QL_HIP(hipStreamEndCapture(o, &graph));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  QL_HIP(e);
+  return exec;
+}
+
 void build(qlles_batch* b) {
   if (b->built) return;
   make_streams(b);
@@ -97,6 +135,7 @@ void build(qlles_batch* b) {
     auto& x = b->m[i];
     x.g = capture(x, x.uh, x.r, x.ur, b->chunk, b->streams[i % b->lanes]);
   }
+  if (b->single) b->all = capture_all(b);
   b->built = true;
 }
 
@@ -106,6 +145,17 @@ void run(qlles_batch* b, const std::vector<int>& n, hipStream_t caller) {
   QL_HIP(hipEventRecord(b->fork, caller));
   for (hipStream_t s : b->streams) QL_HIP(hipStreamWaitEvent(s, b->fork, 0));
   std::vector<int> left(n);
+  if (b->single && b->all) {
+    // whole chunks that every member still needs: one launch of the joint graph
+    int common = *std::min_element(left.begin(), left.end()) / b->chunk;
+    for (int c = 0; c < common; ++c) QL_HIP(hipGraphLaunch(b->all, b->streams[0]));
+    for (int& v : left) v -= common * b->chunk;
+    // the other streams continue after the joint launches
+    if (common > 0 && b->lanes > 1) {
+      QL_HIP(hipEventRecord(b->join[0], b->streams[0]));
+      for (int l = 1; l < b->lanes; ++l) QL_HIP(hipStreamWaitEvent(b->streams[l], b->join[0], 0));
+    }
+  }
   // round-robin over members so that every stream always has queued work
   for (bool any = true; any;) {
     any = false;
@@ -174,20 +224,22 @@ int qlles_batch_add(qlles_batch* b, qlles_plan* p, qlles_interp* it, void* u_hat
   });
 }
 
-int qlles_batch_configure(qlles_batch* b, int lanes, int chunk) {
+int qlles_batch_configure(qlles_batch* b, int lanes, int chunk, int single_graph) {
   if (!b) return fail(QLLES_ERR_ARG, "null batch");
   if (lanes < 1 || lanes > 16 || chunk < 1 || chunk > 1000) return fail(QLLES_ERR_ARG, "need 1 <= lanes <= 16, 1 <= chunk <= 1000");
   return guarded([&] {
     drop_graphs(b);
     b->lanes = lanes;
     b->chunk = chunk;
+    b->single = single_graph ? 1 : 0;
   });
 }
 
-int qlles_batch_get_config(const qlles_batch* b, int* lanes, int* chunk, double* us_per_traj_step) {
+int qlles_batch_get_config(const qlles_batch* b, int* lanes, int* chunk, int* single_graph, double* us_per_traj_step) {
   if (!b) return fail(QLLES_ERR_ARG, "null batch");
   if (lanes) *lanes = b->lanes;
   if (chunk) *chunk = b->chunk;
+  if (single_graph) *single_graph = b->single;
   if (us_per_traj_step) *us_per_traj_step = b->tuned_us_per_traj_step;
   return QLLES_OK;
 }
@@ -234,19 +286,23 @@ int qlles_batch_tune(qlles_batch* b, int trial_steps, void* stream) {
       t.m.push_back(y);
     }
     QL_HIP(hipStreamSynchronize(caller));
+    // kernel variants first, member by member (plan state, shared with solo runs)
+    for (const auto& x : b->m) qlapi::tune_plan(x.p, x.uh, std::max(2, trial_steps / 4), nullptr, caller);
     const int nm = (int)b->m.size();
     double best = -1.0;
-    int best_l = 1, best_c = 1;
+    int best_l = 1, best_c = 1, best_s = 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     QL_HIP(hipEventCreate(&e0));
     QL_HIP(hipEventCreate(&e1));
     try {
+      for (int single : {0, 1})
       for (int lanes : {1, 2, 4, 8, 16}) {
         if (lanes > nm && lanes > 1) break;
         for (int chunk : {1, 4, 16, 64}) {
           if (chunk > trial_steps && chunk > 1) break;
           t.lanes = lanes;
           t.chunk = chunk;
+          t.single = single;
           drop_graphs(&t);
           const int n = std::max(chunk, trial_steps / chunk * chunk);
           run(&t, std::vector<int>(nm, chunk), caller);  // warm-up (also builds the graphs)
@@ -261,6 +317,7 @@ int qlles_batch_tune(qlles_batch* b, int trial_steps, void* stream) {
             best = us;
             best_l = lanes;
             best_c = chunk;
+            best_s = single;
           }
         }
       }
@@ -280,6 +337,7 @@ int qlles_batch_tune(qlles_batch* b, int trial_steps, void* stream) {
     drop_graphs(b);
     b->lanes = best_l;
     b->chunk = best_c;
+    b->single = best_s;
     b->tuned_us_per_traj_step = best;
   });
   (void)hipStreamSynchronize(caller);

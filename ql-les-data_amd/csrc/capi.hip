@@ -116,6 +116,24 @@ int qlles_filter(qlles_plan* p, const void* u_hat, int ckp_N, void* u_lres, void
   return guarded([&] { p->eng->filter(u_hat, ckp_N, u_lres, (hipStream_t)stream); });
 }
 
+int qlles_plan_set_variant(qlles_plan* p, int xv, int zv) {
+  if (!p) return fail(QLLES_ERR_ARG, "null plan");
+  return guarded([&] { p->eng->set_variant(xv, zv); });
+}
+
+int qlles_plan_get_variant(const qlles_plan* p, int* xv, int* zv, int* nvar) {
+  if (!p) return fail(QLLES_ERR_ARG, "null plan");
+  p->eng->get_variant(xv, zv);
+  if (nvar) *nvar = p->eng->variants();
+  return QLLES_OK;
+}
+
+int qlles_plan_tune(qlles_plan* p, const void* u_hat, int trial_steps, double* best_us, void* stream) {
+  if (!p || !u_hat) return fail(QLLES_ERR_ARG, "null argument");
+  if (trial_steps < 1) return fail(QLLES_ERR_ARG, "trial_steps must be >= 1");
+  return guarded([&] { qlapi::tune_plan(p, u_hat, trial_steps, best_us, (hipStream_t)stream); });
+}
+
 int qlles_rhs_jvp(qlles_plan* p, const void* u_hat, const void* v, void* out, void* stream) {
   if (!p || !u_hat || !v || !out) return fail(QLLES_ERR_ARG, "null argument");
   if (out == u_hat || out == v) return fail(QLLES_ERR_ARG, "out may not alias an input");

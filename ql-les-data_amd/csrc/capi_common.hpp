@@ -58,4 +58,51 @@ inline void run_steps(qlles_plan* p, qlles_interp* it, void* u_out, void* u_hat,
   };
   p->eng->step(u_out, u_hat, nullptr, nsteps, e_inj, s);
 }
+// time every kernel variant of the plan over `trials` steps on a scratch copy
+// of u_hat (the caller's state does not move) and keep the fastest
+inline void tune_plan(qlles_plan* p, const void* u_hat, int trials, double* best_us, hipStream_t s) {
+  const int N = p->desc.N, dim = p->desc.dim;
+  const size_t rows = dim == 3 ? (size_t)N * N : (size_t)N;
+  const size_t bytes = 3 * rows * (N / 2 + 1) * 2 * (p->desc.dtype == QLLES_F32 ? 4 : 8);
+  void* tmp = nullptr;
+  QL_HIP(hipMalloc(&tmp, bytes));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int bx = 0, bz = 0;
+  double best = -1.0;
+  try {
+    QL_HIP(hipEventCreate(&e0));
+    QL_HIP(hipEventCreate(&e1));
+    const int nv = p->eng->variants();
+    for (int xv = 0; xv < nv; ++xv)
+      for (int zv = 0; zv < nv; ++zv) {
+        p->eng->set_variant(xv, zv);
+        QL_HIP(hipMemcpyAsync(tmp, u_hat, bytes, hipMemcpyDeviceToDevice, s));
+        p->eng->step(nullptr, tmp, nullptr, 1, nullptr, s);  // warm-up
+        QL_HIP(hipEventRecord(e0, s));
+        p->eng->step(nullptr, tmp, nullptr, trials, nullptr, s);
+        QL_HIP(hipEventRecord(e1, s));
+        QL_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        QL_HIP(hipEventElapsedTime(&ms, e0, e1));
+        const double us = 1e3 * ms / trials;
+        if (best < 0 || us < best) {
+          best = us;
+          bx = xv;
+          bz = zv;
+        }
+      }
+  } catch (...) {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(tmp);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(tmp);
+  p->eng->set_variant(bx, bz);
+  if (best_us) *best_us = best;
+}
 }  // namespace qlapi

@@ -31,6 +31,11 @@ class Engine {
   virtual void dissipation(const void* uhat, double* out, hipStream_t s) = 0;
   virtual void energy_hat(const void* uhat, double* out, hipStream_t s) = 0;
   virtual void filter(const void* uhat, int n, void* ulres, hipStream_t s) = 0;
+  // kernel variants: xv = x / y-pass tile width, zv = z-pass (2-D y-pass)
+  // radix plan; 2 choices each for grids up to 128, 1 above
+  virtual int variants() const = 0;
+  virtual void set_variant(int xv, int zv) = 0;
+  virtual void get_variant(int* xv, int* zv) const = 0;
   // tangent-linear and adjoint maps of the RHS and of one RK4 step at u_hat
   // (3-D, explicit viscous term, forcing none or fixed power); inputs and
   // outputs are [3][N][N][N/2+1] spectra and may not alias each other.

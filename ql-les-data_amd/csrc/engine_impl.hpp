@@ -40,7 +40,7 @@ class EngineT final : public Engine {
   double* alpha_ = nullptr;
   double* dscr_ = nullptr;
   const cx<T>* twN_ = nullptr;
-  int nxblk_ = 0;
+  int nxblk_ = 0;  // capacity of the per-block partials (largest x-pass grid over the variants)
 
   template <class P>
   P* alloc(size_t n) {
@@ -76,29 +76,66 @@ class EngineT final : public Engine {
     return c;
   }
 
+  // ---------------- kernel variants ----------------
+  // Grids up to 128 carry a second tile width for the x / y passes (xv_) and
+  // a second radix plan for the z-pass / 2-D y-pass (zv_); the autotuner
+  // (set_variant / tune) picks per plan.  Larger grids keep variant 0.
+  static constexpr bool HAS_V = N <= 128;
+  int xv_ = 0, zv_ = 0;
+  template <class F>
+  void with_xv(F&& f) const {
+    if constexpr (HAS_V) {
+      if (xv_) {
+        f(std::integral_constant<int, 1>{});
+        return;
+      }
+    }
+    f(std::integral_constant<int, 0>{});
+  }
+  template <class F>
+  void with_zv(F&& f) const {
+    if constexpr (HAS_V) {
+      if (zv_) {
+        f(std::integral_constant<int, 1>{});
+        return;
+      }
+    }
+    f(std::integral_constant<int, 0>{});
+  }
+
   // ---------------- 3-D launches ----------------
   // stage and viscous mode select a compile-time specialisation of the x-pass
-  template <int SG, bool VIF>
+  template <int SG, bool VIF, int V>
   static void launch_x3_t(NS3X<T>& a, dim3 g, hipStream_t s) {
-    hipLaunchKernelGGL((k_ns3_x<T, N, SG, VIF>), g, dim3(XCfg<N>::THREADS), 0, s, a);
+    hipLaunchKernelGGL((k_ns3_x<T, N, SG, VIF, V>), g, dim3(XCfg<N, V>::THREADS), 0, s, a);
   }
+  template <int V>
   static void launch_x3(NS3X<T>& a, dim3 g, hipStream_t s) {
     const bool vif = a.c.visc_if != 0;
     switch (a.c.stage) {
-      case -1: launch_x3_t<-1, false>(a, g, s); break;
-      case 0: vif ? launch_x3_t<0, true>(a, g, s) : launch_x3_t<0, false>(a, g, s); break;
-      case 1: vif ? launch_x3_t<1, true>(a, g, s) : launch_x3_t<1, false>(a, g, s); break;
-      case 2: vif ? launch_x3_t<2, true>(a, g, s) : launch_x3_t<2, false>(a, g, s); break;
-      case 3: vif ? launch_x3_t<3, true>(a, g, s) : launch_x3_t<3, false>(a, g, s); break;
-      case 4: vif ? launch_x3_t<4, true>(a, g, s) : launch_x3_t<4, false>(a, g, s); break;
+      case -1: launch_x3_t<-1, false, V>(a, g, s); break;
+      case 0: vif ? launch_x3_t<0, true, V>(a, g, s) : launch_x3_t<0, false, V>(a, g, s); break;
+      case 1: vif ? launch_x3_t<1, true, V>(a, g, s) : launch_x3_t<1, false, V>(a, g, s); break;
+      case 2: vif ? launch_x3_t<2, true, V>(a, g, s) : launch_x3_t<2, false, V>(a, g, s); break;
+      case 3: vif ? launch_x3_t<3, true, V>(a, g, s) : launch_x3_t<3, false, V>(a, g, s); break;
+      case 4: vif ? launch_x3_t<4, true, V>(a, g, s) : launch_x3_t<4, false, V>(a, g, s); break;
       default: throw std::invalid_argument("bad stage");
     }
   }
   double C() const { return (double)rows_ * Nh * 2.0 * sizeof(T); }  // one spectral component
   // x-pass / y-inverse tiles along kz: full tiles below N/2 plus one Nyquist-column tile
-  static int xtiles() {
-    constexpr int ntr = (N / 2) / XCfg<N>::TIX;
-    return ntr > 0 ? ntr + 1 : (Nh + XCfg<N>::TIX - 1) / XCfg<N>::TIX;
+  template <int V>
+  static int xtiles_v() {
+    constexpr int tix = XCfg<N, V>::TIX;
+    constexpr int ntr = (N / 2) / tix;
+    return ntr > 0 ? ntr + 1 : (Nh + tix - 1) / tix;
+  }
+  int xtiles() const { return xv_ && HAS_V ? xtiles_v<HAS_V ? 1 : 0>() : xtiles_v<0>(); }
+  // blocks of the 3-D x-pass grid (FP64 energy partials per block) / 2-D grid
+  int xblocks() const {
+    if (DIM == 3) return xtiles() * N;
+    return xv_ && HAS_V ? (Nh + XCfg<N, HAS_V ? 1 : 0>::TIX - 1) / XCfg<N, HAS_V ? 1 : 0>::TIX
+                        : (Nh + XCfg<N>::TIX - 1) / XCfg<N>::TIX;
   }
   double xpass_bytes(const StageCtl<T>& c, bool inv, int ncomp, int nx) const {
     const double cc = C();
@@ -123,7 +160,7 @@ class EngineT final : public Engine {
     dim3 g(xtiles(), N);
     hipEvent_t e0 = nullptr;
     prof_begin(s, &e0);
-    launch_x3(a, g, s);
+    with_xv([&](auto v) { launch_x3<decltype(v)::value>(a, g, s); });
     QL_LAUNCHED();
     prof_end(s, K_XPASS, xpass_bytes(a.c, a.do_inverse, 3, 5) + (a.copy_dst ? 3.0 * C() : 0.0), e0);
   }
@@ -134,7 +171,10 @@ class EngineT final : public Engine {
     dim3 g(xtiles(), N);
     hipEvent_t e0 = nullptr;
     prof_begin(s, &e0);
-    hipLaunchKernelGGL((k_ns3_yinv<T, N>), g, dim3(XCfg<N>::THREADS), 0, s, a);
+    with_xv([&](auto v) {
+      constexpr int V = decltype(v)::value;
+      hipLaunchKernelGGL((k_ns3_yinv<T, N, V>), g, dim3(XCfg<N, V>::THREADS), 0, s, a);
+    });
     QL_LAUNCHED();
     prof_end(s, K_YINV, (nf == 6 ? 11.0 : 6.0) * C(), e0);
   }
@@ -142,7 +182,10 @@ class EngineT final : public Engine {
     NS3Z<T> a{Y6, NZ, u_out, u_in, qs(6), KZD, KZP, twN_};
     hipEvent_t e0 = nullptr;
     prof_begin(s, &e0);
-    hipLaunchKernelGGL((k_ns3_z<T, N>), dim3(N * N / ZCfg<N>::P), dim3(ZCfg<N>::THREADS), 0, s, a);
+    with_zv([&](auto v) {
+      constexpr int V = decltype(v)::value;
+      hipLaunchKernelGGL((k_ns3_z<T, N, V>), dim3(N * N / ZCfg<N, V>::P), dim3(ZCfg<N, V>::THREADS), 0, s, a);
+    });
     QL_LAUNCHED();
     const double R = (double)N * N * N * sizeof(T);
     prof_end(s, K_ZPASS, (u_in ? 3.0 * C() + 3.0 * R : 6.0 * C()) + (u_out ? 3.0 * R : 0.0) +
@@ -150,10 +193,13 @@ class EngineT final : public Engine {
   }
   void yfwd3(hipStream_t s) {
     NS3YF<T> a{NZ, NY, KZD, KZP, KYD, twN_};
-    dim3 g((KZD + XCfg<N>::TIX - 1) / XCfg<N>::TIX, N);
     hipEvent_t e0 = nullptr;
     prof_begin(s, &e0);
-    hipLaunchKernelGGL((k_ns3_yfwd<T, N>), g, dim3(XCfg<N>::THREADS), 0, s, a);
+    with_xv([&](auto v) {
+      constexpr int V = decltype(v)::value;
+      dim3 g((KZD + XCfg<N, V>::TIX - 1) / XCfg<N, V>::TIX, N);
+      hipLaunchKernelGGL((k_ns3_yfwd<T, N, V>), g, dim3(XCfg<N, V>::THREADS), 0, s, a);
+    });
     QL_LAUNCHED();
     prof_end(s, K_YFWD, 3.0 * N * (N + KYD) * KZD * 2.0 * sizeof(T), e0);
   }
@@ -179,7 +225,11 @@ class EngineT final : public Engine {
   void x2(NS2X<T>& a, hipStream_t s) {
     hipEvent_t e0 = nullptr;
     prof_begin(s, &e0);
-    hipLaunchKernelGGL((k_ns2_x<T, N>), dim3((Nh + XCfg<N>::TIX - 1) / XCfg<N>::TIX), dim3(XCfg<N>::THREADS), 0, s, a);
+    with_xv([&](auto v) {
+      constexpr int V = decltype(v)::value;
+      hipLaunchKernelGGL((k_ns2_x<T, N, V>), dim3((Nh + XCfg<N, V>::TIX - 1) / XCfg<N, V>::TIX),
+                         dim3(XCfg<N, V>::THREADS), 0, s, a);
+    });
     QL_LAUNCHED();
     prof_end(s, K_XPASS, xpass_bytes(a.c, a.do_inverse, 2, 3), e0);
   }
@@ -187,8 +237,11 @@ class EngineT final : public Engine {
     NS2Y<T> a{X5, NY, u_out, u_in, LD, KZD, KZP, twN_};
     hipEvent_t e0 = nullptr;
     prof_begin(s, &e0);
-    hipLaunchKernelGGL((k_ns2_y<T, N>), dim3((N + ZCfg<N>::P - 1) / ZCfg<N>::P), dim3(ZCfg<N>::THREADS), 0, s,
-                       a);
+    with_zv([&](auto v) {
+      constexpr int V = decltype(v)::value;
+      hipLaunchKernelGGL((k_ns2_y<T, N, V>), dim3((N + ZCfg<N, V>::P - 1) / ZCfg<N, V>::P), dim3(ZCfg<N, V>::THREADS),
+                         0, s, a);
+    });
     QL_LAUNCHED();
     const double R = (double)N * N * sizeof(T);
     prof_end(s, K_ZPASS, (u_in ? C() + 2.0 * R : 3.0 * C()) + (u_out ? 2.0 * R : 0.0) + 2.0 * N * KZD * 2.0 * sizeof(T),
@@ -339,7 +392,7 @@ class EngineT final : public Engine {
       Y6 = alloc<cx<T>>(6 * fsz);
       NZ = alloc<cx<T>>((size_t)3 * N * N * KZP);
       NY = alloc<cx<T>>((size_t)3 * N * KYD * KZP);
-      nxblk_ = xtiles() * N;
+      nxblk_ = std::max(xtiles_v<0>(), xtiles_v<HAS_V ? 1 : 0>()) * N;  // partials capacity over variants
     } else {
       S1a = alloc<cx<T>>(2 * fsz);
       S1b = alloc<cx<T>>(2 * fsz);
@@ -347,7 +400,7 @@ class EngineT final : public Engine {
       X5 = alloc<cx<T>>(3 * fsz);
       Y6 = alloc<cx<T>>(3 * fsz);  // scratch for transforms / diagnostics
       NY = alloc<cx<T>>((size_t)2 * N * KZP);
-      nxblk_ = (Nh + XCfg<N>::TIX - 1) / XCfg<N>::TIX;
+      nxblk_ = (Nh + XCfg<N, HAS_V ? 1 : 0>::TIX - 1) / XCfg<N, HAS_V ? 1 : 0>::TIX;
     }
     partials_ = alloc<double>(std::max(RED_BLOCKS * 4, nxblk_ * 2));
     alpha_ = alloc<double>(8);
@@ -401,7 +454,7 @@ class EngineT final : public Engine {
         }
         if (ekin) {
           const double n6 = (double)N * N * N * (double)N * N * N;
-          hipLaunchKernelGGL(k_ekin_alpha, dim3(1), dim3(RED_THREADS), 0, s, partials_, nxblk_, 1.0 / n6,
+          hipLaunchKernelGGL(k_ekin_alpha, dim3(1), dim3(RED_THREADS), 0, s, partials_, xblocks(), 1.0 / n6,
                              d_.e_kin_init, d_.dt, alpha_, e_inj ? e_inj + n : nullptr);
           QL_LAUNCHED();
           a = xargs3(nxt, ld_nxt, q_nxt);
@@ -542,6 +595,18 @@ class EngineT final : public Engine {
     hipLaunchKernelGGL(k_energy_hat_final, dim3(1), dim3(RED_THREADS), 0, s, partials_, RED_BLOCKS, 1.0 / (nd * nd),
                        out);
     QL_LAUNCHED();
+  }
+
+  int variants() const override { return HAS_V ? 2 : 1; }
+  void set_variant(int xv, int zv) override {
+    if (xv < 0 || zv < 0 || xv >= variants() || zv >= variants())
+      throw std::invalid_argument("kernel variant out of range for this grid");
+    xv_ = xv;
+    zv_ = zv;
+  }
+  void get_variant(int* xv, int* zv) const override {
+    if (xv) *xv = xv_;
+    if (zv) *zv = zv_;
   }
 
   void rhs_jvp(const void* uhat, const void* v, void* out, hipStream_t s) override {

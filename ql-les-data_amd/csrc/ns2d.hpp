@@ -57,10 +57,10 @@ __device__ double forced_shell_energy2(const cx<T>* us, int ld, int kf2, double*
   return 0.5 * acc[0];
 }
 
-template <typename T, int N>
-__global__ void __launch_bounds__(XCfg<N>::THREADS)
+template <typename T, int N, int V = 0>
+__global__ void __launch_bounds__((XCfg<N, V>::THREADS))
 k_ns2_x(NS2X<T> a) {
-  using C = XCfg<N>;
+  using C = XCfg<N, V>;
   constexpr int Nh = N / 2 + 1;
   using FF = XFwd<T, N>;
   using IF = XInv<T, N>;
@@ -206,10 +206,10 @@ struct NS2Y {
   const cx<T>* tw;
 };
 
-template <typename T, int N>
-__global__ void __launch_bounds__(ZCfg<N>::THREADS)
+template <typename T, int N, int V = 0>
+__global__ void __launch_bounds__((ZCfg<N, V>::THREADS))
 k_ns2_y(NS2Y<T> a) {
-  using C = ZCfg<N>;
+  using C = ZCfg<N, V>;
   constexpr int M = C::M, TP = C::TP, P = C::P;
   constexpr int Nh = M + 1;
   constexpr int ZP = C::ZPAD;

@@ -57,11 +57,12 @@
 #define QL_WPE_YF 0
 #endif
 #define QL_WPE_ATTR_(n) __attribute__((amdgpu_waves_per_eu(n)))
-// the x-pass target applies to FP32; FP64 keeps twice the registers per
-// element and is left to the compiler
-template <typename T> struct WpeX { static constexpr int value = sizeof(T) == 4 ? QL_WPE_X : 0; };
+// the x-pass target applies to FP32 at N >= 256, where it fits without
+// private-memory spills; FP64 (twice the registers per element) and the
+// smaller grids are left to the compiler
+template <typename T, int N> struct WpeX { static constexpr int value = (sizeof(T) == 4 && N >= 256) ? QL_WPE_X : 0; };
 #if QL_WPE_X > 0
-#define QL_ATTR_X __attribute__((amdgpu_waves_per_eu(WpeX<T>::value > 0 ? WpeX<T>::value : 1)))
+#define QL_ATTR_X __attribute__((amdgpu_waves_per_eu(WpeX<T, N>::value > 0 ? WpeX<T, N>::value : 1)))
 #else
 #define QL_ATTR_X
 #endif
@@ -136,11 +137,13 @@ template <> struct XPlan<128> { using R = Radices<4, 4, 8>; static constexpr int
 #define QL_TIX 16
 #endif
 
-template <int N>
+// V selects the tile width: 0 = QL_TIX columns, 1 = half of it (more
+// workgroups for small grids; chosen per plan by the autotuner)
+template <int N, int V = 0>
 struct XCfg {
   using R = typename XPlan<N>::R;
   static constexpr int TP = XPlan<N>::TP;
-  static constexpr int TIX = QL_TIX;  // columns (contiguous kz / ky) per workgroup tile
+  static constexpr int TIX = V == 0 ? QL_TIX : QL_TIX / 2;  // columns (contiguous kz / ky) per workgroup tile
   static constexpr int THREADS = TIX * TP;
   static constexpr int LS = TIX + LPAD;
 };
@@ -227,10 +230,10 @@ template <int SG, bool VIF> struct AccOut { static constexpr bool value = (SG ==
 // compile-time so the epilogue is straight-line code: all of a slot's loads
 // are issued unconditionally (clamped addresses, values selected) and only
 // the stores are predicated, which lets loads of later slots overlap.
-template <typename T, int N, int SG, bool VIF>
-__global__ void __launch_bounds__(XCfg<N>::THREADS) QL_ATTR_X
+template <typename T, int N, int SG, bool VIF, int V = 0>
+__global__ void __launch_bounds__((XCfg<N, V>::THREADS)) QL_ATTR_X
 k_ns3_x(NS3X<T> a) {
-  using C = XCfg<N>;
+  using C = XCfg<N, V>;
   constexpr int Nh = N / 2 + 1;
   using FF = XFwd<T, N>;
   using IF = XInv<T, N>;
@@ -493,10 +496,10 @@ struct NS3Y {
   const cx<T>* tw;
 };
 
-template <typename T, int N>
-__global__ void __launch_bounds__(XCfg<N>::THREADS) QL_ATTR_YI
+template <typename T, int N, int V = 0>
+__global__ void __launch_bounds__((XCfg<N, V>::THREADS)) QL_ATTR_YI
 k_ns3_yinv(NS3Y<T> a) {
-  using C = XCfg<N>;
+  using C = XCfg<N, V>;
   constexpr int Nh = N / 2 + 1;
   using F = XInv<T, N>;
   __shared__ cx<T> s_tw[F::TWL > 0 ? F::TWL : 1];
@@ -604,10 +607,15 @@ template <> struct ZPlan<128> { using R = Radices<4, 4, 4, 2>; };
 template <> struct ZPlan<64> { using R = Radices<4, 4, 4>; };
 #endif
 
-template <int N>
+// V selects the radix plan of the half-length FFTs: 0 = ZPlan, 1 = the
+// generic Plan (fewer passes, more elements per lane)
+template <int M, int V> struct ZPlanV { using R = typename ZPlan<M>::R; };
+template <int M> struct ZPlanV<M, 1> { using R = typename Plan<M>::R; };
+
+template <int N, int V = 0>
 struct ZCfg {
   static constexpr int M = N / 2;
-  using R = typename ZPlan<M>::R;
+  using R = typename ZPlanV<M, V>::R;
   static constexpr int TP = M / R::at(0);  // lanes per pencil = last-pass butterflies of the c2r
   static constexpr int P0 = QL_ZTHREADS / TP;
   static constexpr int P = (N <= 64) ? (P0 > 8 ? 8 : P0) : (P0 > 16 ? 16 : P0);
@@ -657,9 +665,9 @@ __device__ __forceinline__ void r2c_post_store(cx<T>* dst, const cx<T>* pb, cons
 // the three products stay live.
 // state of one z-pass workgroup; the prefetch ring is two named buffers so
 // every index is a compile-time constant (no private-memory arrays)
-template <typename T, int N>
+template <typename T, int N, int V = 0>
 struct ZState {
-  using C = ZCfg<N>;
+  using C = ZCfg<N, V>;
   static constexpr int M = C::M, TP = C::TP, P = C::P, Nh = M + 1, NT = C::THREADS;
   static constexpr int ZP = C::ZPAD;
   // every pencil's TP lanes lie in one wavefront, and each wavefront loads
@@ -777,10 +785,10 @@ struct ZState {
   }
 };
 
-template <typename T, int N, bool UIN>
+template <typename T, int N, bool UIN, int V>
 __device__ __forceinline__ void ns3_z_body(const NS3Z<T>& a, cx<T>* s_tw, cx<T>* s_twi, cx<T>* s_twf,
                                            cx<T>* s_buf) {
-  using St = ZState<T, N>;
+  using St = ZState<T, N, V>;
   constexpr int M = St::M, TP = St::TP, NO = St::NO;
   using CI = typename St::CI;
   using CF = typename St::CF;
@@ -816,7 +824,7 @@ __device__ __forceinline__ void ns3_z_body(const NS3Z<T>& a, cx<T>* s_tw, cx<T>*
       for (int s = 0; s < NO; ++s) dst[CI::out_index(z.j, s)] = z.uu[f][s];
     }
   }
-  cx<T>* pb = s_buf + z.p * ZCfg<N>::LSR;
+  cx<T>* pb = s_buf + z.p * ZCfg<N, V>::LSR;
 #pragma unroll
   for (int cc = 0; cc < 3; ++cc) {
     cx<T> v[CF::MAXE];
@@ -832,17 +840,17 @@ __device__ __forceinline__ void ns3_z_body(const NS3Z<T>& a, cx<T>* s_tw, cx<T>*
   }
 }
 
-template <typename T, int N>
-__global__ void __launch_bounds__(ZCfg<N>::THREADS) QL_ATTR_Z
+template <typename T, int N, int V = 0>
+__global__ void __launch_bounds__((ZCfg<N, V>::THREADS)) QL_ATTR_Z
 k_ns3_z(NS3Z<T> a) {
-  using St = ZState<T, N>;
+  using St = ZState<T, N, V>;
   __shared__ cx<T> s_tw[N];
   __shared__ cx<T> s_twi[St::CI::TWL > 0 ? St::CI::TWL : 1];
   __shared__ cx<T> s_twf[St::CF::TWL > 0 ? St::CF::TWL : 1];
-  __shared__ cx<T> s_buf[ZCfg<N>::P * ZCfg<N>::LSR];
+  __shared__ cx<T> s_buf[ZCfg<N, V>::P * ZCfg<N, V>::LSR];
   // the register prefetch ring is indexed by compile-time field numbers only
-  if (a.u_in != nullptr) ns3_z_body<T, N, true>(a, s_tw, s_twi, s_twf, s_buf);
-  else ns3_z_body<T, N, false>(a, s_tw, s_twi, s_twf, s_buf);
+  if (a.u_in != nullptr) ns3_z_body<T, N, true, V>(a, s_tw, s_twi, s_twf, s_buf);
+  else ns3_z_body<T, N, false, V>(a, s_tw, s_twi, s_twf, s_buf);
 }
 
 // ---------------------------------------------------------------------------
@@ -856,10 +864,10 @@ struct NS3YF {
   const cx<T>* tw;
 };
 
-template <typename T, int N>
-__global__ void __launch_bounds__(XCfg<N>::THREADS) QL_ATTR_YF
+template <typename T, int N, int V = 0>
+__global__ void __launch_bounds__((XCfg<N, V>::THREADS)) QL_ATTR_YF
 k_ns3_yfwd(NS3YF<T> a) {
-  using C = XCfg<N>;
+  using C = XCfg<N, V>;
   using F = XFwd<T, N>;
   __shared__ cx<T> s_tw[F::TWL > 0 ? F::TWL : 1];
   __shared__ cx<T> s_buf[N * C::LS];

@@ -33,7 +33,8 @@ EXPORTS = (
     "qlles_step_advect", "qlles_batch_create", "qlles_batch_destroy", "qlles_batch_add",
     "qlles_batch_configure", "qlles_batch_tune", "qlles_batch_get_config", "qlles_batch_run",
     "qlles_rhs_jvp", "qlles_rhs_vjp", "qlles_step_jvp", "qlles_step_vjp", "qlles_irfftn_adjoint",
-    "qlles_rfftn_adjoint", "qlles_interp_hat_vjp",
+    "qlles_rfftn_adjoint", "qlles_interp_hat_vjp", "qlles_plan_set_variant", "qlles_plan_get_variant",
+    "qlles_plan_tune",
 )
 KINDS = ("x-pass", "y-inverse", "z-pass", "y-forward", "other")
 
@@ -92,9 +93,9 @@ def load():
         "qlles_batch_create": (ci, [ctypes.POINTER(vp)]),
         "qlles_batch_destroy": (ci, [vp]),
         "qlles_batch_add": (ci, [vp, vp, vp, vp, vp, ci, vp, ctypes.POINTER(ci)]),
-        "qlles_batch_configure": (ci, [vp, ci, ci]),
+        "qlles_batch_configure": (ci, [vp, ci, ci, ci]),
         "qlles_batch_tune": (ci, [vp, ci, vp]),
-        "qlles_batch_get_config": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), pd]),
+        "qlles_batch_get_config": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci), pd]),
         "qlles_batch_run": (ci, [vp, vp, ci, vp]),
         "qlles_rhs_jvp": (ci, [vp, vp, vp, vp, vp]),
         "qlles_rhs_vjp": (ci, [vp, vp, vp, vp, vp, vp]),
@@ -103,6 +104,9 @@ def load():
         "qlles_irfftn_adjoint": (ci, [vp, vp, vp, vp]),
         "qlles_rfftn_adjoint": (ci, [vp, vp, vp, vp]),
         "qlles_interp_hat_vjp": (ci, [vp, ci, vp, vp, ci, vp, vp, vp, vp]),
+        "qlles_plan_set_variant": (ci, [vp, ci, ci]),
+        "qlles_plan_get_variant": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci)]),
+        "qlles_plan_tune": (ci, [vp, vp, ci, pd, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -153,6 +157,21 @@ class Plan:
     @property
     def workspace_bytes(self):
         return int(_lib.qlles_plan_workspace_bytes(self.handle))
+
+    def variant(self):
+        """(xv, zv, number of choices per knob) of the kernel variants in use."""
+        xv, zv, nv = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(_lib.qlles_plan_get_variant(self.handle, ctypes.byref(xv), ctypes.byref(zv), ctypes.byref(nv)))
+        return xv.value, zv.value, nv.value
+
+    def set_variant(self, xv, zv):
+        check(_lib.qlles_plan_set_variant(self.handle, int(xv), int(zv)))
+
+    def tune(self, u_hat, trial_steps=8):
+        """Time every kernel variant on a scratch copy of u_hat and keep the fastest; returns us/step."""
+        us = ctypes.c_double()
+        check(_lib.qlles_plan_tune(self.handle, ptr(u_hat), int(trial_steps), ctypes.byref(us), stream_ptr()))
+        return us.value
 
     def close(self):
         if getattr(self, "handle", None) is not None and _lib is not None:

@@ -41,8 +41,8 @@ class Batch:
         self.members.append((step, u_hat, interp, r, u_r))
         return idx.value
 
-    def configure(self, lanes, chunk):
-        _lib.check(self._lib.qlles_batch_configure(self.handle, int(lanes), int(chunk)))
+    def configure(self, lanes, chunk, single_graph=False):
+        _lib.check(self._lib.qlles_batch_configure(self.handle, int(lanes), int(chunk), 1 if single_graph else 0))
 
     def tune(self, trial_steps=16):
         """Pick streams x steps-per-graph by timing on scratch copies (trajectories do not move)."""
@@ -50,10 +50,10 @@ class Batch:
         return self.config()
 
     def config(self):
-        lanes, chunk, us = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        lanes, chunk, single, us = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         _lib.check(self._lib.qlles_batch_get_config(self.handle, ctypes.byref(lanes), ctypes.byref(chunk),
-                                                    ctypes.byref(us)))
-        return dict(lanes=lanes.value, chunk=chunk.value, us_per_traj_step=us.value)
+                                                    ctypes.byref(single), ctypes.byref(us)))
+        return dict(lanes=lanes.value, chunk=chunk.value, single_graph=bool(single.value), us_per_traj_step=us.value)
 
     def run(self, nsteps):
         """Advance every member: ``nsteps`` an int (all the same) or a list (uneven lengths)."""

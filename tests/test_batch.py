@@ -32,12 +32,13 @@ def test_batch_validation_without_device():
     assert lib.qlles_batch_create(ctypes.byref(h)) == 0
     try:
         assert lib.qlles_batch_add(h, None, None, None, None, 0, None, None) == _lib.QLLES_ERR_ARG
-        assert lib.qlles_batch_configure(h, 0, 1) == _lib.QLLES_ERR_ARG
-        assert lib.qlles_batch_configure(h, 2, 0) == _lib.QLLES_ERR_ARG
-        assert lib.qlles_batch_configure(h, 2, 4) == 0
-        lanes, chunk, us = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
-        assert lib.qlles_batch_get_config(h, ctypes.byref(lanes), ctypes.byref(chunk), ctypes.byref(us)) == 0
-        assert (lanes.value, chunk.value) == (2, 4)
+        assert lib.qlles_batch_configure(h, 0, 1, 0) == _lib.QLLES_ERR_ARG
+        assert lib.qlles_batch_configure(h, 2, 0, 0) == _lib.QLLES_ERR_ARG
+        assert lib.qlles_batch_configure(h, 2, 4, 1) == 0
+        lanes, chunk, single, us = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        assert lib.qlles_batch_get_config(h, ctypes.byref(lanes), ctypes.byref(chunk), ctypes.byref(single),
+                                          ctypes.byref(us)) == 0
+        assert (lanes.value, chunk.value, single.value) == (2, 4, 1)
         assert lib.qlles_batch_run(h, None, -3, None) == _lib.QLLES_ERR_ARG
         assert lib.qlles_batch_run(h, None, 5, None) == 0  # empty batch: nothing to do
     finally:
@@ -98,12 +99,15 @@ def test_batch_bit_identical_to_solo(gpu):
     lengths = [7, 5, 9, 3]  # uneven trajectories; chunk 2 leaves eager remainders
     b.configure(lanes=2, chunk=2)
     b.run(lengths)
+    b.configure(lanes=3, chunk=2, single_graph=True)  # one joint graph for the common chunk
+    b.run([4, 4, 6, 4])
     cfg = b.tune(trial_steps=4)  # runs on scratch copies: states must not move
     assert cfg["lanes"] in (1, 2, 4, 8, 16) and cfg["chunk"] in (1, 4, 16, 64) and cfg["us_per_traj_step"] > 0
     b.run(5)
     torch.cuda.synchronize()
     for k, ((step, _, it, _), (uh_s, r_s), (uh_b, r_b), ur_b) in enumerate(zip(mem, solo, bat, urs)):
         ur1 = _solo(step, uh_s, it, r_s, lengths[k])
+        ur1 = _solo(step, uh_s, it, r_s, [4, 4, 6, 4][k])
         ur2 = _solo(step, uh_s, it, r_s, 5)
         torch.cuda.synchronize()
         assert np.array_equal(host(uh_s), host(uh_b)), f"member {k}: spectrum differs from solo run"
@@ -111,3 +115,35 @@ def test_batch_bit_identical_to_solo(gpu):
             assert np.array_equal(host(r_s), host(r_b)), f"member {k}: positions differ"
             assert np.array_equal(host(ur2), host(ur_b)), f"member {k}: particle velocities differ"
         del ur1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dim,N,case,kw", [
+    (3, 32, "HIT", dict(forcing_type="power", power=0.2, kf=2)),
+    (2, 64, "Kolm", dict(forcing_type="kolmogorov", drag=0.1, kolm_amp=1.0, kolm_k=4, kf=None)),
+])
+def test_plan_variants_match_oracle(gpu, dim, N, case, kw):
+    """Every kernel variant (tile width x radix plan) takes the same FP64 steps as variant (0, 0), which the
+    parity tests pin to the oracle; then the per-plan tuner picks one of them."""
+    from l3es_amd import set_up_solver
+
+    kw = dict(kw)
+    kf = kw.pop("kf")
+    _, uh, _, _, step, _, _, _, _ = set_up_solver(N, 2e-3, dim, case, 2e-3, 21, 8, kf, precision="float64",
+                                                  warmup=False, **kw)
+    xv0, zv0, nv = step.plan.variant()
+    assert (xv0, zv0) == (0, 0) and nv == 2
+    u0 = host(uh)
+    ref = None
+    for xv in range(nv):
+        for zv in range(nv):
+            step.plan.set_variant(xv, zv)
+            y = torch.from_numpy(u0).cuda()
+            step.advance(None, y, 3, None, use_u=False)
+            out = host(y)
+            if ref is None:
+                ref = out
+            err = np.abs(out - ref).max() / np.abs(ref).max()
+            assert err < 1e-12, (xv, zv, err)
+    best = step.plan.tune(torch.from_numpy(u0).cuda(), trial_steps=2)
+    assert best > 0 and step.plan.variant()[:2] in [(a, b) for a in range(nv) for b in range(nv)]
