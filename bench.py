@@ -151,18 +151,22 @@ def run_batch(args):
     torch.cuda.set_device(0)
     mem = []
     t0 = time.perf_counter()
-    for i in range(args.batch_members):
+
+    def make(i, precision):
         dim, N = BATCH_SIZES[i % len(BATCH_SIZES)]
         if dim == 2:
-            n = 32
-            _, uh, _, _, step, _, _, _, _ = set_up_solver(N, 1e-3, 2, "Kolm", 1e-3, 100 + i, n, None,
+            _, uh, _, _, step, _, _, _, _ = set_up_solver(N, 1e-3, 2, "Kolm", 1e-3, 100 + i, 32, None,
                                                           forcing_type="kolmogorov", drag=0.1, kolm_amp=1.0,
-                                                          kolm_k=4, precision="float32", warmup=False)
+                                                          kolm_k=4, precision=precision, warmup=False)
         else:
-            n = 16
-            _, uh, _, _, step, _, _, _, _ = set_up_solver(N, 1e-3, 3, "HIT", 5e-4, 100 + i, n, 2,
+            _, uh, _, _, step, _, _, _, _ = set_up_solver(N, 1e-3, 3, "HIT", 5e-4, 100 + i, 16, 2,
                                                           e_kin_target=0.5, forcing_type="power", power=0.1,
-                                                          spectrum="k4exp", precision="float32", warmup=False)
+                                                          spectrum="k4exp", precision=precision, warmup=False)
+        return dim, N, uh, step
+
+    for i in range(args.batch_members):
+        dim, N, uh, step = make(i, "float32")
+        n = 32 if dim == 2 else 16
         it = spectral_interpolator_wrapper(n, tuple(range(1, dim + 1)), backend="nufft", precision="float32")
         r = torch.from_numpy(pos_init_cartesian(n, dim)).cuda()
         mem.append((dim, N, step, uh, it, r))
@@ -201,6 +205,19 @@ def run_batch(args):
     torch.cuda.synchronize()
     t_b = time.perf_counter() - t0
     log(f"[bench] batch: tuned {cfg}; {total} trajectory-steps: batched {t_b:.3f}s, solo {t_solo:.3f}s")
+    # energy error of every FP32 member against its FP64 twin (same seed, same steps, solo on the GPU;
+    # the FP64 path matches the NumPy oracle to 1e-12 in the parity tests)
+    from l3es_amd.utils import kinetic_energy_hat
+
+    errs = []
+    for i, ((dim, N, step, uh, it, r), k) in enumerate(zip(mem, lengths)):
+        _, _, uh64, step64 = make(i, "float64")
+        step64.advance(None, uh64, args.warmup + k, None, use_u=False)
+        e32 = kinetic_energy_hat(uh)[0]
+        e64 = kinetic_energy_hat(uh64)[0]
+        errs.append(abs(e32 - e64) / abs(e64))
+        del uh64, step64
+    variants = [m[2].plan.variant()[:2] for m in mem]
     out = {
         "metric": "trajectory-steps/s, 24 uneven trajectories batched on one GPU",
         "value": total / t_b, "unit": "trajectory-steps/s", "n_gpus": 1, "steps": total, "warmup": args.warmup,
@@ -210,6 +227,8 @@ def run_batch(args):
                                "HIT 64^3-128^3 (16^3 particles), uneven lengths K..1.75K, NUFFT tol 1e-6",
                    "sizes": [f"{d}D {N}" for (d, N, *_rest) in mem], "lengths": lengths},
         "batched_tuned": cfg, "solo_sequential_traj_steps_per_s": total / t_solo, "speedup_vs_solo": t_solo / t_b,
+        "kernel_variants": variants,
+        "ke_rel_err_fp32_vs_fp64_worst": max(errs), "ke_rel_err_fp32_vs_fp64": errs,
     }
     print(json.dumps(out), flush=True)
 
