@@ -37,6 +37,7 @@ struct qlles_interp {
   void* coef;   // NUFFT: [ncomp][nf^d] zero-padded coefficients; DFT: [ncomp][n^d] shifted coefficients
   void* fine;   // NUFFT: [ncomp][nf^d] fine-grid values
   void* tmp;    // [ncomp][n^(d-1)][n/2+1] rfft of a physical n-grid field
+  void* packed; // 3-D NUFFT: [nf^3][4] real fine-grid values (u, v, w, 0) for the gather
   void* adj_a;  // adjoint (3-D NUFFT, allocated on first use): [3][nf^3] complex
   void* adj_b;  //   second [3][nf^3] complex buffer
 };
@@ -200,6 +201,106 @@ k_gather(const cx<T>* __restrict__ f, int nf, int w, T beta, const double* __res
     for (int c = 0; c < DIM; ++c) {
       ur[(size_t)pid * DIM + c] = acc[c];
       if (r_out) r_out[(size_t)pid * DIM + c] = wrap2pi(r[(size_t)pid * DIM + c] + dt * (double)acc[c]);
+    }
+  }
+}
+
+// real parts of the three fine-grid components, interleaved per point
+template <typename T>
+__global__ void k_pack3(const cx<T>* __restrict__ f, T* __restrict__ out, size_t cs) {
+  for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < cs; o += (size_t)gridDim.x * blockDim.x) {
+    out[4 * o] = f[o].re;
+    out[4 * o + 1] = f[cs + o].re;
+    out[4 * o + 2] = f[2 * cs + o].re;
+    out[4 * o + 3] = (T)0;
+  }
+}
+
+// the 3-D gather on the packed grid: one 4-component load per stencil point;
+// the kernel width W is a compile-time constant so each lane's W loads along
+// x are issued back to back
+template <typename T, int W>
+__global__ void __launch_bounds__(256)
+k_gather3p(const T* __restrict__ f4, int nf, T beta, const double* __restrict__ r, int np,
+           T* __restrict__ ur, double dt, double* __restrict__ r_out) {
+  constexpr int w = W;
+  const int lane = threadIdx.x & 63;
+  // workgroups go round-robin over the 8 XCDs; give each XCD a contiguous run
+  // of particles (neighbours in the caller's order), so its L2 holds their
+  // part of the fine grid
+  const int nb = gridDim.x, q8 = nb / 8, r8 = nb % 8;
+  const int xcd = blockIdx.x % 8, kb = blockIdx.x / 8;
+  const int blk = xcd * q8 + (xcd < r8 ? xcd : r8) + kb;
+  const int pid = blk * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (pid >= np) return;  // whole wavefront exits together
+  const double h = 6.283185307179586476925286766559 / nf;
+  const double halfw = 0.5 * w;
+  const T two_over_w = (T)(2.0 / w);
+  double t[3];
+  int i0[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    t[d] = r[(size_t)pid * 3 + d] / h;
+    i0[d] = (int)ceil(t[d] - halfw);
+  }
+  // One 1-D kernel weight and wrapped index per lane -- lanes [0, W) for x,
+  // [W, 2W) for y, [2W, 3W) for z -- then broadcast by cross-lane moves, so
+  // a wavefront evaluates 3W kernels instead of 64 * (W + 2).
+  static_assert(3 * W <= 64, "kernel width");
+  const bool pow2 = (nf & (nf - 1)) == 0;
+  T wl = 0;
+  int jl = 0;
+  if (lane < 3 * W) {
+    const int ax = lane / W, dl = lane % W;
+    const double td = ax == 0 ? t[0] : (ax == 1 ? t[1] : t[2]);
+    const int i = (ax == 0 ? i0[0] : (ax == 1 ? i0[1] : i0[2])) + dl;
+    wl = es_kernel((T)(td - i) * two_over_w, beta);
+    jl = pow2 ? (i & (nf - 1)) : ((i % nf) + nf) % nf;
+  }
+  T acc[3] = {0, 0, 0};
+  T wxs[W];
+  int jxs[W];
+#pragma unroll
+  for (int dx = 0; dx < W; ++dx) {
+    wxs[dx] = __shfl(wl, dx, 64);
+    jxs[dx] = __shfl(jl, dx, 64);
+  }
+  constexpr int NIT = (W * W + 63) / 64;
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int q = lane + 64 * it;
+    const int qc = q < W * W ? q : 0;  // every lane takes part in the moves
+    const int dy = qc / W, dz = qc % W;
+    const T wyz = __shfl(wl, W + dy, 64) * __shfl(wl, 2 * W + dz, 64);
+    const int jy = __shfl(jl, W + dy, 64), jz = __shfl(jl, 2 * W + dz, 64);
+    if (q >= W * W) continue;
+#pragma unroll
+    for (int dx = 0; dx < W; ++dx) {
+      const T wx = wxs[dx] * wyz;
+      const T* p = f4 + 4 * ((jxs[dx] * nf + jy) * nf + jz);  // nf <= 128: 32-bit offsets
+      if constexpr (sizeof(T) == 4) {
+        const float4 v = *reinterpret_cast<const float4*>(p);  // 16-byte aligned point
+        acc[0] += wx * v.x;
+        acc[1] += wx * v.y;
+        acc[2] += wx * v.z;
+      } else {
+        const double2 v = *reinterpret_cast<const double2*>(p);
+        acc[0] += wx * v.x;
+        acc[1] += wx * v.y;
+        acc[2] += wx * p[2];
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      ur[(size_t)pid * 3 + c] = acc[c];
+      if (r_out) r_out[(size_t)pid * 3 + c] = wrap2pi(r[(size_t)pid * 3 + c] + dt * (double)acc[c]);
     }
   }
 }
@@ -446,6 +547,35 @@ __global__ void k_zero_cx(cx<T>* p, size_t n) {
     p[i] = cx<T>{0, 0};
 }
 
+template <typename T, int W>
+void launch_gather3p_w(dim3 g, hipStream_t s, const T* pk, int nf, T beta, const double* r, int np, T* ur, double dt,
+                       double* r_out) {
+  hipLaunchKernelGGL((k_gather3p<T, W>), g, dim3(256), 0, s, pk, nf, beta, r, np, ur, dt, r_out);
+}
+template <typename T>
+void launch_gather3p(int w, dim3 g, hipStream_t s, const T* pk, int nf, T beta, const double* r, int np, T* ur,
+                     double dt, double* r_out) {
+  switch (w) {
+    case 2: launch_gather3p_w<T, 2>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
+    case 3: launch_gather3p_w<T, 3>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
+    case 4: launch_gather3p_w<T, 4>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
+    case 5: launch_gather3p_w<T, 5>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
+    case 6: launch_gather3p_w<T, 6>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
+    case 7: launch_gather3p_w<T, 7>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
+    case 8: launch_gather3p_w<T, 8>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
+    case 9: launch_gather3p_w<T, 9>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
+    case 10: launch_gather3p_w<T, 10>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
+    case 11: launch_gather3p_w<T, 11>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
+    case 12: launch_gather3p_w<T, 12>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
+    case 13: launch_gather3p_w<T, 13>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
+    case 14: launch_gather3p_w<T, 14>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
+    case 15: launch_gather3p_w<T, 15>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
+    case 16: launch_gather3p_w<T, 16>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
+    default: throw std::invalid_argument("NUFFT kernel width out of range");
+  }
+  QL_LAUNCHED();
+}
+
 size_t tsize(int dtype) { return dtype == QLLES_F32 ? sizeof(float) : sizeof(double); }
 
 template <typename T>
@@ -477,6 +607,7 @@ void setup_t(qlles_interp* it) {
     QL_HIP(hipMalloc(&it->coef, sizeof(cx<T>) * per_f * d));
     QL_HIP(hipMemset(it->coef, 0, sizeof(cx<T>) * per_f * d));
     QL_HIP(hipMalloc(&it->fine, sizeof(cx<T>) * per_f * d));
+    if (d == 3) QL_HIP(hipMalloc(&it->packed, sizeof(T) * 4 * per_f));
   } else {
     QL_HIP(hipMalloc(&it->coef, sizeof(cx<T>) * per_n * d));
   }
@@ -507,8 +638,12 @@ void run_t(qlles_interp* it, const cx<T>* uh, int Ns, int ld, size_t q, double s
       c2c_strided<T>(nf, co, fi, 3, nf * nf, false, (T)1, twf, s);
       c2c_strided<T>(nf, fi, fi, 3 * nf, nf, false, (T)1, twf, s);
       c2c_contig<T>(nf, fi, fi, 3 * nf * nf, false, (T)1, twf, s);
-      hipLaunchKernelGGL((k_gather<T, 3>), dim3((np + 3) / 4), dim3(256), 0, s, fi, nf, it->w, (T)it->beta, r, np,
-                         ur, dt, r_out);
+      const size_t cs = (size_t)nf * nf * nf;
+      T* pk = static_cast<T*>(it->packed);
+      hipLaunchKernelGGL((k_pack3<T>), dim3((int)std::min<size_t>((cs + 255) / 256, 4096)), dim3(256), 0, s, fi, pk,
+                         cs);
+      QL_LAUNCHED();
+      launch_gather3p<T>(it->w, dim3((np + 3) / 4), s, pk, nf, (T)it->beta, r, np, ur, dt, r_out);
     } else {
       c2c_strided<T>(nf, co, fi, 2, nf, false, (T)1, twf, s);
       c2c_contig<T>(nf, fi, fi, 2 * nf, false, (T)1, twf, s);
@@ -604,6 +739,7 @@ void destroy_interp(qlles_interp* it) {
   if (it->corr) (void)hipFree(it->corr);
   if (it->coef) (void)hipFree(it->coef);
   if (it->fine) (void)hipFree(it->fine);
+  if (it->packed) (void)hipFree(it->packed);
   if (it->tmp) (void)hipFree(it->tmp);
   delete it;
 }
