@@ -128,6 +128,13 @@ int qlles_dissipation(qlles_plan* plan, const void* u_hat, double* out, void* st
  * the forcing shell 0 < |k|^2 <= kf^2 (spectral_solver.py:94,:98). */
 int qlles_energy_hat(qlles_plan* plan, const void* u_hat, double* out, void* stream);
 
+/* energy_spectrum(vel, mul_fac, dim) (utils.py:35-97): shell-binned E(k) of
+ * the physical field u (3-D (3,N,N,N); 2-D (3,N,N,1), components 0-1 used)
+ * into ek[0..N), a device array of N doubles; FP64 binning with the
+ * reference's half-space weights (2 inside the halved axis, 1 on its 0 and
+ * N/2 planes), shell = int(|k| + 0.5), ek *= 4 pi k^2 / (samples + eps). */
+int qlles_energy_spectrum(qlles_plan* plan, const void* u, double mul_fac, double* ek, void* stream);
+
 /* spectral_filtering(u, ckp_N) (utils.py:100-138) evaluated directly from
  * u_hat: crop to [-n/2, n/2)^dim, scale (n/N)^dim, take the real part.
  * u_lres: (3,n,n,n) / (2,n,n) real, n = ckp_N <= N, n even. */

@@ -281,4 +281,60 @@ __global__ void k_window(const cx<T>* __restrict__ uh, int N, int dim, int ld, c
   }
 }
 
+// Shell-binned energy spectrum (utils.py:35-97) of an rfft-layout spectrum
+// vh ([ncomp][rows][N/2+1], rows = N*N in 3-D, N in 2-D; the halved axis is
+// the last one): per mode, e = fact * mul_fac * sum_c |vh_c|^2 / N^(2 dim)
+// with fact = 2 inside the halved axis and 1 on its 0 / N/2 planes, binned
+// by shell = int(|k| + 0.5) < nbins together with the sample count fact.
+// Each block accumulates its modes in LDS (FP64 atomics: the order inside a
+// block is not fixed) and writes its histogram; k_shell_final sums the
+// blocks in a fixed order.
+template <typename T>
+__global__ void __launch_bounds__(256) k_shell_bin(const cx<T>* __restrict__ vh, int N, int dim, int ncomp,
+                                                   double scale2, double mul_fac, int nbins,
+                                                   double* __restrict__ partials) {
+  extern __shared__ double s_h[];  // [2][nbins]
+  for (int i = threadIdx.x; i < 2 * nbins; i += blockDim.x) s_h[i] = 0.0;
+  __syncthreads();
+  const int Nh = N / 2 + 1;
+  const size_t rows = dim == 3 ? (size_t)N * N : (size_t)N;
+  const size_t total = rows * Nh, cs = total;
+  for (size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x; m < total; m += (size_t)gridDim.x * blockDim.x) {
+    const int kz = (int)(m % Nh);
+    const size_t r = m / Nh;
+    int k2;
+    if (dim == 3) {
+      const int kx = kfreq((int)(r / N), N), ky = kfreq((int)(r % N), N);
+      k2 = kx * kx + ky * ky + kz * kz;
+    } else {
+      const int kx = kfreq((int)r, N);
+      k2 = kx * kx + kz * kz;
+    }
+    const int shell = (int)(sqrt((double)k2) + 0.5);
+    if (shell >= nbins) continue;
+    const double fact = (kz > 0 && 2 * kz < N) ? 2.0 : 1.0;
+    double e = 0.0;
+    for (int c = 0; c < ncomp; ++c) {
+      const cx<T> v = vh[c * cs + m];
+      e += (double)v.re * v.re + (double)v.im * v.im;
+    }
+    atomicAdd(&s_h[shell], e * scale2 * fact * mul_fac);
+    atomicAdd(&s_h[nbins + shell], fact);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 2 * nbins; i += blockDim.x) partials[(size_t)blockIdx.x * 2 * nbins + i] = s_h[i];
+}
+
+static __global__ void k_shell_final(const double* __restrict__ partials, int nblocks, int nbins, double* __restrict__ ek) {
+  const double eps = 2.220446049250313e-16;
+  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nbins; b += gridDim.x * blockDim.x) {
+    double e = 0.0, n = 0.0;
+    for (int k = 0; k < nblocks; ++k) {
+      e += partials[(size_t)k * 2 * nbins + b];
+      n += partials[(size_t)k * 2 * nbins + nbins + b];
+    }
+    ek[b] = e * 4.0 * 3.14159265358979323846 * (double)b * (double)b / (n + eps);
+  }
+}
+
 }  // namespace qk

@@ -116,6 +116,11 @@ int qlles_filter(qlles_plan* p, const void* u_hat, int ckp_N, void* u_lres, void
   return guarded([&] { p->eng->filter(u_hat, ckp_N, u_lres, (hipStream_t)stream); });
 }
 
+int qlles_energy_spectrum(qlles_plan* p, const void* u, double mul_fac, double* ek, void* stream) {
+  if (!p || !u || !ek) return fail(QLLES_ERR_ARG, "null argument");
+  return guarded([&] { p->eng->spectrum(u, mul_fac, ek, (hipStream_t)stream); });
+}
+
 int qlles_plan_set_variant(qlles_plan* p, int xv, int zv) {
   if (!p) return fail(QLLES_ERR_ARG, "null plan");
   return guarded([&] { p->eng->set_variant(xv, zv); });

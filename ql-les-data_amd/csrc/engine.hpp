@@ -31,6 +31,8 @@ class Engine {
   virtual void dissipation(const void* uhat, double* out, hipStream_t s) = 0;
   virtual void energy_hat(const void* uhat, double* out, hipStream_t s) = 0;
   virtual void filter(const void* uhat, int n, void* ulres, hipStream_t s) = 0;
+  // shell-binned energy spectrum of the physical field u into ek[0..N) (FP64)
+  virtual void spectrum(const void* u, double mul_fac, double* ek, hipStream_t s) = 0;
   // kernel variants: xv = x / y-pass tile width, zv = z-pass (2-D y-pass)
   // radix plan; 2 choices each for grids up to 128, 1 above
   virtual int variants() const = 0;

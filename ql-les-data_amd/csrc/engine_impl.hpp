@@ -37,6 +37,7 @@ class EngineT final : public Engine {
   cx<T>* AS_[NAS] = {};
   T* AP_[NAP] = {};
   double* partials_ = nullptr;
+  double* spec_part_ = nullptr;  // per-block shell histograms (energy spectrum)
   double* alpha_ = nullptr;
   double* dscr_ = nullptr;
   const cx<T>* twN_ = nullptr;
@@ -594,6 +595,19 @@ class EngineT final : public Engine {
     QL_LAUNCHED();
     hipLaunchKernelGGL(k_energy_hat_final, dim3(1), dim3(RED_THREADS), 0, s, partials_, RED_BLOCKS, 1.0 / (nd * nd),
                        out);
+    QL_LAUNCHED();
+  }
+
+  void spectrum(const void* u_v, double mul_fac, double* ek, hipStream_t s) override {
+    cx<T>* vh = Y6;  // [3][rows][Nh] scratch
+    rfftn(u_v, vh, s);
+    constexpr int NBLK = 256;
+    if (!spec_part_) spec_part_ = alloc<double>((size_t)NBLK * 2 * N);
+    const double nd = DIM == 3 ? (double)N * N * N : (double)N * N;
+    hipLaunchKernelGGL((k_shell_bin<T>), dim3(NBLK), dim3(256), 2 * N * sizeof(double), s, vh, N, DIM,
+                       DIM == 3 ? 3 : 2, 1.0 / (nd * nd), mul_fac, N, spec_part_);
+    QL_LAUNCHED();
+    hipLaunchKernelGGL(k_shell_final, dim3((N + 255) / 256), dim3(256), 0, s, spec_part_, NBLK, N, ek);
     QL_LAUNCHED();
   }
 

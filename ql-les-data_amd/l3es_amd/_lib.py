@@ -34,7 +34,7 @@ EXPORTS = (
     "qlles_batch_configure", "qlles_batch_tune", "qlles_batch_get_config", "qlles_batch_run",
     "qlles_rhs_jvp", "qlles_rhs_vjp", "qlles_step_jvp", "qlles_step_vjp", "qlles_irfftn_adjoint",
     "qlles_rfftn_adjoint", "qlles_interp_hat_vjp", "qlles_plan_set_variant", "qlles_plan_get_variant",
-    "qlles_plan_tune",
+    "qlles_plan_tune", "qlles_energy_spectrum",
 )
 KINDS = ("x-pass", "y-inverse", "z-pass", "y-forward", "other")
 
@@ -107,6 +107,7 @@ def load():
         "qlles_plan_set_variant": (ci, [vp, ci, ci]),
         "qlles_plan_get_variant": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci)]),
         "qlles_plan_tune": (ci, [vp, vp, ci, pd, vp]),
+        "qlles_energy_spectrum": (ci, [vp, vp, cd, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -65,36 +65,21 @@ def kinetic_energy_hat(u_hat, kf=0):
 
 
 def energy_spectrum(vel, mul_fac=1.0, is_scalar_field=False, dim=3):
-    """Shell-binned E(k) (utils.py:35-97).
+    """Shell-binned E(k) (utils.py:35-97): transform and FP64 binning on the GPU (qlles_energy_spectrum).
 
-    The transform runs on the GPU; the shell binning (a diagnostic, n bins)
-    runs on the host in FP64 with the reference's weights.
+    Returns a float64 NumPy array of n bins like the reference.
     """
     del is_scalar_field
     if dim == 2:
-        v = vel[:2, :, :, 0] if vel.dim() == 4 else vel[:2]
-        full = torch.zeros((3,) + tuple(v.shape[1:]) + (1,), dtype=v.dtype, device=v.device)
-        full[:2, :, :, 0] = v
-        n = v.shape[1]
-        shape_hat = (3, n, n // 2 + 1)
+        if vel.dim() == 3:
+            vel = vel[..., None]
+        full = torch.zeros((3,) + tuple(vel.shape[1:3]) + (1,), dtype=vel.dtype, device=vel.device)
+        full[:2] = vel[:2]
     else:
         full = vel.contiguous()
-        n = vel.shape[1]
-        shape_hat = (3, n, n, n // 2 + 1)
+    n = full.shape[1]
     plan = _lib.Plan(dim, n, full.dtype, 0.0, 1.0)
-    cdt = torch.complex64 if full.dtype == torch.float32 else torch.complex128
-    vh = torch.empty(shape_hat, dtype=cdt, device=full.device)
-    _lib.check(_lib.load().qlles_rfftn(plan.handle, _lib.ptr(full), _lib.ptr(vh), _lib.stream_ptr()))
-    a = (vh.abs() ** 2).sum(0).double().cpu().numpy() / float(n ** dim) ** 2
-    k = np.fft.fftfreq(n, 1.0 / n)
-    kh = np.abs(k[: n // 2 + 1])
-    if dim == 3:
-        K = np.sqrt(k[:, None, None] ** 2 + k[None, :, None] ** 2 + kh[None, None, :] ** 2)
-        fact = np.broadcast_to(np.where((kh > 0) & (kh < n // 2), 2, 1)[None, None, :], K.shape)
-    else:
-        K = np.sqrt(k[:, None] ** 2 + kh[None, :] ** 2)
-        fact = np.broadcast_to(np.where((kh > 0) & (kh < n // 2), 2, 1)[None, :], K.shape)
-    shell = (K + 0.5).astype(int).ravel()
-    n_samples = np.bincount(shell, weights=fact.ravel(), minlength=n)[:n]
-    ek = np.bincount(shell, weights=(a * fact * mul_fac).ravel(), minlength=n)[:n]
-    return ek * 4 * np.pi * np.arange(n) ** 2 / (n_samples + EPS)
+    ek = torch.empty(n, dtype=torch.float64, device=full.device)
+    _lib.check(_lib.load().qlles_energy_spectrum(plan.handle, _lib.ptr(full.contiguous()), float(mul_fac),
+                                                 _lib.ptr(ek), _lib.stream_ptr()))
+    return ek.cpu().numpy()
