@@ -207,14 +207,17 @@ def set_up_solver(N, nu, dim, case, dt, seed, ckp_N, kf, e_kin_target=None, forc
 
 
 def write_u(u_hat, i, dst_path, plan, ckp_N, suffix=""):
-    """Spectrally filtered velocity checkpoint u_{ckp_N}_{i:05d}{suffix}.npy (utils.py:141-166)."""
+    """Spectrally filtered velocity checkpoint u_{ckp_N}_{i:05d}{suffix}.npy (utils.py:141-166):
+    (3, n, n, n) in 3-D, (2, n, n) in 2-D, float64 like the reference's x64 arrays."""
     from .utils import spectral_filtering_hat
 
+    if ckp_N > plan.N or ckp_N % 2 or plan.N % 2:
+        raise ValueError("ckp_N must be even and <= N (utils.py:157-158)")
     os.makedirs(dst_path, exist_ok=True)
     u_lres = spectral_filtering_hat(u_hat, ckp_N, plan)
-    arr = u_lres.cpu().numpy()
+    arr = u_lres.double().cpu().numpy()
     if plan.dim == 2:
-        arr = arr[..., None]
+        arr = arr[:2]
     np.save(os.path.join(dst_path, f"u_{ckp_N}_{i:05d}{suffix}.npy"), arr)
 
 

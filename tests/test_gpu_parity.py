@@ -384,3 +384,23 @@ def test_energy_spectrum_gpu(gpu, dim, N, prec):
     ek = energy_spectrum(dev(u, tdt(prec)), mul_fac=1.5, dim=dim)
     assert ek.shape == ref.shape
     assert np.abs(ek - ref).max() <= (1e-12 if prec == "float64" else 2e-6) * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("dim,N,n", [(3, 32, 16), (2, 64, 32)])
+def test_write_u_frames(gpu, tmp_path, dim, N, n):
+    """u_{ckp_N}_{step:05d}.npy frames (utils.py:141-166): shape and values of spectral_filtering."""
+    from l3es_amd import _lib
+    from l3es_amd.spectral_solver import write_u
+
+    rng = np.random.default_rng(7)
+    shape = (3, N, N, N) if dim == 3 else (3, N, N, 1)
+    u = rng.standard_normal(shape)
+    if dim == 2:
+        u[2] = 0.0
+    uh = np.fft.rfftn(u, axes=(1, 2, 3)) if dim == 3 else np.fft.rfftn(u[..., 0], axes=(1, 2))
+    plan = _lib.Plan(dim, N, torch.float64, 0.0, 1.0)
+    write_u(dev(uh), 7, str(tmp_path), plan, n)
+    arr = np.load(tmp_path / f"u_{n}_00007.npy")
+    ref = O.spectral_filtering(u if dim == 3 else u[:2, :, :, 0], n)
+    assert arr.shape == ref.shape
+    assert np.abs(arr - ref).max() < 1e-12 * np.abs(ref).max()
