@@ -404,3 +404,49 @@ def test_write_u_frames(gpu, tmp_path, dim, N, n):
     ref = O.spectral_filtering(u if dim == 3 else u[:2, :, :, 0], n)
     assert arr.shape == ref.shape
     assert np.abs(arr - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+# ---------------------------------------------------------------------------
+# the benchmark configurations that run as parity cases
+# ---------------------------------------------------------------------------
+def test_config_tgv64_fp64_dt1e3(gpu):
+    """TGV 64^3 FP64, nu = 1/1600, dt = 1e-3, 10 RK4 steps: kinetic energy and full fields vs the oracle."""
+    ug, uhg, _, _ = _run_gpu(64, 1.0 / 1600, 3, "TGV3D", 1e-3, 10, "float64", forcing_type="none")
+    uo, uho, _, _ = _run_oracle(64, 1.0 / 1600, 3, "TGV3D", 1e-3, 10)
+    eg = 0.5 * np.mean(np.sum(ug * ug, axis=0))
+    eo = 0.5 * np.mean(np.sum(uo * uo, axis=0))
+    assert abs(eg - eo) < 1e-13 * eo
+    assert rel(ug, uo) < 1e-12 and rel(uhg, uho) < 1e-12
+
+
+@pytest.mark.parametrize("prec,tf,tr", [("float64", 1e-11, 2e-7), ("float32", 2e-5, 2e-5)])
+def test_config_kolmogorov256_particles(gpu, prec, tf, tr):
+    """2-D Kolmogorov 256^2 (f_x = sin 4y, drag 0.1, nu = 1e-3, dt = 1e-3) with 64^2 jittered particles
+    advected by the NUFFT (tol 1e-6): 20 combined-mode steps against the oracle's step + filter + DFT
+    interpolation + shift chain (combined.py:97,127,130,199)."""
+    from l3es_amd import _lib, set_up_solver
+    from l3es_amd.integrator import spectral_interpolator_wrapper
+
+    N, n, K, dt = 256, 64, 20, 1e-3
+    kw = dict(forcing_type="kolmogorov", drag=0.1, kolm_amp=1.0, kolm_k=4)
+    u0 = O.init_u_kolm_np(N, seed=3)
+    _, uh, _, _, step, _, _, _, _ = set_up_solver(N, 1e-3, 2, "Kolm", dt, 3, n, None, precision=prec,
+                                                  warmup=False, u_init=u0, **kw)
+    it = spectral_interpolator_wrapper(n, (1, 2), backend="nufft", precision=prec, tol=1e-6)
+    r0 = O.pos_init_jittered(n, 2, seed=5)
+    r = dev(r0)
+    ur = torch.empty((r.shape[0], 2), dtype=tdt(prec), device=r.device)
+    lib = _lib.load()
+    _lib.check(lib.qlles_step_advect(step.plan.handle, it.it.handle, None, _lib.ptr(uh), K, _lib.ptr(r), r.shape[0],
+                                     _lib.ptr(ur), None, _lib.stream_ptr()))
+    torch.cuda.synchronize()
+    uo, uho, so, _, _ = O.set_up_solver(N, 1e-3, 2, "Kolm", dt, seed=3, warmup=False, u_init=u0, **kw)
+    ro = r0.copy()
+    for _ in range(K):
+        uo, uho, _ = so(uo, uho)
+        uro = O.interpolate_dft(O.spectral_filtering(uo[:2, :, :, 0], n), ro)
+        ro = O.shift_fn(ro, dt * uro)
+    assert rel(host(uh), uho) < tf
+    d = np.abs(host(r) - ro)
+    d = np.minimum(d, 2 * np.pi - d)  # periodic distance
+    assert d.max() < tr
