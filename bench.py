@@ -304,6 +304,26 @@ def run_adjoint(args):
     print(json.dumps(out), flush=True)
 
 
+def energy_trajectory(args, precision, nsamp=10, every=10):
+    """FP64 Parseval kinetic energy every `every` steps over nsamp samples from the same initial field
+    (untimed), for the FP32-vs-FP64 energy-trajectory error."""
+    import torch
+
+    from l3es_amd import set_up_solver
+    from l3es_amd.utils import kinetic_energy_hat
+
+    _, uh, _, _, step, _, _, _, _ = set_up_solver(
+        args.N, args.nu, 3, "HIT", args.dt, args.seed, args.ckp_N, args.kf, args.e_kin, "power",
+        precision=precision, spectrum="k4exp", power=args.power, warmup=False)
+    out = [kinetic_energy_hat(uh)[0]]
+    for _ in range(nsamp):
+        step.advance(None, uh, every, None, use_u=False)
+        out.append(kinetic_energy_hat(uh)[0])
+    del uh, step
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(args):
     """The FP64 NumPy/SciPy restatement (oracle) timed on this host's cores, bounded sample."""
     import numpy as np
@@ -411,6 +431,12 @@ def main():
         ke_err = abs(e32 - e64) / abs(e64)
         fp64 = dict(steps_per_s=sps64, ms_per_step=1000.0 / sps64, speedup_fp32_over_fp64=sps32 / sps64,
                     e_kin=e64, finite=ok64)
+        t32 = energy_trajectory(args, "float32")
+        t64 = energy_trajectory(args, "float64")
+        traj_err = max(abs(a - b) / abs(b) for a, b in zip(t32, t64))
+        fp64["ke_trajectory_fp32"] = t32
+        fp64["ke_trajectory_fp64"] = t64
+        fp64["ke_trajectory_max_rel_err"] = traj_err
     N = args.N
     s = 4
     C = 2 * s * N * N * (N // 2 + 1)
@@ -434,6 +460,7 @@ def main():
                     traffic=traffic, traffic_unit="bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
                     traffic_detail=tinfo,
                     alg_bytes_per_launch=kernels[dom]["alg_bytes_per_launch"], avg_us=kernels[dom]["avg_us"])
+    cpu = None if args.no_cpu else cpu_baseline(args)
     out = {
         "metric": METRIC,
         "value": sps32,
@@ -460,7 +487,8 @@ def main():
         "ke_rel_err_fp32_vs_fp64": ke_err,
         "ke_horizon_steps": horizon,
         "finite": ok32,
-        "cpu_baseline": None if args.no_cpu else cpu_baseline(args),
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu_baseline": (sps32 / cpu["value"]) if cpu else None,
     }
     print(json.dumps(out), flush=True)
 
