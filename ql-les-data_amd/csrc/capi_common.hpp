@@ -67,7 +67,8 @@ inline void tune_plan(qlles_plan* p, const void* u_hat, int trials, double* best
   void* tmp = nullptr;
   QL_HIP(hipMalloc(&tmp, bytes));
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  int bx = 0, bz = 0;
+  int bx = 0, bz = 0, ox = 0, oz = 0;
+  p->eng->get_variant(&ox, &oz);
   double best = -1.0;
   try {
     QL_HIP(hipEventCreate(&e0));
@@ -96,6 +97,7 @@ inline void tune_plan(qlles_plan* p, const void* u_hat, int trials, double* best
     if (e1) (void)hipEventDestroy(e1);
     (void)hipStreamSynchronize(s);
     (void)hipFree(tmp);
+    p->eng->set_variant(ox, oz);  // leave the plan as it was
     throw;
   }
   (void)hipEventDestroy(e0);
