@@ -144,17 +144,39 @@ def comp_dt(u, dx, nu, cfl=1.0, plan=None):
 def custom_jit_integrate(path_pref, rejit, integrate_fn, u, u_hat):
     """Compile/cache hook (spectral_solver.py:130-153).
 
-    The HIP kernels are compiled ahead of time into libqlles.so and the plan is
-    built when the step is created, so there is nothing to trace or cache.
+    The HIP kernels are compiled ahead of time into libqlles.so, so the part of
+    the reference's compile step that remains is the choice of kernel variants
+    (x/y tile width, z radix plan).  rejit or no ``{path_pref}.json``: time the
+    variants on a scratch copy of u_hat (qlles_plan_tune) and store the choice;
+    otherwise load it.  path_pref None: tune without storing.
     """
-    del path_pref, rejit, u, u_hat
+    del u
+    import json
+
+    plan = integrate_fn.plan
+    path = None if path_pref is None else f"{path_pref}.json"
+    if rejit or path is None or not os.path.exists(path):
+        us = plan.tune(u_hat.contiguous(), trial_steps=4)
+        xv, zv, nv = plan.variant()
+        if path is not None:
+            os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
+            with open(path, "w") as f:
+                json.dump(dict(xv=xv, zv=zv, variants=nv, us_per_step=us, N=plan.N, dim=plan.dim,
+                               dtype=str(plan.dtype)), f)
+    else:
+        with open(path) as f:
+            cfg = json.load(f)
+        plan.set_variant(int(cfg["xv"]), int(cfg["zv"]))
     return integrate_fn
 
 
 def set_up_solver(N, nu, dim, case, dt, seed, ckp_N, kf, e_kin_target=None, forcing_type="ekin_tot",
                   rejit=True, precision="float64", viscous_mode="explicit", spectrum="k53", power=0.1,
-                  drag=0.0, kolm_amp=1.0, kolm_k=4, u_init=None, warmup=True):
-    """Solver setup (spectral_solver.py:156-230); returns the reference's 9-tuple."""
+                  drag=0.0, kolm_amp=1.0, kolm_k=4, u_init=None, warmup=True, cache_dir=False):
+    """Solver setup (spectral_solver.py:156-230); returns the reference's 9-tuple.
+
+    The kernel variants are tuned here (rejit) or loaded from the ``.cache/`` file
+    when ``cache_dir`` is true and ``rejit`` false (the reference's compile cache)."""
     dtype = _dtype(precision)
     dev = _dev()
     L = 2 * np.pi
@@ -199,7 +221,15 @@ def set_up_solver(N, nu, dim, case, dt, seed, ckp_N, kf, e_kin_target=None, forc
     integrate_fn = rk4_wrapper(dt, rhs_wrapper(N, nu, fft_axes, precision, rhs_kind, kf or 0, power,
                                                kolm_k, kolm_amp, drag),
                                forcing_mask, e_kin_init, forcing_type, fft_axes, viscous_mode)
-    integrate_fn = custom_jit_integrate(None, rejit, integrate_fn, u, u_hat)
+    # cache name as spectral_solver.py:218-221, plus the precision
+    device_type = "gpu"
+    if forcing_type != "none":
+        path_pref = f".cache/spectral_{device_type}_{N}_{dim}_{kf}_{forcing_type}"
+    else:
+        path_pref = f".cache/spectral_{device_type}_{N}_{dim}"
+    if dtype == torch.float32:
+        path_pref += "_f32"
+    integrate_fn = custom_jit_integrate(path_pref if cache_dir else None, rejit, integrate_fn, u, u_hat)
     e_inj = torch.zeros((), dtype=torch.float64, device=dev)
     if warmup:  # the reference advances one step during setup (:226)
         u, u_hat, e_inj = integrate_fn(u, u_hat)
