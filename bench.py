@@ -319,9 +319,10 @@ def energy_trajectory(args, precision, nsamp=10, every=10):
     for _ in range(nsamp):
         step.advance(None, uh, every, None, use_u=False)
         out.append(kinetic_energy_hat(uh)[0])
+    final = uh.to(torch.complex128)
     del uh, step
     torch.cuda.empty_cache()
-    return out
+    return out, final
 
 
 def cpu_baseline(args):
@@ -431,9 +432,14 @@ def main():
         ke_err = abs(e32 - e64) / abs(e64)
         fp64 = dict(steps_per_s=sps64, ms_per_step=1000.0 / sps64, speedup_fp32_over_fp64=sps32 / sps64,
                     e_kin=e64, finite=ok64)
-        t32 = energy_trajectory(args, "float32")
-        t64 = energy_trajectory(args, "float64")
+        t32, f32 = energy_trajectory(args, "float32")
+        t64, f64 = energy_trajectory(args, "float64")
         traj_err = max(abs(a - b) / abs(b) for a, b in zip(t32, t64))
+        import torch
+
+        fp64["field_rel_l2_err_fp32_vs_fp64"] = float(torch.linalg.vector_norm(f32 - f64) /
+                                                      torch.linalg.vector_norm(f64))
+        del f32, f64
         fp64["ke_trajectory_fp32"] = t32
         fp64["ke_trajectory_fp64"] = t64
         fp64["ke_trajectory_max_rel_err"] = traj_err
