@@ -352,6 +352,114 @@ def cpu_baseline(args):
                        f"{N}^3 power-forced HIT field after 1 warm-up step; particles excluded")
 
 
+RELAX_CASES = (  # (dim, particles per axis, dt, u_ref, relax_dt_factor): hit32_1.yaml, kolm64_1.yaml
+    (3, 32, 0.002, 4.0, 2.0),
+    (2, 64, 0.001, 6.0, 1.4142135624),
+)
+# FP64 operations per evaluated neighbour pair of the relaxation pass: inside the
+# cutoff (displacement 12, distance 6, unit vector 4, kernel gradient 20, its
+# vector 3, Tait pressures 2 + pair pressure 5, prefactor 6, accumulation 21),
+# and for a cell-list candidate outside it (displacement and distance, compare)
+RELAX_FLOP_IN, RELAX_FLOP_OUT = 79, 19
+
+
+def run_relax(args):
+    """SPH relaxation branch of the combined loop (combined.py:132-142, :199) at the dataset
+    configurations: 32^3 particles in 3-D (hit32_1.yaml) and 64^2 in 2-D (kolm64_1.yaml), on a
+    jittered lattice (the reference's relaxed .h5 initial states are not available)."""
+    import numpy as np
+    import torch
+
+    from l3es_amd import relax_advect, relax_wrapper, rho_computer
+
+    torch.cuda.set_device(0)
+    L = 2 * np.pi
+    results = []
+    for dim, n, dt, u_ref, fac in RELAX_CASES:
+        rng = np.random.default_rng(args.seed)
+        r0 = (np.stack(np.meshgrid(*([np.arange(n)] * dim), indexing="ij"), -1).reshape(-1, dim) + 0.5) * (L / n)
+        r0 = np.mod(r0 + (rng.random(r0.shape) * 2 - 1) * 0.25 * L / n, L)
+        u0 = rng.normal(size=r0.shape) * 0.5 * u_ref
+        r = torch.from_numpy(r0).cuda()
+        u = torch.from_numpy(u0).cuda()
+        relax_fn, _ = relax_wrapper(n, dim, L, is_physical=True, u_ref=u_ref)
+        comp_rho = rho_computer(n, dim, L)
+        for _ in range(args.warmup):
+            relax_advect(relax_fn, r, u, dt, fac)
+            comp_rho(r)
+        torch.cuda.synchronize()
+        K = args.steps
+        t0 = time.perf_counter()
+        for _ in range(K):
+            relax_advect(relax_fn, r, u, dt, fac)
+        torch.cuda.synchronize()
+        t_relax = (time.perf_counter() - t0) / K
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ev[0].record()
+        for _ in range(K):
+            relax_advect(relax_fn, r, u, dt, fac)
+        ev[1].record()
+        for _ in range(K):
+            comp_rho(r)
+        ev[2].record()
+        torch.cuda.synchronize()
+        gpu_relax = ev[0].elapsed_time(ev[1]) / K * 1e-3
+        gpu_rho = ev[1].elapsed_time(ev[2]) / K * 1e-3
+        # pair statistics of the final state (cell list candidates and pairs within the cutoff)
+        rh = r.cpu().numpy()
+        from oracle import sph as osph
+
+        nc = (n - 1) // 3
+        cells = np.minimum((rh / (L / nc)).astype(np.int64), nc - 1)
+        cid = np.ravel_multi_index(tuple(cells.T), (nc,) * dim)
+        cnt = np.bincount(cid, minlength=nc ** dim).reshape((nc,) * dim)
+        nb = cnt.copy().astype(np.int64) * 0
+        for off in np.ndindex(*((3,) * dim)):
+            nb += np.roll(cnt, tuple(o - 1 for o in off), axis=tuple(range(dim)))
+        cand = int((cnt * nb).sum())
+        i_s, _ = osph.neighbor_pairs(rh, 3 * L / n, L, method="kdtree")
+        pairs = len(i_s)
+        flop = RELAX_FLOP_IN * pairs + RELAX_FLOP_OUT * (cand - pairs)
+        t_acc = max(gpu_relax - gpu_rho, 1e-9)  # the acceleration kernel: relax call minus the density call
+        cpu = None
+        if not args.no_cpu:
+            rs, us = r0.copy(), u0.copy()
+            osph.relax_advect(rs, us, dt, fac, n, dim, L, u_ref, method="kdtree")  # warm (imports)
+            c0 = time.perf_counter()
+            reps = 0
+            while time.perf_counter() - c0 < args.cpu_seconds / len(RELAX_CASES) or reps == 0:
+                rs, us, _ = osph.relax_advect(rs, us, dt, fac, n, dim, L, u_ref, method="kdtree")
+                reps += 1
+            cpu = (time.perf_counter() - c0) / reps
+        res = {"dim": dim, "particles": n ** dim, "relax_dt_factor": fac, "dt": dt, "u_ref": u_ref,
+               "wall_us_per_call": 1e6 * t_relax, "gpu_us_per_call": 1e6 * gpu_relax,
+               "density_us_per_call": 1e6 * gpu_rho, "accel_kernel_us": 1e6 * t_acc,
+               "candidate_pairs": cand, "pairs_within_cutoff": pairs,
+               "accel_fp64_tflops": flop / t_acc * 1e-12, "cpu_oracle_s_per_call": cpu,
+               "speedup_vs_cpu": (cpu / t_relax) if cpu else None}
+        log(f"[bench] relax {dim}-D {n}^{dim}: {1e6 * t_relax:.1f} us/call (density {1e6 * gpu_rho:.1f} us), "
+            f"{pairs} pairs, accel ~{res['accel_fp64_tflops']:.2f} TFLOP/s, cpu {cpu}")
+        results.append(res)
+    main_case = results[0]
+    peak = 78.6
+    out = {
+        "metric": "SPH relaxation calls/s (combined.py:132-142,199) at 32^3 particles, 3-D",
+        "value": 1.0 / (main_case["wall_us_per_call"] * 1e-6), "unit": "calls/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": main_case["wall_us_per_call"] * 1e-3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "shift by dt u, cell-list density + pressure/background acceleration (quintic "
+                               "kernel, Tait EoS), velocity correction and shift; 32^3 particles (hit32_1) and "
+                               "64^2 (kolm64_1) on a jittered lattice"},
+        "roofline": {"bound": "fp64", "achieved": main_case["accel_fp64_tflops"], "peak": peak,
+                     "unit": "TFLOP/s", "frac": main_case["accel_fp64_tflops"] / peak, "traffic": None},
+        "cpu_baseline": None if main_case["cpu_oracle_s_per_call"] is None else {
+            "value": 1.0 / main_case["cpu_oracle_s_per_call"], "unit": "calls/s", "cores": 1, "kind": "port",
+            "sample": "oracle relax_advect (NumPy, scipy cKDTree neighbour search) on the same 32^3 state"},
+        "cases": results,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -374,9 +482,9 @@ def main():
     ap.add_argument("--copy-probe-mib", dest="copy_probe", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (counter calibration)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH/WRITE passes")
-    ap.add_argument("--workload", choices=("hit256", "batch", "adjoint"), default="hit256",
+    ap.add_argument("--workload", choices=("hit256", "batch", "adjoint", "relax"), default="hit256",
                     help="hit256: the headline 256^3 step (default); batch: 24 uneven batched trajectories; "
-                         "adjoint: gradient through K checkpointed steps")
+                         "adjoint: gradient through K checkpointed steps; relax: SPH particle relaxation")
     ap.add_argument("--adj-N", dest="adj_N", type=int, default=128)
     ap.add_argument("--adj-K", dest="adj_K", type=int, default=50)
     ap.add_argument("--adj-every", dest="adj_every", type=int, default=10)
@@ -415,6 +523,9 @@ def main():
         return
     if args.workload == "adjoint":
         run_adjoint(args)
+        return
+    if args.workload == "relax":
+        run_relax(args)
         return
 
     import torch

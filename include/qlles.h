@@ -251,6 +251,40 @@ int qlles_batch_run(qlles_batch* b, const int* nsteps, int nsteps_all, void* str
 int qlles_interp_hat_vjp(qlles_interp* it, int N, const void* u_hat, const double* r, int np, const void* u_bar,
                          void* lam_hat, double* r_bar, void* stream);
 
+/* ---- SPH density and particle relaxation ---------------------------------
+ * An SPH context = relax_wrapper(n, dim, box, is_physical, u_ref)
+ * (relax.py:16-74) and rho_computer(n, dim, box) (utils.py:169-206): the
+ * Morris quintic kernel with h = box/n (support 3h), particle mass
+ * dx^dim rho_ref, Tait EoS p = p_eos (rho/rho_ref - 1) with
+ * p_eos = (10 u_ref / l_ref)^2 rho_ref, l_ref = 1 (is_physical) or dx.
+ * Neighbours are every particle, itself included, closer than 3h under the
+ * periodic minimum image (the reference's jax_md neighbour list with
+ * mask_self=False); a cell grid replaces the list.  Positions r are (np, dim)
+ * FP64 in physical units; all outputs are FP64.  7 <= n <= 256 (3-D),
+ * 4096 (2-D); np may differ from n^dim (the workspace grows on demand).
+ * Results do not depend on run-to-run scheduling (fixed summation order). */
+typedef struct qlles_sph qlles_sph;
+int qlles_sph_create(int dim, int n, double box, int is_physical, double u_ref, qlles_sph** out);
+int qlles_sph_destroy(qlles_sph* sph);
+/* comp_rho(r) (utils.py:208-216): rho (np,) = m sum_j W(|r_ij|) */
+int qlles_sph_density(qlles_sph* sph, const double* r, int np, double* rho, void* stream);
+/* loop_body(r) (relax.py:76-120): acc (np, dim) = pressure + 0.5 background
+ * (transport-velocity) acceleration in physical units; rho (optional) gets
+ * the density it used */
+int qlles_sph_relax_acc(qlles_sph* sph, const double* r, int np, double* acc, double* rho, void* stream);
+/* sph_body(r, u) (relax.py:123-176; is_physical only): acc_p, acc_visc,
+ * acc_tvf (np, dim) for velocities u (np, dim) FP64 */
+int qlles_sph_forces(qlles_sph* sph, const double* r, const double* u, int np, double* acc_p, double* acc_visc,
+                     double* acc_tvf, void* stream);
+/* The relaxation branch of the combined loop plus its particle shift
+ * (combined.py:132-142, :199; is_physical only), in place:
+ *   r_t = shift(r, dt u_r); a = loop_body(r_t); r_t = shift(r_t, (factor dt)^2 a)
+ *   u_r = displacement(r_t, r) / dt;  r = shift(r, dt u_r)
+ * u_r (np, dim) is in u_dtype (QLLES_F32 / QLLES_F64; the shift uses the FP64
+ * value); acc (optional) receives a. */
+int qlles_sph_relax_advect(qlles_sph* sph, double* r, int np, void* u_r, int u_dtype, double dt, double factor,
+                           double* acc, void* stream);
+
 /* message of the last failed call on this thread */
 const char* qlles_last_error(void);
 

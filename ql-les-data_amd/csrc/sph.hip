@@ -1,0 +1,737 @@
+// SPH density and particle relaxation (relax.py:16-178, utils.py:169-218,
+// combined.py:132-142,199).
+//
+// The reference sums the Morris quintic kernel (support 3h) over a jax_md
+// neighbour list.  Here each call bins the particles into a periodic cell
+// grid of nc >= 2 cells per axis, with cells no smaller than the cutoff:
+//   1. k_sph_bin      cell of every particle (+ the shifted positions of the
+//                     fused relax/advect entry), per-cell counts (atomics)
+//   2. k_sph_scan     exclusive scan of the counts (one block)
+//   3. k_sph_scatter  particle ids into their cell's range (atomics)
+//   4. k_sph_rank     each id's rank among its cell's ids gives its slot, so
+//                     cells are in ascending id order; positions (and
+//                     velocities) gathered into cell order, SoA FP64
+//   5. k_sph_pass     density rho_i = m sum_j W(r_ij) over the 3^dim cells
+//                     around i, one workgroup per cell, candidates in LDS
+//   6. k_sph_pass     pressure / background / viscous accelerations, and for
+//                     the relax entry the velocity correction and the shift
+// Step 4 makes the order of every sum independent of the atomics, so results
+// are bit-reproducible from run to run.  In steps 5-6 LPP consecutive lanes
+// share one particle: lane l takes candidates l, l+LPP, ... of the cell's
+// neighbourhood, and the LPP partial sums are combined by an xor butterfly
+// (equal bits on every lane).  Everything is FP64, as in the reference.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "capi_common.hpp"
+
+using qlapi::fail;
+using qlapi::guarded;
+
+namespace {
+
+struct SphGeom {
+  double box;      // periodic side in working units (L / l_ref)
+  double half;     // box / 2
+  double inv_cell; // cells per working length unit
+  double cutoff;   // 3 h
+  double cutoff2;  // (3 h)^2
+  double inv_h;    // 1 / h
+  double sigma;    // kernel normalisation
+  double mass;     // particle mass dx^dim rho_ref
+  double p_eos;    // reference pressure (10 u_ref / l_ref)^2 rho_ref
+  double lref;     // length unit
+  int nc;          // cells per axis
+  int cap;         // stride of the SoA arrays
+};
+
+}  // namespace
+
+struct qlles_sph {
+  int dim = 0, n = 0;
+  double L = 0.0;
+  SphGeom g{};
+  int ncells = 0;
+  int cap = 0;
+  int* cnt = nullptr;      // [ncells]
+  int* start = nullptr;    // [ncells + 1]
+  int* fill = nullptr;     // [ncells]
+  int* perm = nullptr;     // [cap] particle id per cell-ordered slot
+  int* cell_of = nullptr;  // [cap] cell per particle id
+  int* tmp = nullptr;      // [cap] ids scattered into their cell's range, any order
+  double* xw = nullptr;    // [cap][dim] wrapped positions in working units, by particle id
+  double* rt = nullptr;    // [cap][dim] shifted physical positions (relax entry)
+  double* pos = nullptr;   // [dim][cap] cell-ordered positions
+  double* vel = nullptr;   // [dim][cap] cell-ordered velocities (forces entry)
+  double* rho = nullptr;   // [cap] cell-ordered densities
+  std::vector<void*> cell_bufs, part_bufs;
+};
+
+namespace {
+
+constexpr double kEps = 2.220446049250313e-16;  // finfo(float64).eps, relax.py:13
+
+__device__ __forceinline__ double ipow4(double x) {
+  const double x2 = x * x;
+  return x2 * x2;
+}
+__device__ __forceinline__ double ipow5(double x) { return x * ipow4(x); }
+
+// kernel_fn.w, kernel_fn.grad_w of the quintic spline
+__device__ __forceinline__ double quintic_w(double r, const SphGeom& g) {
+  const double q = r * g.inv_h;
+  const double q1 = fmax(0.0, 1.0 - q), q2 = fmax(0.0, 2.0 - q), q3 = fmax(0.0, 3.0 - q);
+  return g.sigma * (ipow5(q3) - 6.0 * ipow5(q2) + 15.0 * ipow5(q1));
+}
+__device__ __forceinline__ double quintic_dw(double r, const SphGeom& g) {
+  const double q = r * g.inv_h;
+  const double q1 = fmax(0.0, 1.0 - q), q2 = fmax(0.0, 2.0 - q), q3 = fmax(0.0, 3.0 - q);
+  return g.sigma * g.inv_h * (-5.0 * ipow4(q3) + 30.0 * ipow4(q2) - 75.0 * ipow4(q1));
+}
+
+// jnp.mod(x, side) for side > 0 (space.periodic shift)
+__device__ __forceinline__ double mod_side(double x, double side) {
+  const double m = fmod(x, side);
+  return m < 0.0 ? m + side : m;
+}
+// displacement mod(d + side/2, side) - side/2 for |d| <= side (both points
+// wrapped into [0, side]); the two branches equal fmod's exact result
+__device__ __forceinline__ double min_image(double d, const SphGeom& g) {
+  double t = d + g.half;
+  if (t >= g.box) t -= g.box;
+  else if (t < 0.0) t += g.box;
+  return t - g.half;
+}
+
+__device__ __forceinline__ double wrap_box(double x, double box) {
+  const double w = x - box * floor(x / box);
+  return w;
+}
+
+__device__ __forceinline__ int cell_coord(double x, const SphGeom& g) {
+  int c = (int)floor(x * g.inv_cell);
+  return c < 0 ? 0 : (c >= g.nc ? g.nc - 1 : c);
+}
+
+// 1. cells (+ optional shift r + dt u, physical units)
+template <int DIM>
+__global__ void k_sph_bin(const double* __restrict__ r, const void* __restrict__ u, int u_f32, double dt, double L,
+                          int np, SphGeom g, double* __restrict__ rt, double* __restrict__ xw,
+                          int* __restrict__ cell_of, int* __restrict__ cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  int c = 0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const size_t e = (size_t)i * DIM + d;
+    double x = r[e];
+    if (u) {
+      const double v = u_f32 ? (double)static_cast<const float*>(u)[e] : static_cast<const double*>(u)[e];
+      x = mod_side(x + dt * v, L);  // combined.py:136
+      rt[e] = x;
+    }
+    const double w = wrap_box(x / g.lref, g.box);  // relax.py:78
+    xw[e] = w;
+    c = c * g.nc + cell_coord(w, g);
+  }
+  cell_of[i] = c;
+  atomicAdd(&cnt[c], 1);
+}
+
+// 2. exclusive scan (one block of 1024); clears the scatter cursors and the
+// counts for the next call
+__global__ __launch_bounds__(1024) void k_sph_scan(int* __restrict__ cnt, int* __restrict__ start,
+                                                   int* __restrict__ fill, int ncells) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int per = (ncells + 1023) / 1024;
+  const int b = min(t * per, ncells), e = min(b + per, ncells);
+  int s = 0;
+  for (int k = b; k < e; ++k) s += cnt[k];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = t ? part[t - 1] : 0;
+  for (int k = b; k < e; ++k) {
+    start[k] = run;
+    run += cnt[k];
+    cnt[k] = 0;
+    fill[k] = 0;
+  }
+  if (t == 1023) start[ncells] = part[1023];
+}
+
+// 3. ids into their cell's range (the order inside a cell is fixed in step 4)
+__global__ void k_sph_scatter(const int* __restrict__ cell_of, const int* __restrict__ start, int* __restrict__ fill,
+                              int* __restrict__ tmp, int np) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const int c = cell_of[i];
+  tmp[start[c] + atomicAdd(&fill[c], 1)] = i;
+}
+
+// 4. one thread per particle: its rank among the ids of its cell gives its
+// slot, so every cell ends up in ascending id order; cell-ordered SoA gathers
+template <int DIM>
+__global__ void k_sph_rank(const int* __restrict__ cell_of, const int* __restrict__ start,
+                           const int* __restrict__ tmp, const double* __restrict__ xw, const double* __restrict__ u,
+                           int np, int cap, int* __restrict__ perm, double* __restrict__ pos,
+                           double* __restrict__ vel) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const int c = cell_of[i];
+  const int b = start[c], e = start[c + 1];
+  int rank = 0;
+  for (int k = b; k < e; ++k) rank += tmp[k] < i;
+  const int slot = b + rank;
+  perm[slot] = i;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    pos[(size_t)d * cap + slot] = xw[(size_t)i * DIM + d];
+    if (u) vel[(size_t)d * cap + slot] = u[(size_t)i * DIM + d];
+  }
+}
+
+// neighbour cells of `cell`: 3 per axis (offsets -1, 0, 1), or 2 when the
+// grid has only 2 cells per axis; each distinct cell appears once
+template <int DIM>
+__device__ int stencil(int cell, int nc, int* out) {
+  const int m = nc >= 3 ? 3 : 2;
+  int base[DIM];
+  for (int d = DIM - 1; d >= 0; --d) {
+    const int cd = cell % nc;
+    cell /= nc;
+    base[d] = nc >= 3 ? cd - 1 + nc : cd;  // + nc keeps the modulo non-negative
+  }
+  const int cnt = DIM == 3 ? m * m * m : m * m;
+  for (int t = 0; t < cnt; ++t) {
+    int c = 0, rest = t;
+    int o[DIM];
+    for (int d = DIM - 1; d >= 0; --d) {
+      o[d] = rest % m;
+      rest /= m;
+    }
+    for (int d = 0; d < DIM; ++d) c = c * nc + (base[d] + o[d]) % nc;
+    out[t] = c;
+  }
+  return cnt;
+}
+
+// blocks are dealt round-robin over the 8 XCDs; give each XCD a contiguous
+// run of cells so that neighbouring cells share an L2
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+  const int per = n / 8, rem = n % 8;
+  const int x = b % 8, k = b / 8;
+  return x * per + min(x, rem) + k;
+}
+
+constexpr int kBlock = 256;
+constexpr int kTile = 1024;  // candidates staged in LDS at a time
+static_assert(kTile % kBlock == 0, "whole tile rows per thread");
+
+enum { PASS_DENSITY = 0, ACC_RELAX = 1, ACC_FORCES = 2, ACC_ADVECT = 3 };
+
+struct PassOut {
+  double* rho;       // PASS_DENSITY: densities by particle id (optional)
+  double* acc;       // relax acceleration (np, dim), physical units (optional for ACC_ADVECT); acc_p for forces
+  double* acc_visc;  // ACC_FORCES: viscous and transport-velocity terms
+  double* acc_tvf;
+  // ACC_ADVECT
+  const double* rt;  // shifted positions r_t (np, dim)
+  double* r;         // in: r_0, out: shift(r_0, dt u)
+  void* u_r;         // out: displacement(r_t2, r_0) / dt in the interpolator dtype
+  int u_f32;
+  double dt, fdt2, L;
+};
+
+template <int DIM, int MODE>
+struct PassLds {
+  static constexpr int NS = DIM == 3 ? 27 : 9;
+  static constexpr bool ACC = MODE != PASS_DENSITY, VEL = MODE == ACC_FORCES;
+  static constexpr int NC = MODE == PASS_DENSITY ? 1 : (MODE == ACC_FORCES ? 3 * DIM : DIM);  // sums per home
+  int beg[NS], off[NS + 1], cnt;
+  double sh[NS][DIM];  // periodic image shift of each stencil cell
+  double x[DIM][kTile];
+  double rho[ACC ? kTile : 1], p[ACC ? kTile : 1], ir[ACC ? kTile : 1];
+  double v[VEL ? DIM : 1][VEL ? kTile : 1];
+  double red[NC][kBlock];  // per-lane partial sums
+};
+
+// 5./6. pair passes, one workgroup per home cell.  The particles of the
+// neighbour cells (the candidates, in stencil order) are staged in LDS in
+// tiles.  With nc >= 3 cells per axis each stencil cell is staged at its
+// periodic image next to the home cell (SHIFT): a pair closer than the cutoff
+// (< one cell) is then exactly its minimum image, and farther pairs stay
+// farther, so the displacement is a plain difference.  (nc == 2 lists each
+// cell once for two images and keeps the minimum-image formula.)
+// lpp lanes share a home particle, lane l taking candidates l, l+lpp, ...
+// Per tile a lane first marks its candidates inside the cutoff in a register
+// bitmask, then does the kernel arithmetic for the marked ones only.  The
+// lanes' partial sums are added in lane order, so every sum has a fixed
+// order.
+//   PASS_DENSITY  rho_i = m sum_j W(r_ij)      (utils.py:208-216, relax.py:80-87)
+//   ACC_*         accelerations                 (relax.py:90-119, :132-176)
+//   ACC_ADVECT    + velocity correction and shift (combined.py:140-142, :199)
+template <int DIM, int MODE, bool SHIFT>
+__device__ void run_pass(PassLds<DIM, MODE>& L, const double* __restrict__ pos, const double* __restrict__ vel,
+                         double* __restrict__ rho_s, const int* __restrict__ perm, const SphGeom& g,
+                         const PassOut& out, int hb, int he) {
+  constexpr int NW = 4;  // mask words: a lane holds at most kTile / 4 candidates of a tile
+  constexpr bool ACC = MODE != PASS_DENSITY;
+  constexpr bool VEL = MODE == ACC_FORCES;
+  constexpr int NC = PassLds<DIM, MODE>::NC;
+  static_assert(kTile <= 4 * 64 * NW, "a lane's candidates of one tile must fit its mask");
+  const int nh = he - hb;
+  const int lpp = max(4, min(64, kBlock / nh));  // lanes per home particle
+  const int hpc = kBlock / lpp;                   // home particles per chunk
+  const int T = L.off[L.cnt];
+  const int lane = threadIdx.x % lpp;
+  for (int h0 = hb; h0 < he; h0 += hpc) {
+    const int hl = threadIdx.x / lpp;
+    const int h = h0 + hl;
+    const bool act = hl < hpc && h < he;
+    double p[DIM], ui[DIM];
+    double rho_i = 1.0, p_i = 0.0, ir_i = 1.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      p[d] = act ? pos[(size_t)d * g.cap + h] : 0.0;
+      ui[d] = VEL && act ? vel[(size_t)d * g.cap + h] : 0.0;
+    }
+    if (ACC && act) {
+      rho_i = rho_s[h];
+      p_i = g.p_eos * (rho_i / 1.0 - 1.0) + 0.0;  // TaitEoS, gamma 1, p_background 0 (relax.py:66)
+      ir_i = 1.0 / rho_i;
+    }
+    double a0[DIM] = {}, a1[DIM] = {}, a2[DIM] = {};
+    double w = 0.0;
+    for (int t0 = 0; t0 < T; t0 += kTile) {
+      const int tn = min(kTile, T - t0);
+      __syncthreads();  // the previous tile is consumed
+      {
+        // all loads of this thread's (at most kTile / kBlock) candidates are
+        // issued before the first LDS store
+        constexpr int NL = kTile / kBlock;
+        constexpr int NV = DIM + (ACC ? 1 : 0) + (VEL ? DIM : 0);
+        double val[NL][NV];
+        int cell_t[NL];
+#pragma unroll
+        for (int u = 0; u < NL; ++u) {
+          const int k = (int)threadIdx.x + u * kBlock;
+          cell_t[u] = -1;
+          if (k >= tn) continue;
+          const int gi = t0 + k;
+          int lo = 0, hi = L.cnt;  // stencil cell t with off[t] <= gi < off[t+1]
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (L.off[mid] <= gi) lo = mid;
+            else hi = mid;
+          }
+          cell_t[u] = lo;
+          const int j = L.beg[lo] + (gi - L.off[lo]);
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) {
+            val[u][d] = pos[(size_t)d * g.cap + j];
+            if (VEL) val[u][DIM + (ACC ? 1 : 0) + d] = vel[(size_t)d * g.cap + j];
+          }
+          if (ACC) val[u][DIM] = rho_s[j];
+        }
+#pragma unroll
+        for (int u = 0; u < NL; ++u) {
+          if (cell_t[u] < 0) continue;
+          const int k = (int)threadIdx.x + u * kBlock;
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) {
+            L.x[d][k] = SHIFT ? val[u][d] + L.sh[cell_t[u]][d] : val[u][d];
+            if (VEL) L.v[d][k] = val[u][DIM + (ACC ? 1 : 0) + d];
+          }
+          if (ACC) {
+            const double rj = val[u][DIM];
+            L.rho[k] = rj;
+            L.p[k] = g.p_eos * (rj / 1.0 - 1.0) + 0.0;
+            L.ir[k] = 1.0 / rj;
+          }
+        }
+      }
+      __syncthreads();
+      if (!act) continue;
+      // this lane's candidates of the tile: k = lane + i * lpp, i < nk
+      const int nk = lane < tn ? (tn - lane + lpp - 1) / lpp : 0;
+      uint64_t mask[NW];
+#pragma unroll
+      for (int q = 0; q < NW; ++q) {
+        uint64_t bits = 0;
+        const int nb = min(64, nk - 64 * q);
+#pragma unroll 4
+        for (int b = 0; b < nb; ++b) {
+          const int k = lane + (q * 64 + b) * lpp;
+          double r2 = 0.0;
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) {
+            const double x = SHIFT ? p[d] - L.x[d][k] : min_image(p[d] - L.x[d][k], g);
+            r2 += x * x;
+          }
+          bits |= (uint64_t)(r2 < g.cutoff2) << b;  // W and dW/dr vanish from 3h on
+        }
+        mask[q] = bits;
+      }
+      // one loop over the marked candidates of every word, in candidate order
+      int q = 0;
+      uint64_t bits = mask[0];
+      while (true) {
+        while (bits == 0 && q + 1 < NW) bits = mask[++q];
+        if (bits == 0) break;
+        {
+          const int b = __builtin_ctzll(bits);
+          bits &= bits - 1;
+          const int k = lane + (q * 64 + b) * lpp;
+          double x[DIM], r2 = 0.0;
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) {
+            x[d] = SHIFT ? p[d] - L.x[d][k] : min_image(p[d] - L.x[d][k], g);
+            r2 += x[d] * x[d];
+          }
+          const double r = r2 > 0.0 ? sqrt(r2) : 0.0;
+          if (MODE == PASS_DENSITY) {
+            w += quintic_w(r, g);
+            continue;
+          }
+          const double der = quintic_dw(r, g);
+          const double rho_j = L.rho[k], p_j = L.p[k], ir_j = L.ir[k];
+          const double p_ij = (rho_j * p_i + rho_i * p_j) / (rho_i + rho_j);  // relax.py:99
+          const double pref = g.mass * (ir_i * ir_i + ir_j * ir_j);          // relax.py:102
+          const double ire = 1.0 / (r + kEps);                                // e_ij = r_ij / (d + EPS), :92
+          const double dk = der * ire;
+          if (MODE == ACC_FORCES) {
+            const double cp = -pref * p_ij * dk, ct = pref * (-g.p_eos) * dk, cv = pref * 1.0 * ire * der;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+              a0[d] += cp * x[d];                 // relax.py:149
+              a1[d] += cv * (ui[d] - L.v[d][k]);  // relax.py:151-154
+              a2[d] += ct * x[d];                 // relax.py:158
+            }
+          } else {
+            const double ca = (-pref * p_ij + 0.5 * pref * (-g.p_eos)) * dk;  // relax.py:103,107
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) a0[d] += ca * x[d];
+          }
+        }
+      }
+    }
+    // partial sums through LDS, added in lane order by each home's first lane
+    if (MODE == PASS_DENSITY) {
+      L.red[0][threadIdx.x] = w;
+    } else {
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        L.red[d][threadIdx.x] = a0[d];
+        if (MODE == ACC_FORCES) {
+          L.red[DIM + d][threadIdx.x] = a1[d];
+          L.red[2 * DIM + d][threadIdx.x] = a2[d];
+        }
+      }
+    }
+    __syncthreads();
+    double sum[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      sum[c] = 0.0;
+      if (act && lane == 0)
+        for (int l = 0; l < lpp; ++l) sum[c] += L.red[c][threadIdx.x + l];
+    }
+    __syncthreads();  // L.red is rewritten by the next chunk
+    if (!act || lane != 0) continue;
+    const int i = perm[h];
+    if (MODE == PASS_DENSITY) {
+      const double rho = g.mass * sum[0];
+      rho_s[h] = rho;
+      if (out.rho) out.rho[i] = rho;
+      continue;
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const size_t e = (size_t)i * DIM + d;
+      if (MODE == ACC_FORCES) {
+        out.acc[e] = sum[d];
+        out.acc_visc[e] = sum[DIM + d];
+        out.acc_tvf[e] = sum[2 * DIM + d];
+        continue;
+      }
+      const double a = sum[d] * g.lref;  // relax.py:119
+      if (out.acc) out.acc[e] = a;
+      if (MODE == ACC_ADVECT) {
+        const double r0 = out.r[e];
+        const double r_t = mod_side(out.rt[e] + out.fdt2 * a, out.L);  // combined.py:140
+        double dd = r_t - r0;                                            // combined.py:142
+        dd = mod_side(dd + out.L * 0.5, out.L) - 0.5 * out.L;
+        const double v = dd / out.dt;
+        if (out.u_f32) static_cast<float*>(out.u_r)[e] = (float)v;
+        else static_cast<double*>(out.u_r)[e] = v;
+        out.r[e] = mod_side(r0 + out.dt * v, out.L);  // combined.py:199
+      }
+    }
+  }
+}
+
+template <int DIM, int MODE>
+__global__ __launch_bounds__(kBlock) void k_sph_pass(const double* __restrict__ pos, const double* __restrict__ vel,
+                                                     double* __restrict__ rho_s, const int* __restrict__ start,
+                                                     const int* __restrict__ perm, int ncells, SphGeom g,
+                                                     PassOut out) {
+  __shared__ PassLds<DIM, MODE> L;
+  const int c = xcd_remap(blockIdx.x, ncells);
+  const int hb = start[c], he = start[c + 1];
+  if (hb == he) return;  // block-uniform
+  if (threadIdx.x == 0) {
+    int cells[PassLds<DIM, MODE>::NS];
+    const int n = stencil<DIM>(c, g.nc, cells);
+    // home cell coordinates, to tell which stencil cells wrap around the box
+    int hc[DIM];
+    for (int d = DIM - 1, cc = c; d >= 0; --d) {
+      hc[d] = cc % g.nc;
+      cc /= g.nc;
+    }
+    L.off[0] = 0;
+    for (int t = 0; t < n; ++t) {
+      const int b = start[cells[t]];
+      L.beg[t] = b;
+      L.off[t + 1] = L.off[t] + (start[cells[t] + 1] - b);
+      for (int d = DIM - 1, cc = cells[t]; d >= 0; --d) {
+        const int nd = cc % g.nc;
+        cc /= g.nc;
+        const int o = nd - hc[d];  // -1, 0, 1 or, across the box, +-(nc-1)
+        L.sh[t][d] = o > 1 ? -g.box : (o < -1 ? g.box : 0.0);
+      }
+    }
+    L.cnt = n;
+  }
+  __syncthreads();
+  if (g.nc >= 3) run_pass<DIM, MODE, true>(L, pos, vel, rho_s, perm, g, out, hb, he);
+  else run_pass<DIM, MODE, false>(L, pos, vel, rho_s, perm, g, out, hb, he);
+}
+
+double quintic_sigma(double h, int dim) {
+  // jax_sph QuinticKernel normalisation (Morris, Fox & Zhu 1997); the 3-D
+  // value 3/(359 pi) is the library's and integrates to 360/359
+  return dim == 2 ? 7.0 / 478.0 / M_PI / (h * h) : 3.0 / 359.0 / M_PI / (h * h * h);
+}
+
+template <class P>
+P* sph_alloc(std::vector<void*>& list, size_t n) {
+  void* p = nullptr;
+  QL_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(P)));
+  list.push_back(p);
+  return static_cast<P*>(p);
+}
+
+void free_all(std::vector<void*>& list) {
+  for (void* p : list) (void)hipFree(p);
+  list.clear();
+}
+
+// particle arrays for np particles (grown on demand; a growth waits for the
+// device before releasing the old arrays)
+void ensure_cap(qlles_sph* S, int np) {
+  if (np <= S->cap && S->perm) return;
+  if (S->perm) QL_HIP(hipDeviceSynchronize());
+  free_all(S->part_bufs);
+  S->perm = nullptr;
+  const int cap = std::max(np, S->cap);
+  const size_t d = S->dim;
+  auto& L = S->part_bufs;
+  S->perm = sph_alloc<int>(L, cap);
+  S->cell_of = sph_alloc<int>(L, cap);
+  S->tmp = sph_alloc<int>(L, cap);
+  S->xw = sph_alloc<double>(L, d * cap);
+  S->rt = sph_alloc<double>(L, d * cap);
+  S->pos = sph_alloc<double>(L, d * cap);
+  S->vel = sph_alloc<double>(L, d * cap);
+  S->rho = sph_alloc<double>(L, cap);
+  S->cap = cap;
+  S->g.cap = cap;
+}
+
+int blocks(long long n, int b) { return (int)((n + b - 1) / b); }
+
+// steps 1-5; u (shift velocity) optional, uvel (FP64 velocities to gather) optional
+template <int DIM>
+void build_and_density(qlles_sph* S, const double* r, const void* u, int u_f32, double dt, int np,
+                       const double* uvel, double* rho_out, hipStream_t s) {
+  ensure_cap(S, np);
+  const SphGeom& g = S->g;
+  // the counts start at zero: cleared at creation and by every scan
+  hipLaunchKernelGGL((k_sph_bin<DIM>), dim3(blocks(np, 256)), dim3(256), 0, s, r, u, u_f32, dt, S->L, np, g, S->rt,
+                     S->xw, S->cell_of, S->cnt);
+  hipLaunchKernelGGL(k_sph_scan, dim3(1), dim3(1024), 0, s, S->cnt, S->start, S->fill, S->ncells);
+  hipLaunchKernelGGL(k_sph_scatter, dim3(blocks(np, 256)), dim3(256), 0, s, S->cell_of, S->start, S->fill, S->tmp,
+                     np);
+  hipLaunchKernelGGL((k_sph_rank<DIM>), dim3(blocks(np, 256)), dim3(256), 0, s, S->cell_of, S->start, S->tmp, S->xw,
+                     uvel, np, S->cap, S->perm, S->pos, S->vel);
+  PassOut o{};
+  o.rho = rho_out;
+  hipLaunchKernelGGL((k_sph_pass<DIM, PASS_DENSITY>), dim3(S->ncells), dim3(kBlock), 0, s, S->pos,
+                     S->vel, S->rho, S->start, S->perm, S->ncells, g, o);
+  QL_LAUNCHED();
+}
+
+template <int DIM, int MODE>
+void accel(qlles_sph* S, const PassOut& o, hipStream_t s) {
+  hipLaunchKernelGGL((k_sph_pass<DIM, MODE>), dim3(S->ncells), dim3(kBlock), 0, s, S->pos, S->vel,
+                     S->rho, S->start, S->perm, S->ncells, S->g, o);
+  QL_LAUNCHED();
+}
+
+template <class F>
+void by_dim(const qlles_sph* S, F&& f) {
+  if (S->dim == 3) f(std::integral_constant<int, 3>{});
+  else f(std::integral_constant<int, 2>{});
+}
+
+}  // namespace
+
+extern "C" {
+
+int qlles_sph_create(int dim, int n, double box, int is_physical, double u_ref, qlles_sph** out) {
+  if (!out) return fail(QLLES_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (dim != 2 && dim != 3) return fail(QLLES_ERR_SHAPE, "dim must be 2 or 3");
+  // cutoff 3 L/n below L/2; workspace ~170 B per particle
+  if (n < 7 || n > (dim == 3 ? 256 : 4096)) return fail(QLLES_ERR_SHAPE, "n must be in [7, 256] (3-D) / [7, 4096] (2-D)");
+  if (!(box > 0.0) || !std::isfinite(box)) return fail(QLLES_ERR_ARG, "box side must be positive");
+  if (!std::isfinite(u_ref)) return fail(QLLES_ERR_ARG, "u_ref must be finite");
+  qlles_sph* S = new (std::nothrow) qlles_sph();
+  if (!S) return fail(QLLES_ERR_HIP, "host allocation failed");
+  const int rc = guarded([&] {
+    S->dim = dim;
+    S->n = n;
+    S->L = box;
+    // relax.py:30-41, 61-64 (utils.py:181-186 with l_ref = 1)
+    const double dx_phys = box / n;
+    const double lref = is_physical ? 1.0 : dx_phys;
+    const double dx = dx_phys / lref;
+    SphGeom& g = S->g;
+    g.lref = lref;
+    g.box = box / lref;
+    g.half = g.box * 0.5;
+    g.inv_h = 1.0 / dx;
+    g.cutoff = 3.0 * dx;
+    g.cutoff2 = g.cutoff * g.cutoff;
+    g.sigma = quintic_sigma(dx, dim);
+    g.mass = std::pow(dx, dim) * 1.0;
+    const double u_eos = u_ref / lref, c_eos = 10.0 * u_eos;
+    g.p_eos = c_eos * c_eos * 1.0;
+    // cells at least one cutoff wide: box / cutoff = n / 3, one fewer when exact
+    g.nc = (n - 1) / 3;
+    g.inv_cell = g.nc / g.box;
+    S->ncells = dim == 3 ? g.nc * g.nc * g.nc : g.nc * g.nc;
+    S->cnt = sph_alloc<int>(S->cell_bufs, S->ncells);
+    S->start = sph_alloc<int>(S->cell_bufs, S->ncells + 1);
+    S->fill = sph_alloc<int>(S->cell_bufs, S->ncells);
+    QL_HIP(hipMemset(S->cnt, 0, sizeof(int) * S->ncells));  // later calls leave them cleared (k_sph_scan)
+    QL_HIP(hipDeviceSynchronize());
+    ensure_cap(S, dim == 3 ? n * n * n : n * n);
+  });
+  if (rc != QLLES_OK) {
+    free_all(S->cell_bufs);
+    free_all(S->part_bufs);
+    delete S;
+    return rc;
+  }
+  *out = S;
+  return QLLES_OK;
+}
+
+int qlles_sph_destroy(qlles_sph* S) {
+  if (!S) return QLLES_OK;
+  (void)hipDeviceSynchronize();
+  free_all(S->cell_bufs);
+  free_all(S->part_bufs);
+  delete S;
+  return QLLES_OK;
+}
+
+int qlles_sph_density(qlles_sph* S, const double* r, int np, double* rho, void* stream) {
+  if (!S) return fail(QLLES_ERR_ARG, "null argument");
+  if (np < 0) return fail(QLLES_ERR_ARG, "np must be >= 0");
+  if (np == 0) return QLLES_OK;  // empty arrays may have NULL data
+  if (!r || !rho) return fail(QLLES_ERR_ARG, "null argument");
+  return guarded([&] {
+    by_dim(S, [&](auto D) {
+      build_and_density<decltype(D)::value>(S, r, nullptr, 0, 0.0, np, nullptr, rho, (hipStream_t)stream);
+    });
+  });
+}
+
+int qlles_sph_relax_acc(qlles_sph* S, const double* r, int np, double* acc, double* rho, void* stream) {
+  if (!S) return fail(QLLES_ERR_ARG, "null argument");
+  if (np < 0) return fail(QLLES_ERR_ARG, "np must be >= 0");
+  if (np == 0) return QLLES_OK;
+  if (!r || !acc) return fail(QLLES_ERR_ARG, "null argument");
+  return guarded([&] {
+    by_dim(S, [&](auto D) {
+      constexpr int DIM = decltype(D)::value;
+      build_and_density<DIM>(S, r, nullptr, 0, 0.0, np, nullptr, rho, (hipStream_t)stream);
+      PassOut o{};
+      o.acc = acc;
+      accel<DIM, ACC_RELAX>(S, o, (hipStream_t)stream);
+    });
+  });
+}
+
+int qlles_sph_forces(qlles_sph* S, const double* r, const double* u, int np, double* acc_p, double* acc_visc,
+                     double* acc_tvf, void* stream) {
+  if (!S) return fail(QLLES_ERR_ARG, "null argument");
+  if (np < 0) return fail(QLLES_ERR_ARG, "np must be >= 0");
+  if (S->g.lref != 1.0) return fail(QLLES_ERR_ARG, "sph forces need physical units (is_physical)");
+  if (np == 0) return QLLES_OK;
+  if (!r || !u || !acc_p || !acc_visc || !acc_tvf) return fail(QLLES_ERR_ARG, "null argument");
+  return guarded([&] {
+    by_dim(S, [&](auto D) {
+      constexpr int DIM = decltype(D)::value;
+      build_and_density<DIM>(S, r, nullptr, 0, 0.0, np, u, nullptr, (hipStream_t)stream);
+      PassOut o{};
+      o.acc = acc_p;
+      o.acc_visc = acc_visc;
+      o.acc_tvf = acc_tvf;
+      accel<DIM, ACC_FORCES>(S, o, (hipStream_t)stream);
+    });
+  });
+}
+
+int qlles_sph_relax_advect(qlles_sph* S, double* r, int np, void* u_r, int u_dtype, double dt, double factor,
+                           double* acc, void* stream) {
+  if (!S) return fail(QLLES_ERR_ARG, "null argument");
+  if (np < 0) return fail(QLLES_ERR_ARG, "np must be >= 0");
+  if (np > 0 && (!r || !u_r)) return fail(QLLES_ERR_ARG, "null argument");
+  if (u_dtype != QLLES_F32 && u_dtype != QLLES_F64) return fail(QLLES_ERR_ARG, "u_dtype must be QLLES_F32/F64");
+  if (!(dt != 0.0) || !std::isfinite(dt) || !std::isfinite(factor))
+    return fail(QLLES_ERR_ARG, "dt must be finite and non-zero, factor finite");
+  if (S->g.lref != 1.0) return fail(QLLES_ERR_ARG, "the relaxation step needs physical units (is_physical)");
+  if (np == 0) return QLLES_OK;
+  return guarded([&] {
+    by_dim(S, [&](auto D) {
+      constexpr int DIM = decltype(D)::value;
+      const int f32 = u_dtype == QLLES_F32;
+      build_and_density<DIM>(S, r, u_r, f32, dt, np, nullptr, nullptr, (hipStream_t)stream);
+      PassOut o{};
+      o.acc = acc;
+      o.rt = S->rt;
+      o.r = r;
+      o.u_r = u_r;
+      o.u_f32 = f32;
+      o.dt = dt;
+      const double fd = factor * dt;
+      o.fdt2 = fd * fd;
+      o.L = S->L;
+      accel<DIM, ACC_ADVECT>(S, o, (hipStream_t)stream);
+    });
+  });
+}
+
+}  // extern "C"

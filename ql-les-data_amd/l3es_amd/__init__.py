@@ -2,7 +2,8 @@
 
 Host-side mirror of the reference package ``l3es`` for the hot path only:
 spectral_solver (RHS, RK4, forcing, setup, driver), init_fields, integrator
-(particle interpolation) and the spectral utilities.  All arithmetic runs in
+(particle interpolation), relax (SPH relaxation), combined (DNS + particles)
+and the spectral utilities (filter, spectrum, dissipation, SPH density).  All arithmetic runs in
 ``ql-les-data_amd/lib/libqlles.so`` through the C ABI in ``include/qlles.h``.
 """
 
@@ -11,3 +12,5 @@ from .spectral_solver import (comp_dt, custom_jit_integrate, rhs_wrapper, rk4_wr
                               set_up_solver, simulate)
 from .integrator import shift_fn, spectral_interpolator_wrapper  # noqa: F401
 from .batch import Batch  # noqa: F401
+from .relax import relax_advect, relax_wrapper  # noqa: F401
+from .utils import rho_computer  # noqa: F401

@@ -34,7 +34,8 @@ EXPORTS = (
     "qlles_batch_configure", "qlles_batch_tune", "qlles_batch_get_config", "qlles_batch_run",
     "qlles_rhs_jvp", "qlles_rhs_vjp", "qlles_step_jvp", "qlles_step_vjp", "qlles_irfftn_adjoint",
     "qlles_rfftn_adjoint", "qlles_interp_hat_vjp", "qlles_plan_set_variant", "qlles_plan_get_variant",
-    "qlles_plan_tune", "qlles_energy_spectrum",
+    "qlles_plan_tune", "qlles_energy_spectrum", "qlles_sph_create", "qlles_sph_destroy", "qlles_sph_density",
+    "qlles_sph_relax_acc", "qlles_sph_forces", "qlles_sph_relax_advect",
 )
 KINDS = ("x-pass", "y-inverse", "z-pass", "y-forward", "other")
 
@@ -108,6 +109,12 @@ def load():
         "qlles_plan_get_variant": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci)]),
         "qlles_plan_tune": (ci, [vp, vp, ci, pd, vp]),
         "qlles_energy_spectrum": (ci, [vp, vp, cd, vp, vp]),
+        "qlles_sph_create": (ci, [ci, ci, cd, ci, cd, ctypes.POINTER(vp)]),
+        "qlles_sph_destroy": (ci, [vp]),
+        "qlles_sph_density": (ci, [vp, vp, ci, vp, vp]),
+        "qlles_sph_relax_acc": (ci, [vp, vp, ci, vp, vp, vp]),
+        "qlles_sph_forces": (ci, [vp, vp, vp, ci, vp, vp, vp, vp]),
+        "qlles_sph_relax_advect": (ci, [vp, vp, ci, vp, ci, cd, cd, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -177,6 +184,28 @@ class Plan:
     def close(self):
         if getattr(self, "handle", None) is not None and _lib is not None:
             _lib.qlles_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Sph:
+    """Owner of a qlles_sph (SPH density / relaxation context on an n^dim particle lattice)."""
+
+    def __init__(self, dim, n, box, is_physical, u_ref):
+        lib = load()
+        h = ctypes.c_void_p()
+        check(lib.qlles_sph_create(dim, n, float(box), int(bool(is_physical)), float(u_ref), ctypes.byref(h)))
+        self.handle = h
+        self.dim, self.n = dim, n
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and _lib is not None:
+            _lib.qlles_sph_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -2,8 +2,11 @@
 
 Per step (combined.py:97,127,130,199): advance the spectral field, evaluate
 the ckp_N-filtered velocity at the particles, then r <- (r + dt u_r) mod 2 pi.
-The filter, interpolation and shift are one fused GPU call.  SPH relaxation,
-density and plotting are not part of this library.
+Without relaxation the filter, interpolation and shift are one fused GPU
+call.  With relax_dt_factor > 0 the velocity is interpolated at r, then one
+fused call relaxes it and shifts r (combined.py:132-142, :199).  The density
+column rho_max comes from comp_rho(r) (combined.py:145-149).  Plotting is not
+part of this library; particle frames are written as .npz (no h5py).
 """
 
 import csv
@@ -14,6 +17,7 @@ import numpy as np
 import torch
 
 from .integrator import set_up_integrator
+from .relax import relax_advect
 from .spectral_solver import diagnostics, rk4_wrapper, rhs_wrapper, set_up_solver
 
 
@@ -23,16 +27,16 @@ def combined(case, N=256, dim=3, nu=0.000625, t_final=0.1, t_burnin=0, dt=0.01, 
              dst_path=None, precision="float64", viscous_mode="explicit", spectrum="k53", power=0.1, drag=0.0,
              kolm_amp=1.0, kolm_k=4, tol=1e-6, jitter=0.0, verbose=True, write_frames=True):
     del vis_freq
-    if relax_dt_factor > 0.0:
-        raise NotImplementedError("SPH relaxation (relax.py) is outside this library")
     int_path = os.path.join(dst_path, "com") if dst_path else None
     if int_path:
         os.makedirs(int_path, exist_ok=True)
     u, u_hat, _, dx, integrate_fn, L, fft_axes, e_kin_init, e_inj = set_up_solver(
         N, nu, dim, case, dt, seed, ckp_N, kf, e_kin_target, forcing_type, rejit, precision, viscous_mode,
         spectrum, power, drag, kolm_amp, kolm_k)
-    r, _, interpolator, _, _, _, _ = set_up_integrator(state_0_path, dim, ckp_N, dft_splits, u_ref,
-                                                        interp_backend, precision, tol, jitter, seed)
+    r, comp_rho, interpolator, relax_fn, _, _, _ = set_up_integrator(state_0_path, dim, ckp_N, dft_splits, u_ref,
+                                                                      interp_backend, precision, tol, jitter, seed)
+    if relax_dt_factor > 0.0 and relax_fn is None:
+        raise ValueError("relax_dt_factor > 0 needs u_ref and ckp_N >= 7")
     diag_path = os.path.join(dst_path, "diagnostics.csv") if dst_path else None
     if diag_path:
         with open(diag_path, "w", newline="") as fh:
@@ -56,20 +60,32 @@ def combined(case, N=256, dim=3, nu=0.000625, t_final=0.1, t_burnin=0, dt=0.01, 
         t_temp = time()
         is_ckp = (i % ckp_freq == 0) and (i >= burnin_steps) and int_path is not None and write_frames
         r0 = r.clone() if is_ckp else None
-        interpolator.from_hat(u_hat, r, dt, u_r)  # combined.py:127,130,199
+        need_rho = comp_rho is not None and (i % log_freq == 0 or is_ckp)
+        rho = None
+        if relax_dt_factor > 0.0:
+            interpolator.from_hat(u_hat, r, 0.0, u_r)  # combined.py:127,130
+            u_r0 = u_r.clone() if is_ckp else None  # combined.py:131
+            rho = comp_rho(r) if need_rho else None  # combined.py:147
+            relax_advect(relax_fn, r, u_r, dt, relax_dt_factor)  # combined.py:132-142, :199
+        else:
+            rho = comp_rho(r) if need_rho else None
+            interpolator.from_hat(u_hat, r, dt, u_r)  # combined.py:127,130,199
+            u_r0 = u_r
         torch.cuda.synchronize()
         t_int += time() - t_temp
+        rho_max = float(rho.max()) if rho is not None else None
         if i % log_freq == 0:
             d = diagnostics(u, integrate_fn.plan).cpu().numpy()
             dt_est = float(1.0 / (d[2:5] / dx + nu / dx ** 2).min())
-            rows.append([i, float(i * dt), float(d[1]), float(d[0]), dt_est, float(e_buf[0]), None])
+            rows.append([i, float(i * dt), float(d[1]), float(d[0]), dt_est, float(e_buf[0]), rho_max])
             if diag_path:
                 with open(diag_path, "a", newline="") as fh:
                     csv.writer(fh).writerow(rows[-1])
             if verbose:
-                print(f"Step {i}/{tstep_max}, u_max = {d[1]:.3f}, E_kin = {d[0]:.3f}, dt_est = {dt_est:.5f}")
+                rho_txt = f", rho_max = {rho_max:.3f}" if rho_max is not None else ""
+                print(f"Step {i}/{tstep_max}, u_max = {d[1]:.3f}, E_kin = {d[0]:.3f}, dt_est = {dt_est:.5f}{rho_txt}")
         if is_ckp:
-            np.savez(os.path.join(int_path, f"step_{i:05d}.npz"), r=r0.cpu().numpy(), u=u_r.cpu().numpy())
+            np.savez(os.path.join(int_path, f"step_{i:05d}.npz"), r=r0.cpu().numpy(), u=u_r0.cpu().numpy())
     if verbose:
         print(f"t_tot = {time() - t0:.3f}, t_sim = {t_sim}, t_int = {t_int:.3f}")
     return dict(u=u, u_hat=u_hat, r=r, u_r=u_r, rows=rows, t_sim=t_sim, t_int=t_int, e_inj_acc=e_acc)

@@ -6,6 +6,8 @@ import numpy as np
 import torch
 
 from . import _lib
+from .relax import relax_wrapper
+from .utils import rho_computer
 
 
 def shift_fn(r, dr, box_size=2 * np.pi):
@@ -71,13 +73,14 @@ def pos_init_jittered(n, dim, L=2 * np.pi, jitter=0.25, seed=0):
 
 def set_up_integrator(state_0_path, dim, N, dft_splits, u_ref, interp_backend, precision="float64", tol=1e-6,
                       jitter=0.0, seed=0):
-    """Particles + interpolator (integrator.py:96-114).
+    """Particles, density, interpolator and relaxation closures (integrator.py:96-114).
 
     Relaxed SPH states (.h5 via jax_sph) are not read here; a ``.npy`` file of
-    (np, dim) positions may be given instead.  SPH relaxation and density are
-    not part of this library, so comp_rho / relax_fn / sph_fn are None.
+    (np, dim) positions may be given instead.  comp_rho = rho_computer(N, dim,
+    L) and (relax_fn, sph_fn) = relax_wrapper(N, dim, L, is_physical=True,
+    u_ref) as in the reference; they are None when N < 7 (the kernel support
+    3 L/N must stay below L/2) or, for the relaxation, when u_ref is None.
     """
-    del u_ref
     L = 2 * np.pi
     fft_axes = (1, 2, 3) if dim == 3 else (1, 2)
     if state_0_path is not None and os.path.exists(state_0_path) and state_0_path.endswith(".npy"):
@@ -87,6 +90,10 @@ def set_up_integrator(state_0_path, dim, N, dft_splits, u_ref, interp_backend, p
     else:
         r = pos_init_cartesian(N, dim, L)
     r = torch.from_numpy(np.ascontiguousarray(r, dtype=np.float64)).to(torch.device("cuda", torch.cuda.current_device()))
+    comp_rho = rho_computer(N, dim=dim, L=L) if N >= 7 else None
     interpolator = spectral_interpolator_wrapper(N, fft_axes, splits=dft_splits or 1, backend=interp_backend,
                                                  precision=precision, tol=tol)
-    return r, None, interpolator, None, None, L, fft_axes
+    relax_fn = sph_fn = None
+    if N >= 7 and u_ref is not None:
+        relax_fn, sph_fn = relax_wrapper(N, dim, L, is_physical=True, u_ref=u_ref)
+    return r, comp_rho, interpolator, relax_fn, sph_fn, L, fft_axes

@@ -83,3 +83,23 @@ def energy_spectrum(vel, mul_fac=1.0, is_scalar_field=False, dim=3):
     _lib.check(_lib.load().qlles_energy_spectrum(plan.handle, _lib.ptr(full.contiguous()), float(mul_fac),
                                                  _lib.ptr(ek), _lib.stream_ptr()))
     return ek.cpu().numpy()
+
+
+def rho_computer(N, dim=3, L=2 * np.pi):
+    """Density from coordinates (utils.py:169-218): comp_rho(r) -> rho (np,) FP64
+    on the device, rho_i = m sum_j W(|r_i - r_j|) with the quintic kernel of
+    smoothing length L/N over every particle within 3 L/N, itself included."""
+    ctx = _lib.Sph(dim, N, L, True, 0.0)
+    lib = _lib.load()
+    dev = torch.device("cuda", torch.cuda.current_device())
+
+    def comp_rho(r):
+        r = torch.as_tensor(r).to(device=dev, dtype=torch.float64).contiguous()
+        if r.dim() != 2 or r.shape[1] != dim:
+            raise ValueError(f"expected an (np, {dim}) array, got {tuple(r.shape)}")
+        rho = torch.empty(r.shape[0], dtype=torch.float64, device=dev)
+        _lib.check(lib.qlles_sph_density(ctx.handle, _lib.ptr(r), r.shape[0], _lib.ptr(rho), _lib.stream_ptr()))
+        return rho
+
+    comp_rho.ctx = ctx
+    return comp_rho

@@ -69,3 +69,20 @@ def test_forcing_validation_without_device():
     assert lib.qlles_interp_create(3, 33, 0, 0, 1e-6, ctypes.byref(h)) == _lib.QLLES_ERR_SHAPE
     assert lib.qlles_interp_create(3, 32, 0, 0, 0.5, ctypes.byref(h)) == _lib.QLLES_ERR_ARG
     assert lib.qlles_step(None, None, None, None, 1, None, None) == _lib.QLLES_ERR_ARG
+
+
+@pytest.mark.parametrize("dim,n,box,u_ref,code", [
+    (4, 32, 6.28, 1.0, 2), (3, 6, 6.28, 1.0, 2), (3, 257, 6.28, 1.0, 2), (2, 4097, 6.28, 1.0, 2),
+    (3, 32, 0.0, 1.0, 1), (3, 32, -1.0, 1.0, 1), (2, 32, 6.28, float("nan"), 1),
+])
+def test_sph_create_rejects_bad_arguments(dim, n, box, u_ref, code):
+    from l3es_amd import _lib
+
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libqlles.so not built")
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    rc = lib.qlles_sph_create(dim, n, box, 1, u_ref, ctypes.byref(h))
+    assert rc == code
+    assert not h.value
+    assert lib.qlles_last_error().decode()
