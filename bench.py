@@ -440,6 +440,35 @@ def run_relax(args):
         log(f"[bench] relax {dim}-D {n}^{dim}: {1e6 * t_relax:.1f} us/call (density {1e6 * gpu_rho:.1f} us), "
             f"{pairs} pairs, accel ~{res['accel_fp64_tflops']:.2f} TFLOP/s, cpu {cpu}")
         results.append(res)
+    # particle -> grid transfers of the point-cloud spectrum (turbulence.py:253-377) at 32^3
+    from l3es_amd import ur_to_u_dft_wrapper, ur_to_u_mls_wrapper
+    from oracle import pointcloud as opc
+
+    n = 32
+    rng = np.random.default_rng(args.seed + 1)
+    rp = np.mod((np.stack(np.meshgrid(*([np.arange(n)] * 3), indexing="ij"), -1).reshape(-1, 3) + 0.5) * (L / n)
+                + (rng.random((n ** 3, 3)) * 2 - 1) * 0.25 * L / n, L)
+    up = rng.normal(size=rp.shape)
+    rpd, upd = torch.from_numpy(rp).cuda(), torch.from_numpy(up).cuda()
+    pcloud = {"particles": n ** 3}
+    for name, body in (("mls", ur_to_u_mls_wrapper(n, L, 3)), ("dft", ur_to_u_dft_wrapper(n, L, 3))):
+        body(rpd, upd)
+        torch.cuda.synchronize()
+        reps = 10
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            body(rpd, upd)
+        torch.cuda.synchronize()
+        pcloud[f"{name}_gpu_ms"] = 1e3 * (time.perf_counter() - t0) / reps
+    if not args.no_cpu:
+        # bounded sample: every 8th node of the 32^3 grid, time scaled by 8
+        nodes = opc.lattice_ij(n, 3, L)[::8]
+        c0 = time.perf_counter()
+        opc.mls_2nd_order(rp, nodes, up, L, L / n, 3)
+        pcloud["mls_cpu_oracle_ms"] = 8e3 * (time.perf_counter() - c0)
+        pcloud["mls_cpu_sample"] = "oracle mls_2nd_order (NumPy) at every 8th node, time x 8, 1 core"
+        pcloud["mls_speedup_vs_cpu"] = pcloud["mls_cpu_oracle_ms"] / pcloud["mls_gpu_ms"]
+    log(f"[bench] point cloud 32^3: {pcloud}")
     main_case = results[0]
     peak = 78.6
     out = {
@@ -456,6 +485,7 @@ def run_relax(args):
             "value": 1.0 / main_case["cpu_oracle_s_per_call"], "unit": "calls/s", "cores": 1, "kind": "port",
             "sample": "oracle relax_advect (NumPy, scipy cKDTree neighbour search) on the same 32^3 state"},
         "cases": results,
+        "point_cloud": pcloud,
     }
     print(json.dumps(out), flush=True)
 

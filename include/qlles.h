@@ -285,6 +285,21 @@ int qlles_sph_forces(qlles_sph* sph, const double* r, const double* u, int np, d
 int qlles_sph_relax_advect(qlles_sph* sph, double* r, int np, void* u_r, int u_dtype, double dt, double factor,
                            double* acc, void* stream);
 
+/* Particle -> grid transfers of the point-cloud spectrum (turbulence.py),
+ * on the context's n-grid of side box; u_r (np, dim) FP64.
+ * ur_to_u_mls_wrapper(n, box, dim)(r, u_r) (turbulence.py:145-295):
+ * second-order moving least squares with the quintic kernel, h = 2/3 box/n,
+ * at the nodes (i + 1/2) box/n; u_grid is (dim, n, n[, n]) FP64, 'ij' order.
+ * A node with too few particles in reach has a singular system (NaN). */
+int qlles_sph_mls_grid(qlles_sph* sph, const double* r, const double* u_r, int np, double* u_grid, void* stream);
+/* ur_to_u_dft_wrapper (turbulence.py:298-336) up to its inverse transform:
+ * u_hat receives the half spectrum of ifftn(U).real, U_c(k) = sum_j u_jc
+ * exp(-i k.r_j) over the fftfreq modes, in the solver layout
+ * (3, n, n, n/2+1) / (3, n, n/2+1) complex FP64 (component 2 of a 2-D field
+ * zero); qlles_irfftn of an FP64 plan on the n-grid then gives the field on
+ * the nodes m box/n. */
+int qlles_sph_dft_hat(qlles_sph* sph, const double* r, const double* u_r, int np, void* u_hat, void* stream);
+
 /* message of the last failed call on this thread */
 const char* qlles_last_error(void);
 

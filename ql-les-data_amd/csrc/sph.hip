@@ -67,6 +67,11 @@ struct qlles_sph {
   double* vel = nullptr;   // [dim][cap] cell-ordered velocities (forces entry)
   double* rho = nullptr;   // [cap] cell-ordered densities
   std::vector<void*> cell_bufs, part_bufs;
+  // particle -> grid DFT (allocated on first use): phase tables [dim][np][n]
+  // and the full spectrum [dim][n^dim], complex FP64
+  void* dft_E = nullptr;
+  void* dft_U = nullptr;
+  size_t dft_E_elems = 0;
 };
 
 namespace {
@@ -516,6 +521,222 @@ __global__ __launch_bounds__(kBlock) void k_sph_pass(const double* __restrict__ 
   else run_pass<DIM, MODE, false>(L, pos, vel, rho_s, perm, g, out, hb, he);
 }
 
+// ---- particle -> grid transfers (turbulence.py:145-336) ---------------------
+
+// Second-order moving least squares at the grid nodes (i + 1/2) box/n, 'ij'
+// order (mls_2nd_order with the quintic kernel, h = 2/3 dx, support 2 dx;
+// turbulence.py:145-250, :253-295).  One thread per node over the particles
+// of its neighbour cells.  The reference solves D A x = D b with
+// A = sum_j w r r^T, b = sum_j w f r and a diagonal D; this solves A x = b by
+// Cholesky, with the monomials r = [1, s_a, s_a^2/2, s_a s_b] in units of h
+// (s = r_ji / h), which leaves x[0], the value at the node, unchanged and
+// keeps A well scaled.
+template <int DIM>
+__global__ __launch_bounds__(64) void k_sph_mls(const double* __restrict__ pos, const double* __restrict__ vel,
+                                                const int* __restrict__ start, SphGeom g, int n, double h,
+                                                double sigma, double* __restrict__ out) {
+  constexpr int M = DIM == 3 ? 10 : 6;
+  constexpr int NP = M * (M + 1) / 2;
+  const int nodes = DIM == 3 ? n * n * n : n * n;
+  const int node = blockIdx.x * 64 + threadIdx.x;
+  if (node >= nodes) return;
+  const double dxw = g.box / n;
+  double x[DIM];
+  int cell = 0;
+  for (int d = DIM - 1, rest = node; d >= 0; --d) {
+    x[d] = ((rest % n) + 0.5) * dxw;
+    rest /= n;
+  }
+  for (int d = 0; d < DIM; ++d) cell = cell * g.nc + cell_coord(x[d], g);
+  int cells[DIM == 3 ? 27 : 9];
+  const int ns = stencil<DIM>(cell, g.nc, cells);
+  const double cut = 3.0 * h, cut2 = cut * cut, ih = 1.0 / h;
+  double A[NP], B[DIM][M];
+#pragma unroll
+  for (int e = 0; e < NP; ++e) A[e] = 0.0;
+#pragma unroll
+  for (int c = 0; c < DIM; ++c)
+#pragma unroll
+    for (int e = 0; e < M; ++e) B[c][e] = 0.0;
+  for (int t = 0; t < ns; ++t) {
+    const int ke = start[cells[t] + 1];
+    for (int k = start[cells[t]]; k < ke; ++k) {
+      double d[DIM], r2 = 0.0;
+#pragma unroll
+      for (int a = 0; a < DIM; ++a) {
+        d[a] = min_image(pos[(size_t)a * g.cap + k] - x[a], g);  // r_ji = r_j - r_i (turbulence.py:196)
+        r2 += d[a] * d[a];
+      }
+      if (!(r2 < cut2)) continue;
+      const double q = sqrt(r2) * ih;
+      const double q1 = fmax(0.0, 1.0 - q), q2 = fmax(0.0, 2.0 - q), q3 = fmax(0.0, 3.0 - q);
+      const double w = sigma * (ipow5(q3) - 6.0 * ipow5(q2) + 15.0 * ipow5(q1));
+      double row[M];
+      row[0] = 1.0;
+#pragma unroll
+      for (int a = 0; a < DIM; ++a) {
+        const double sa = d[a] * ih;
+        row[1 + a] = sa;
+        row[1 + DIM + a] = 0.5 * sa * sa;
+      }
+      {
+        int e = 1 + 2 * DIM;
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+#pragma unroll
+          for (int b = a + 1; b < DIM; ++b) row[e++] = d[a] * ih * (d[b] * ih);
+      }
+      {
+        int e = 0;
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+          const double wi = w * row[i];
+#pragma unroll
+          for (int j = i; j < M; ++j) A[e++] += wi * row[j];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        const double wf = w * vel[(size_t)c * g.cap + k];
+#pragma unroll
+        for (int i = 0; i < M; ++i) B[c][i] += wf * row[i];
+      }
+    }
+  }
+  // A = U^T U (packed upper rows), then U^T y = b, U x = y per component
+  auto at = [](int i, int j) { return i * M - i * (i - 1) / 2 + (j - i); };
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    double sd = A[at(i, i)];
+#pragma unroll
+    for (int p = 0; p < i; ++p) sd -= A[at(p, i)] * A[at(p, i)];
+    const double uii = sqrt(sd), inv = 1.0 / uii;
+    A[at(i, i)] = uii;
+#pragma unroll
+    for (int j = i + 1; j < M; ++j) {
+      double v = A[at(i, j)];
+#pragma unroll
+      for (int p = 0; p < i; ++p) v -= A[at(p, i)] * A[at(p, j)];
+      A[at(i, j)] = v * inv;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) {
+    double y[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+      double v = B[c][i];
+#pragma unroll
+      for (int p = 0; p < i; ++p) v -= A[at(p, i)] * y[p];
+      y[i] = v / A[at(i, i)];
+    }
+#pragma unroll
+    for (int i = M - 1; i >= 0; --i) {
+      double v = y[i];
+#pragma unroll
+      for (int j = i + 1; j < M; ++j) v -= A[at(i, j)] * y[j];
+      y[i] = v / A[at(i, i)];
+    }
+    out[(size_t)c * nodes + node] = y[0];
+  }
+}
+
+// Exact type-1 sum of ur_to_u_dft_wrapper (turbulence.py:298-336),
+//   U_c(k) = sum_j u_jc exp(-i k.r_j),  k over fftfreq(n) * n per axis,
+// from per-axis phase tables E[a][j][k] = exp(-i f(k) r_ja) (FP64).
+__global__ void k_dft_phase(const double* __restrict__ r, int np, int dim, int n, double2* __restrict__ E) {
+  const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (long long)dim * np * n) return;
+  const int k = (int)(id % n);
+  const long long aj = id / n;
+  const int j = (int)(aj % np), a = (int)(aj / np);
+  const int f = k < (n + 1) / 2 ? k : k - n;
+  double sn, cs;
+  sincos((double)f * r[(size_t)j * dim + a], &sn, &cs);
+  E[id] = make_double2(cs, -sn);
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// one block per row of the leading axes (kx, or kx, ky); lanes over the last
+// axis and over interleaved particle subsets, subsets added in order
+template <int DIM>
+__global__ __launch_bounds__(256) void k_dft_sum(const double2* __restrict__ E, const double* __restrict__ u, int np,
+                                                 int n, double2* __restrict__ U) {
+  __shared__ double2 red[DIM][256];
+  const int W = n <= 32 ? 32 : 64, S = 256 / W;
+  const int row = blockIdx.x;
+  int kl[DIM - 1];
+  for (int d = DIM - 2, rest = row; d >= 0; --d) {
+    kl[d] = rest % n;
+    rest /= n;
+  }
+  const size_t plane = (size_t)np * n;
+  for (int k0 = 0; k0 < n; k0 += W) {
+    const int kz = k0 + (int)threadIdx.x % W, sub = (int)threadIdx.x / W;
+    const bool act = kz < n;
+    double2 acc[DIM];
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) acc[c] = make_double2(0.0, 0.0);
+    if (act)
+      for (int j = sub; j < np; j += S) {
+        double2 e = E[(size_t)(DIM - 1) * plane + (size_t)j * n + kz];
+#pragma unroll
+        for (int d = 0; d < DIM - 1; ++d) e = cmul(E[(size_t)d * plane + (size_t)j * n + kl[d]], e);
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+          const double uc = u[(size_t)j * DIM + c];
+          acc[c].x += uc * e.x;
+          acc[c].y += uc * e.y;
+        }
+      }
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) red[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    if (act && sub == 0)
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        double2 sum = red[c][threadIdx.x];
+        for (int q = 1; q < S; ++q) {
+          sum.x += red[c][threadIdx.x + q * W].x;
+          sum.y += red[c][threadIdx.x + q * W].y;
+        }
+        U[((size_t)c * gridDim.x + row) * n + kz] = sum;
+      }
+    __syncthreads();
+  }
+}
+
+// half spectrum of ifftn(U).real: H(k) = (U(k) + conj(U(-k mod n))) / 2 in
+// the solver layout (3, n, n, n/2+1) / (3, n, n/2+1), component 2 of a 2-D
+// field zero
+template <int DIM>
+__global__ void k_dft_half(const double2* __restrict__ U, int n, double2* __restrict__ H) {
+  const int nh = n / 2 + 1;
+  const long long rows = DIM == 3 ? (long long)n * n : n;
+  const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= 3 * rows * nh) return;
+  const int kz = (int)(id % nh);
+  const long long cr = id / nh;
+  const int row = (int)(cr % rows), c = (int)(cr / rows);
+  if (c >= DIM) {
+    H[id] = make_double2(0.0, 0.0);
+    return;
+  }
+  int neg = 0;
+  for (int d = DIM - 2, rest = row, mul = 1; d >= 0; --d) {
+    const int kd = rest % n;
+    rest /= n;
+    neg += ((n - kd) % n) * mul;
+    mul *= n;
+  }
+  const double2 a = U[((size_t)c * rows + row) * n + kz];
+  const double2 b = U[((size_t)c * rows + neg) * n + (n - kz) % n];
+  H[id] = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+}
+
 double quintic_sigma(double h, int dim) {
   // jax_sph QuinticKernel normalisation (Morris, Fox & Zhu 1997); the 3-D
   // value 3/(359 pi) is the library's and integrates to 360/359
@@ -559,10 +780,11 @@ void ensure_cap(qlles_sph* S, int np) {
 
 int blocks(long long n, int b) { return (int)((n + b - 1) / b); }
 
-// steps 1-5; u (shift velocity) optional, uvel (FP64 velocities to gather) optional
+// steps 1-4 (cell list); u (shift velocity) optional, uvel (FP64 velocities
+// to gather into cell order) optional
 template <int DIM>
-void build_and_density(qlles_sph* S, const double* r, const void* u, int u_f32, double dt, int np,
-                       const double* uvel, double* rho_out, hipStream_t s) {
+void build_cells(qlles_sph* S, const double* r, const void* u, int u_f32, double dt, int np, const double* uvel,
+                 hipStream_t s) {
   ensure_cap(S, np);
   const SphGeom& g = S->g;
   // the counts start at zero: cleared at creation and by every scan
@@ -573,6 +795,15 @@ void build_and_density(qlles_sph* S, const double* r, const void* u, int u_f32, 
                      np);
   hipLaunchKernelGGL((k_sph_rank<DIM>), dim3(blocks(np, 256)), dim3(256), 0, s, S->cell_of, S->start, S->tmp, S->xw,
                      uvel, np, S->cap, S->perm, S->pos, S->vel);
+  QL_LAUNCHED();
+}
+
+// steps 1-5; u (shift velocity) optional, uvel (FP64 velocities to gather) optional
+template <int DIM>
+void build_and_density(qlles_sph* S, const double* r, const void* u, int u_f32, double dt, int np,
+                       const double* uvel, double* rho_out, hipStream_t s) {
+  build_cells<DIM>(S, r, u, u_f32, dt, np, uvel, s);
+  const SphGeom& g = S->g;
   PassOut o{};
   o.rho = rho_out;
   hipLaunchKernelGGL((k_sph_pass<DIM, PASS_DENSITY>), dim3(S->ncells), dim3(kBlock), 0, s, S->pos,
@@ -652,6 +883,8 @@ int qlles_sph_destroy(qlles_sph* S) {
   (void)hipDeviceSynchronize();
   free_all(S->cell_bufs);
   free_all(S->part_bufs);
+  if (S->dft_E) (void)hipFree(S->dft_E);
+  if (S->dft_U) (void)hipFree(S->dft_U);
   delete S;
   return QLLES_OK;
 }
@@ -730,6 +963,58 @@ int qlles_sph_relax_advect(qlles_sph* S, double* r, int np, void* u_r, int u_dty
       o.fdt2 = fd * fd;
       o.L = S->L;
       accel<DIM, ACC_ADVECT>(S, o, (hipStream_t)stream);
+    });
+  });
+}
+
+int qlles_sph_mls_grid(qlles_sph* S, const double* r, const double* u_r, int np, double* u_grid, void* stream) {
+  if (!S || !r || !u_r || !u_grid) return fail(QLLES_ERR_ARG, "null argument");
+  if (np < 1) return fail(QLLES_ERR_ARG, "np must be >= 1");
+  return guarded([&] {
+    by_dim(S, [&](auto D) {
+      constexpr int DIM = decltype(D)::value;
+      hipStream_t s = (hipStream_t)stream;
+      build_cells<DIM>(S, r, nullptr, 0, 0.0, np, u_r, s);
+      const int n = S->n;
+      const long long nodes = DIM == 3 ? (long long)n * n * n : (long long)n * n;
+      const double h = 2.0 / 3.0 * (S->g.box / n);  // turbulence.py:167
+      hipLaunchKernelGGL((k_sph_mls<DIM>), dim3(blocks(nodes, 64)), dim3(64), 0, s, S->pos, S->vel, S->start, S->g,
+                         n, h, quintic_sigma(h, DIM), u_grid);
+      QL_LAUNCHED();
+    });
+  });
+}
+
+int qlles_sph_dft_hat(qlles_sph* S, const double* r, const double* u_r, int np, void* u_hat, void* stream) {
+  if (!S || !u_hat) return fail(QLLES_ERR_ARG, "null argument");
+  if (np < 0) return fail(QLLES_ERR_ARG, "np must be >= 0");
+  if (np > 0 && (!r || !u_r)) return fail(QLLES_ERR_ARG, "null argument");
+  return guarded([&] {
+    by_dim(S, [&](auto D) {
+      constexpr int DIM = decltype(D)::value;
+      hipStream_t s = (hipStream_t)stream;
+      const int n = S->n;
+      const size_t rows = DIM == 3 ? (size_t)n * n : (size_t)n;
+      const size_t e_elems = (size_t)DIM * std::max(np, 1) * n;
+      if (e_elems > S->dft_E_elems) {
+        if (S->dft_E) {
+          QL_HIP(hipDeviceSynchronize());
+          QL_HIP(hipFree(S->dft_E));
+          S->dft_E = nullptr;
+        }
+        QL_HIP(hipMalloc(&S->dft_E, e_elems * sizeof(double2)));
+        S->dft_E_elems = e_elems;
+      }
+      if (!S->dft_U) QL_HIP(hipMalloc(&S->dft_U, (size_t)DIM * rows * n * sizeof(double2)));
+      double2* E = static_cast<double2*>(S->dft_E);
+      double2* U = static_cast<double2*>(S->dft_U);
+      if (np > 0)
+        hipLaunchKernelGGL(k_dft_phase, dim3(blocks((long long)DIM * np * n, 256)), dim3(256), 0, s, r, np, DIM, n,
+                           E);
+      hipLaunchKernelGGL((k_dft_sum<DIM>), dim3((unsigned)rows), dim3(256), 0, s, E, u_r, np, n, U);
+      hipLaunchKernelGGL((k_dft_half<DIM>), dim3(blocks(3LL * rows * (n / 2 + 1), 256)), dim3(256), 0, s, U, n,
+                         static_cast<double2*>(u_hat));
+      QL_LAUNCHED();
     });
   });
 }
