@@ -490,6 +490,60 @@ def run_relax(args):
     print(json.dumps(out), flush=True)
 
 
+DATASET_CASES = (  # the two dataset-generation configurations (configs/hit32_1.yaml, configs/kolm64_1.yaml)
+    dict(name="hit32_1", published_steps_per_s=2.1,
+         kw=dict(case="HIT", N=256, dim=3, nu=0.002, dt=0.002, u_ref=4.0, ckp_N=32, interp_backend="nufft",
+                 relax_dt_factor=2.0, forcing_type="ekin_tot", e_kin_target=1.25, kf=3)),
+    dict(name="kolm64_1", published_steps_per_s=3.9,
+         kw=dict(case="Kolm", N=512, dim=2, nu=0.001, dt=0.001, u_ref=6.0, ckp_N=64, interp_backend="dft",
+                 dft_splits=2, relax_dt_factor=1.4142135624, forcing_type="none")),
+)
+
+
+def run_dataset(args):
+    """The reference's dataset-generation loop end to end (combined.py:18-219): FP64 DNS step, filter +
+    interpolation at the particles, SPH relaxation, diagnostics every 20 steps and a particle frame
+    written every step (.npz here, .h5 in the reference), for both dataset configurations.  The reference
+    states 60 / 80 min per trajectory of 7501 / 18501 steps on one NVIDIA GPU (README.md:35-36), i.e.
+    about 2.1 / 3.9 steps/s including the same work."""
+    import torch
+
+    from l3es_amd.combined import combined
+
+    torch.cuda.set_device(0)
+    res = []
+    for case in DATASET_CASES:
+        kw = dict(case["kw"])
+        dst = tempfile.mkdtemp(prefix="qlles_dataset_", dir="/tmp")
+        try:
+            K = args.steps
+            out = combined(t_final=K * kw["dt"], t_burnin=0, log_freq=20, ckp_freq=1, vis_freq=10 ** 9,
+                           dst_path=dst, precision="float64", seed=args.seed, verbose=False, **kw)
+            frames = len(os.listdir(os.path.join(dst, "com")))
+        finally:
+            shutil.rmtree(dst, ignore_errors=True)
+        sps = out["steps"] / out["t_tot"]
+        row = dict(name=case["name"], steps=out["steps"], seconds=out["t_tot"], steps_per_s=sps,
+                   dns_s=out["t_sim"], particles_s=out["t_int"], frames_written=frames,
+                   rho_max_last=out["rows"][-1][6], published_steps_per_s=case["published_steps_per_s"],
+                   ratio_vs_published=sps / case["published_steps_per_s"])
+        log(f"[bench] dataset {case['name']}: {out['steps']} steps in {out['t_tot']:.2f}s -> {sps:.1f} steps/s "
+            f"(DNS {out['t_sim']:.2f}s, particles {out['t_int']:.2f}s, {frames} frames)")
+        res.append(row)
+    out = {
+        "metric": "dataset-generation steps/s, combined loop with SPH relaxation and per-step frames (hit32_1)",
+        "value": res[0]["steps_per_s"], "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": 0,
+        "ms_per_step": 1e3 / res[0]["steps_per_s"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "configs/hit32_1.yaml and kolm64_1.yaml after burn-in: FP64 DNS 256^3 (ekin_tot "
+                               "forcing switched off at burn-in) / 512^2 Kolmogorov, 32^3 NUFFT / 64^2 DFT particles, "
+                               "SPH relaxation factor 2 / sqrt 2, diagnostics every 20 steps, one .npz frame per "
+                               "step; Cartesian lattice start (the relaxed .h5 states are not available)"},
+        "cases": res,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -512,9 +566,10 @@ def main():
     ap.add_argument("--copy-probe-mib", dest="copy_probe", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (counter calibration)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH/WRITE passes")
-    ap.add_argument("--workload", choices=("hit256", "batch", "adjoint", "relax"), default="hit256",
+    ap.add_argument("--workload", choices=("hit256", "batch", "adjoint", "relax", "dataset"), default="hit256",
                     help="hit256: the headline 256^3 step (default); batch: 24 uneven batched trajectories; "
-                         "adjoint: gradient through K checkpointed steps; relax: SPH particle relaxation")
+                         "adjoint: gradient through K checkpointed steps; relax: SPH particle relaxation; "
+                         "dataset: the dataset-generation loops end to end")
     ap.add_argument("--adj-N", dest="adj_N", type=int, default=128)
     ap.add_argument("--adj-K", dest="adj_K", type=int, default=50)
     ap.add_argument("--adj-every", dest="adj_every", type=int, default=10)
@@ -556,6 +611,9 @@ def main():
         return
     if args.workload == "relax":
         run_relax(args)
+        return
+    if args.workload == "dataset":
+        run_dataset(args)
         return
 
     import torch

@@ -86,6 +86,8 @@ def combined(case, N=256, dim=3, nu=0.000625, t_final=0.1, t_burnin=0, dt=0.01, 
                 print(f"Step {i}/{tstep_max}, u_max = {d[1]:.3f}, E_kin = {d[0]:.3f}, dt_est = {dt_est:.5f}{rho_txt}")
         if is_ckp:
             np.savez(os.path.join(int_path, f"step_{i:05d}.npz"), r=r0.cpu().numpy(), u=u_r0.cpu().numpy())
+    t_tot = time() - t0
     if verbose:
-        print(f"t_tot = {time() - t0:.3f}, t_sim = {t_sim}, t_int = {t_int:.3f}")
-    return dict(u=u, u_hat=u_hat, r=r, u_r=u_r, rows=rows, t_sim=t_sim, t_int=t_int, e_inj_acc=e_acc)
+        print(f"t_tot = {t_tot:.3f}, t_sim = {t_sim}, t_int = {t_int:.3f}")
+    return dict(u=u, u_hat=u_hat, r=r, u_r=u_r, rows=rows, t_sim=t_sim, t_int=t_int, t_tot=t_tot, e_inj_acc=e_acc,
+                steps=tstep_max + 1)
