@@ -36,6 +36,8 @@ class EngineT final : public Engine {
   static constexpr int NAS = 10, NAP = 5;
   cx<T>* AS_[NAS] = {};
   T* AP_[NAP] = {};
+  // physical u and omega of the four RK stage states of a step (u0, w0, u1, w1, ...)
+  T* SF_[8] = {};
   double* partials_ = nullptr;
   double* spec_part_ = nullptr;  // per-block shell histograms (energy spectrum)
   double* alpha_ = nullptr;
@@ -179,8 +181,8 @@ class EngineT final : public Engine {
     QL_LAUNCHED();
     prof_end(s, K_YINV, (nf == 6 ? 11.0 : 6.0) * C(), e0);
   }
-  void z3(const T* u_in, T* u_out, hipStream_t s) {
-    NS3Z<T> a{Y6, NZ, u_out, u_in, qs(6), KZD, KZP, twN_};
+  void z3(const T* u_in, T* u_out, hipStream_t s, T* w_out = nullptr) {
+    NS3Z<T> a{Y6, NZ, u_out, u_in, qs(6), KZD, KZP, twN_, w_out};
     hipEvent_t e0 = nullptr;
     prof_begin(s, &e0);
     with_zv([&](auto v) {
@@ -190,7 +192,7 @@ class EngineT final : public Engine {
     QL_LAUNCHED();
     const double R = (double)N * N * N * sizeof(T);
     prof_end(s, K_ZPASS, (u_in ? 3.0 * C() + 3.0 * R : 6.0 * C()) + (u_out ? 3.0 * R : 0.0) +
-                             3.0 * N * N * KZD * 2.0 * sizeof(T), e0);
+                             (w_out ? 3.0 * R : 0.0) + 3.0 * N * N * KZD * 2.0 * sizeof(T), e0);
   }
   void yfwd3(hipStream_t s) {
     NS3YF<T> a{NZ, NY, KZD, KZP, KYD, twN_};
@@ -294,6 +296,7 @@ class EngineT final : public Engine {
     if (AS_[0]) return;
     for (int i = 0; i < NAS; ++i) AS_[i] = alloc<cx<T>>(spec_elems());
     for (int i = 0; i < NAP; ++i) AP_[i] = alloc<T>(phys_elems());
+    for (int i = 0; i < 8; ++i) SF_[i] = alloc<T>(phys_elems());
   }
   void check_adj() const {
     if (DIM != 3) throw std::invalid_argument("tangent / adjoint maps are implemented for 3-D grids");
@@ -321,19 +324,25 @@ class EngineT final : public Engine {
     hipLaunchKernelGGL((k_cross2<T>), dim3(nblk(n)), dim3(256), 0, s, a, b, c, d, out, n);
     QL_LAUNCHED();
   }
-  // u = irfftn(y) -> AP_[0], w = irfftn(K y) -> AP_[1]
-  void state_fields(const cx<T>* y, hipStream_t s) {
-    to_phys(y, AP_[0], s);
+  // u = irfftn(y) -> up, w = irfftn(K y) -> wp
+  void state_fields(const cx<T>* y, T* up, T* wp, hipStream_t s) {
+    to_phys(y, up, s);
     curl(y, AS_[8], s);
-    to_phys(AS_[8], AP_[1], s);
+    to_phys(AS_[8], wp, s);
   }
-  // out = R'(y) d  (temporaries AS_[7..9], AP_[0..4])
-  void rhs_jvp_at(const cx<T>* y, const cx<T>* d, cx<T>* out, hipStream_t s) {
-    state_fields(y, s);
+  // out = R'(y) d  (temporaries AS_[7..9], AP_[0..4]); up / wp: u and omega
+  // of y when the caller has them, else computed here
+  void rhs_jvp_at(const cx<T>* y, const cx<T>* d, cx<T>* out, hipStream_t s, const T* up = nullptr,
+                  const T* wp = nullptr) {
+    if (up == nullptr) {
+      state_fields(y, AP_[0], AP_[1], s);
+      up = AP_[0];
+      wp = AP_[1];
+    }
     to_phys(d, AP_[2], s);
     curl(d, AS_[8], s);
     to_phys(AS_[8], AP_[3], s);
-    cross2(AP_[2], AP_[1], AP_[0], AP_[3], AP_[4], s);  // du x w + u x dw
+    cross2(AP_[2], wp, up, AP_[3], AP_[4], s);  // du x w + u x dw
     to_spec(AP_[4], AS_[7], s);
     hipLaunchKernelGGL((k_jvp_combine<T>), dim3(nblk(rows_ * Nh)), dim3(256), 0, s, AS_[7], d, out, N, KZD,
                        (T)d_.nu);
@@ -346,14 +355,19 @@ class EngineT final : public Engine {
   }
   // out = R'(y)^T l  (temporaries AS_[7..9], AP_[0..4]); with fixed-power
   // forcing, *dpow (device, optional) accumulates <l, dR/dP>
-  void rhs_vjp_at(const cx<T>* y, const cx<T>* l, cx<T>* out, hipStream_t s, double* dpow = nullptr) {
-    state_fields(y, s);
+  void rhs_vjp_at(const cx<T>* y, const cx<T>* l, cx<T>* out, hipStream_t s, double* dpow = nullptr,
+                  const T* up = nullptr, const T* wp = nullptr) {
+    if (up == nullptr) {
+      state_fields(y, AP_[0], AP_[1], s);
+      up = AP_[0];
+      wp = AP_[1];
+    }
     hipLaunchKernelGGL((k_pd<T>), dim3(nblk(rows_ * Nh)), dim3(256), 0, s, l, AS_[8], N, KZD, 1);
     QL_LAUNCHED();
-    to_phys(AS_[8], AP_[2], s);                           // psi = irfftn(P D l / c)
-    cross2(AP_[1], AP_[2], nullptr, nullptr, AP_[3], s);  // w x psi
+    to_phys(AS_[8], AP_[2], s);                       // psi = irfftn(P D l / c)
+    cross2(wp, AP_[2], nullptr, nullptr, AP_[3], s);  // w x psi
     to_spec(AP_[3], AS_[7], s);
-    cross2(AP_[2], AP_[0], nullptr, nullptr, AP_[3], s);  // psi x u
+    cross2(AP_[2], up, nullptr, nullptr, AP_[3], s);  // psi x u
     to_spec(AP_[3], AS_[8], s);
     hipLaunchKernelGGL((k_vjp_combine<T>), dim3(nblk(rows_ * Nh)), dim3(256), 0, s, AS_[7], AS_[8], l, out, N, KZD,
                        (T)d_.nu);
@@ -364,15 +378,39 @@ class EngineT final : public Engine {
       QL_LAUNCHED();
     }
   }
-  // RK4 stage states y1, y2, y3 of the step from y0 (spectral_solver.py:74-80) -> AS_[0..2]
+  // fused RHS of y that also keeps u = irfftn(y) and omega = irfftn(K y), which
+  // its z-pass forms anyway (3-D)
+  void rhs_fields(const cx<T>* y, cx<T>* du, T* up, T* wp, hipStream_t s) {
+    cx<T>* uh = const_cast<cx<T>*>(y);
+    NS3X<T> a = xargs3(uh, Nh, 0);
+    a.c = ctl(-1);
+    a.us = uh;
+    a.ld_us = Nh;
+    a.do_inverse = 1;
+    x3(a, s);
+    yinv3(6, s);
+    z3(nullptr, up, s, wp);
+    yfwd3(s);
+    a = xargs3(uh, Nh, 0);
+    a.c = ctl(4);
+    a.us = uh;
+    a.ld_us = Nh;
+    a.uout = du;
+    a.ld_uout = Nh;
+    a.do_inverse = 0;
+    x3(a, s);
+  }
+  // RK4 stage states y1, y2, y3 of the step from y0 (spectral_solver.py:74-80)
+  // -> AS_[0..2], and u, omega of y0..y3 -> SF_[0..7]
   void stage_states(const cx<T>* y0, hipStream_t s) {
     const T h = (T)d_.dt;
-    rhs(nullptr, y0, 1, AS_[3], s);
+    rhs_fields(y0, AS_[3], SF_[0], SF_[1], s);
     axpby(y0, AS_[3], AS_[0], (T)1, h / 2, s);
-    rhs(nullptr, AS_[0], 1, AS_[3], s);
+    rhs_fields(AS_[0], AS_[3], SF_[2], SF_[3], s);
     axpby(y0, AS_[3], AS_[1], (T)1, h / 2, s);
-    rhs(nullptr, AS_[1], 1, AS_[3], s);
+    rhs_fields(AS_[1], AS_[3], SF_[4], SF_[5], s);
     axpby(y0, AS_[3], AS_[2], (T)1, h, s);
+    state_fields(AS_[2], SF_[6], SF_[7], s);
   }
 
  public:
@@ -648,7 +686,7 @@ class EngineT final : public Engine {
     const T wk[4] = {h / 6, h / 3, h / 3, h / 6};
     const T wy[3] = {h / 2, h / 2, h};
     for (int st = 0; st < 4; ++st) {
-      rhs_jvp_at(ys[st], st == 0 ? v : dy, dk, s);
+      rhs_jvp_at(ys[st], st == 0 ? v : dy, dk, s, SF_[2 * st], SF_[2 * st + 1]);
       axpby(st == 0 ? v : out, dk, out, (T)1, wk[st], s);
       if (st < 3) axpby(v, dk, dy, (T)1, wy[st], s);
     }
@@ -672,7 +710,7 @@ class EngineT final : public Engine {
     for (int q = 0; q < 4; ++q) {
       const int st = 3 - q;
       axpby(lam, q == 0 ? nullptr : yb, kb, wl[q], wb[q], s);
-      rhs_vjp_at(ys[st], kb, yb, s, dpow);
+      rhs_vjp_at(ys[st], kb, yb, s, dpow, SF_[2 * st], SF_[2 * st + 1]);
       axpby(acc, yb, st == 0 ? out : acc, (T)1, (T)1, s);
     }
   }

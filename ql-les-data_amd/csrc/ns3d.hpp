@@ -594,6 +594,7 @@ struct NS3Z {
   size_t q;
   int KZD, KZP;
   const cx<T>* tw;
+  T* w_out = nullptr;  // optional physical vorticity [3][N][N][N] (fields 3..5 after the c2r)
 };
 
 // Radix plan (forward order) of the half-length FFTs in the z-pass; the c2r
@@ -753,6 +754,13 @@ struct ZState {
 #pragma unroll
     for (int s = 0; s < NO; ++s) v[s] = cscale(v[s], scale);
     // (re, im) hold the (even, odd) z samples; u x omega accumulated per field
+    if constexpr (F >= 3) {
+      if (a.w_out != nullptr) {
+        cx<T>* dst = reinterpret_cast<cx<T>*>(a.w_out + ((size_t)(F - 3) * N * N + row) * N);
+#pragma unroll
+        for (int s = 0; s < NO; ++s) dst[CI::out_index(j, s)] = v[s];
+      }
+    }
     if constexpr (F < 3) {
 #pragma unroll
       for (int s = 0; s < NO; ++s) uu[F][s] = v[s];
