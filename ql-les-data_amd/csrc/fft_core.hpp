@@ -41,6 +41,18 @@ template <typename T> __device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { re
 template <typename T> __device__ __forceinline__ cx<T> cneg(cx<T> a) { return {-a.re, -a.im}; }
 // multiply by +i
 template <typename T> __device__ __forceinline__ cx<T> cmuli(cx<T> a) { return {-a.im, a.re}; }
+// store of a value that a later kernel reads back: NT selects a non-temporal
+// store, which MI355X retires at a higher write rate than a cached one but
+// which leaves nothing in the last-level cache for the reader
+template <bool NT, typename T> __device__ __forceinline__ void st_stream(cx<T>* p, cx<T> v) {
+  if constexpr (NT) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    v2 x = {v.re, v.im};
+    __builtin_nontemporal_store(x, reinterpret_cast<v2*>(p));
+  } else {
+    *p = v;
+  }
+}
 // multiply by -i (FWD) or +i (!FWD)
 template <typename T, bool FWD> __device__ __forceinline__ cx<T> crot(cx<T> a) {
   if constexpr (FWD) return {a.im, -a.re};

@@ -136,6 +136,32 @@ template <> struct XPlan<128> { using R = Radices<4, 4, 8>; static constexpr int
 #ifndef QL_TIX
 #define QL_TIX 16
 #endif
+// Non-temporal stores per output stream (st_stream): x-pass stage fields and
+// X5, y-inverse Y6, z-pass NZ, y-forward NY.  They apply only where one field
+// component (N^3 reals) reaches 64 MiB: smaller fields stay in the 256 MB
+// last-level cache between kernels.  A cached store leaves dirty lines that
+// are written back while the next kernel runs; at 256^3 FP32 the choice below
+// (all streams non-temporal except X5, whose reader is the very next kernel)
+// took one RK stage from 623 to 602 us.
+#ifndef QL_NT_XS
+#define QL_NT_XS 1
+#endif
+#ifndef QL_NT_X5
+#define QL_NT_X5 0
+#endif
+#ifndef QL_NT_YI
+#define QL_NT_YI 1
+#endif
+#ifndef QL_NT_Z
+#define QL_NT_Z 1
+#endif
+#ifndef QL_NT_YF
+#define QL_NT_YF 1
+#endif
+template <typename T, int N>
+struct NTBig {
+  static constexpr bool value = (size_t)N * N * N * sizeof(T) >= ((size_t)64 << 20);
+};
 
 // V selects the tile width: 0 = QL_TIX columns, 1 = half of it (more
 // workgroups for small grids; chosen per plan by the autotuner)
@@ -397,8 +423,9 @@ k_ns3_x(NS3X<T> a) {
           }
         }
         if (valid) {
-          if constexpr (AccOut<SG, VIF>::value) p_acc[l_acc.at(cc, row)] = acc_new;
-          p_out[l_out.at(cc, row)] = out;
+          constexpr bool nt = QL_NT_XS && NTBig<T, N>::value;
+          if constexpr (AccOut<SG, VIF>::value) st_stream<nt>(&p_acc[l_acc.at(cc, row)], acc_new);
+          st_stream<nt>(&p_out[l_out.at(cc, row)], out);
         }
         w[cc][s] = valid ? out : zero;
         if (c.partials != nullptr && valid) {
@@ -477,7 +504,8 @@ k_ns3_x(NS3X<T> a) {
         const ColAt l_x5 = ColAt::make<N>(a.ld, a.q_x5, kz);
 #pragma unroll
         for (int s = 0; s < IF::NOUT; ++s)
-          if (IF::out_valid(j, s)) a.x5[l_x5.at(f, (size_t)IF::out_index(j, s) * N + kyi)] = v[s];
+          if (IF::out_valid(j, s))
+            st_stream<QL_NT_X5 && NTBig<T, N>::value>(&a.x5[l_x5.at(f, (size_t)IF::out_index(j, s) * N + kyi)], v[s]);
       }
     }
   }
@@ -527,7 +555,8 @@ k_ns3_yinv(NS3Y<T> a) {
     if (!ok) return;
 #pragma unroll
     for (int s = 0; s < F::NOUT; ++s)
-      if (F::out_valid(j, s)) a.out[lo.at(f, row0 + F::out_index(j, s))] = v[s];
+      if (F::out_valid(j, s))
+        st_stream<QL_NT_YI && NTBig<T, N>::value>(&a.out[lo.at(f, row0 + F::out_index(j, s))], v[s]);
   };
   // loads run ahead of the transforms: each field is requested one or two
   // transforms before it is consumed (software pipelining)
@@ -656,7 +685,7 @@ __device__ __forceinline__ void r2c_post_store(cx<T>* dst, const cx<T>* pb, cons
     const cx<T> zr = cconj(pb[lds_at<PAD>(k == 0 ? 0 : M - k)]);
     const cx<T> e = cscale(cadd(zk, zr), (T)0.5);
     const cx<T> o = crot<T, true>(cscale(csub(zk, zr), (T)0.5));
-    dst[k] = cadd(e, cmul(s_tw[k], o));
+    st_stream<QL_NT_Z && NTBig<T, 2 * M>::value>(&dst[k], cadd(e, cmul(s_tw[k], o)));
   }
 }
 
@@ -906,7 +935,7 @@ k_ns3_yfwd(NS3YF<T> a) {
         const int k = F::out_index(j, s);
         if (F::out_valid(j, s) && keep_k(k, N, a.KZD)) {
           const int kd = k < a.KZD ? k : k - (N - a.KYD);
-          a.ny[bo + (size_t)kd * a.KZP] = v[s];
+          st_stream<QL_NT_YF && NTBig<T, N>::value>(&a.ny[bo + (size_t)kd * a.KZP], v[s]);
         }
       }
     }
