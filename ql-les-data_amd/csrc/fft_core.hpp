@@ -53,6 +53,17 @@ template <bool NT, typename T> __device__ __forceinline__ void st_stream(cx<T>* 
     *p = v;
   }
 }
+// load of a value read once from a stream far larger than the caches (NT:
+// non-temporal load)
+template <bool NT, typename T> __device__ __forceinline__ cx<T> ld_stream(const cx<T>* p) {
+  if constexpr (NT) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    const v2 x = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+    return {x.x, x.y};
+  } else {
+    return *p;
+  }
+}
 // multiply by -i (FWD) or +i (!FWD)
 template <typename T, bool FWD> __device__ __forceinline__ cx<T> crot(cx<T> a) {
   if constexpr (FWD) return {a.im, -a.re};

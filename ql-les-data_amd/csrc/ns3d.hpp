@@ -158,6 +158,12 @@ template <> struct XPlan<128> { using R = Radices<4, 4, 8>; static constexpr int
 #ifndef QL_NT_YF
 #define QL_NT_YF 1
 #endif
+// non-temporal loads (ld_stream) of the large input streams, each read once:
+// NY, u_s, u_n and the accumulator in the x-pass, X5 in the y-inverse, Y6 in
+// the z-pass, NZ in the y-forward (at 256^3 FP32 one RK stage 602 -> 578 us)
+#ifndef QL_NTL
+#define QL_NTL 1
+#endif
 template <typename T, int N>
 struct NTBig {
   static constexpr bool value = (size_t)N * N * N * sizeof(T) >= ((size_t)64 << 20);
@@ -260,6 +266,7 @@ template <typename T, int N, int SG, bool VIF, int V = 0>
 __global__ void __launch_bounds__((XCfg<N, V>::THREADS)) QL_ATTR_X
 k_ns3_x(NS3X<T> a) {
   using C = XCfg<N, V>;
+  constexpr bool NTL = QL_NTL && NTBig<T, N>::value;
   constexpr int Nh = N / 2 + 1;
   using FF = XFwd<T, N>;
   using IF = XInv<T, N>;
@@ -317,7 +324,7 @@ k_ns3_x(NS3X<T> a) {
 #pragma unroll
         for (int s = 0; s < FF::NIN; ++s) {
           pre[cc][s] = (fwd_ok && FF::in_valid(j, s))
-                           ? a.ny[((size_t)(cc * N + FF::in_index(j, s)) * KYD + kyd) * a.KZP + kz]
+                           ? ld_stream<NTL>(&a.ny[((size_t)(cc * N + FF::in_index(j, s)) * KYD + kyd) * a.KZP + kz])
                            : zero;
         }
       }
@@ -360,9 +367,9 @@ k_ns3_x(NS3X<T> a) {
       cx<T> u3[3], n3[3], c3[3];
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc) {
-        u3[cc] = p_us[l_us.at(cc, row)];
-        if constexpr (SG == 1 || SG == 2) n3[cc] = p_un[l_un.at(cc, row)];
-        if constexpr (AccIn<SG, VIF>::value) c3[cc] = p_acc[l_acc.at(cc, row)];
+        u3[cc] = ld_stream<NTL>(&p_us[l_us.at(cc, row)]);
+        if constexpr (SG == 1 || SG == 2) n3[cc] = ld_stream<NTL>(&p_un[l_un.at(cc, row)]);
+        if constexpr (AccIn<SG, VIF>::value) c3[cc] = ld_stream<NTL>(&p_acc[l_acc.at(cc, row)]);
       }
       const cx<T> n0 = keep_x ? w[0][s] : zero;
       const cx<T> n1 = keep_x ? w[1][s] : zero;
@@ -528,6 +535,7 @@ template <typename T, int N, int V = 0>
 __global__ void __launch_bounds__((XCfg<N, V>::THREADS)) QL_ATTR_YI
 k_ns3_yinv(NS3Y<T> a) {
   using C = XCfg<N, V>;
+  constexpr bool NTL = QL_NTL && NTBig<T, N>::value;
   constexpr int Nh = N / 2 + 1;
   using F = XInv<T, N>;
   __shared__ cx<T> s_tw[F::TWL > 0 ? F::TWL : 1];
@@ -549,7 +557,8 @@ k_ns3_yinv(NS3Y<T> a) {
   auto load = [&](int f, cx<T>* v) {
 #pragma unroll
     for (int s = 0; s < F::NIN; ++s)
-      v[s] = (ok && F::in_valid(j, s)) ? a.in[li.at(f, row0 + F::in_index(j, s))] : cx<T>{0, 0};
+      v[s] = (ok && F::in_valid(j, s)) ? ld_stream<NTL>(&a.in[li.at(f, row0 + F::in_index(j, s))])
+                                       : cx<T>{0, 0};
   };
   auto store = [&](int f, const cx<T>* v) {
     if (!ok) return;
@@ -716,6 +725,7 @@ struct ZState {
   // fields in flight: two in FP32; one in FP64, whose larger registers would
   // otherwise push the ring into private memory
   static constexpr int DEPTH = sizeof(T) == 4 ? QL_ZDEPTH32 : 1;
+  static constexpr bool NTL = QL_NTL && NTBig<T, N>::value;
   const NS3Z<T>& a;
   cx<T>* s_tw;   // flat N-point table (c2r / r2c pre- and post-processing)
   cx<T>* s_twi;  // per-plan table of the inverse half-length FFT
@@ -748,7 +758,7 @@ struct ZState {
 #pragma unroll
     for (int e = 0; e < NEM; ++e) {
       const int idx = ln + e * WL;
-      if (idx < PWV * M) dst[e] = src[idx];
+      if (idx < PWV * M) dst[e] = ld_stream<NTL>(&src[idx]);
     }
     if (ln < PWV) dst[NEM] = a.in[a.q + (size_t)f * N * N + r0 + wrow + ln];
   }
@@ -905,6 +915,7 @@ template <typename T, int N, int V = 0>
 __global__ void __launch_bounds__((XCfg<N, V>::THREADS)) QL_ATTR_YF
 k_ns3_yfwd(NS3YF<T> a) {
   using C = XCfg<N, V>;
+  constexpr bool NTL = QL_NTL && NTBig<T, N>::value;
   using F = XFwd<T, N>;
   __shared__ cx<T> s_tw[F::TWL > 0 ? F::TWL : 1];
   __shared__ cx<T> s_buf[N * C::LS];
@@ -920,7 +931,8 @@ k_ns3_yfwd(NS3YF<T> a) {
     const size_t bi = ((size_t)cc * N + xi) * N * a.KZP + kz;
 #pragma unroll
     for (int s = 0; s < F::NIN; ++s)
-      pre[cc][s] = (ok && F::in_valid(j, s)) ? a.nz[bi + (size_t)F::in_index(j, s) * a.KZP] : cx<T>{0, 0};
+      pre[cc][s] = (ok && F::in_valid(j, s)) ? ld_stream<NTL>(&a.nz[bi + (size_t)F::in_index(j, s) * a.KZP])
+                                             : cx<T>{0, 0};
   }
 #pragma unroll
   for (int cc = 0; cc < 3; ++cc) {
