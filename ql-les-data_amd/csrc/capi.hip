@@ -163,6 +163,20 @@ int qlles_step_vjp(qlles_plan* p, const void* u_hat, const void* lam, void* out,
   return guarded([&] { p->eng->step_vjp(u_hat, lam, out, dpower, (hipStream_t)stream); });
 }
 
+size_t qlles_step_record_bytes(const qlles_plan* p) { return p ? p->eng->record_bytes() : 0; }
+
+int qlles_step_record(qlles_plan* p, const void* u_hat, void* u_hat_out, void* record, void* stream) {
+  if (!p || !u_hat || !u_hat_out || !record) return fail(QLLES_ERR_ARG, "null argument");
+  return guarded([&] { p->eng->step_record(u_hat, u_hat_out, record, (hipStream_t)stream); });
+}
+
+int qlles_step_vjp_record(qlles_plan* p, const void* u_hat, const void* record, const void* lam, void* out,
+                          double* dpower, void* stream) {
+  if (!p || !u_hat || !record || !lam || !out) return fail(QLLES_ERR_ARG, "null argument");
+  if (out == u_hat || out == lam) return fail(QLLES_ERR_ARG, "out may not alias an input");
+  return guarded([&] { p->eng->step_vjp_record(u_hat, record, lam, out, dpower, (hipStream_t)stream); });
+}
+
 int qlles_irfftn_adjoint(qlles_plan* p, const void* g, void* out, void* stream) {
   if (!p || !g || !out) return fail(QLLES_ERR_ARG, "null argument");
   return guarded([&] { p->eng->irfftn_adjoint(g, out, (hipStream_t)stream); });

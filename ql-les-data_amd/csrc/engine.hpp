@@ -46,6 +46,11 @@ class Engine {
   virtual void rhs_vjp(const void* uhat, const void* lam, void* out, double* dpow, hipStream_t s) = 0;
   virtual void step_jvp(const void* uhat, const void* v, void* out, hipStream_t s) = 0;
   virtual void step_vjp(const void* uhat, const void* lam, void* out, double* dpow, hipStream_t s) = 0;
+  // one RK4 step taken stage by stage, keeping what step_vjp_record needs
+  virtual size_t record_bytes() const = 0;
+  virtual void step_record(const void* uhat, void* uhat_out, void* rec, hipStream_t s) = 0;
+  virtual void step_vjp_record(const void* uhat, const void* rec, const void* lam, void* out, double* dpow,
+                               hipStream_t s) = 0;
   // adjoints of irfftn (c/N^d rfftn) and rfftn (N^d irfftn(. / c))
   virtual void irfftn_adjoint(const void* g, void* out, hipStream_t s) = 0;
   virtual void rfftn_adjoint(const void* gh, void* out, hipStream_t s) = 0;

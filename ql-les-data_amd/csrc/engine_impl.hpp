@@ -695,14 +695,66 @@ class EngineT final : public Engine {
   void step_vjp(const void* uhat, const void* lam_in, void* out_v, double* dpow, hipStream_t s) override {
     check_adj();
     ensure_adj();
+    const cx<T>* y0 = static_cast<const cx<T>*>(uhat);
+    stage_states(y0, s);
+    const cx<T>* ys[4] = {y0, AS_[0], AS_[1], AS_[2]};
+    const T* fs[8] = {SF_[0], SF_[1], SF_[2], SF_[3], SF_[4], SF_[5], SF_[6], SF_[7]};
+    vjp_stages(ys, fs, static_cast<const cx<T>*>(lam_in), static_cast<cx<T>*>(out_v), dpow, s);
+  }
+  // a step record: the stage states y1, y2, y3 (spectra), then u and omega of
+  // y0..y3 (physical), in the plan dtype
+  size_t record_bytes() const override {
+    return DIM == 3 ? 3 * spec_elems() * sizeof(cx<T>) + 8 * phys_elems() * sizeof(T) : 0;
+  }
+  void record_views(const void* rec, const cx<T>* (&ys)[3], const T* (&fs)[8]) const {
+    const cx<T>* sp = static_cast<const cx<T>*>(rec);
+    for (int i = 0; i < 3; ++i) ys[i] = sp + i * spec_elems();
+    const T* ph = reinterpret_cast<const T*>(sp + 3 * spec_elems());
+    for (int i = 0; i < 8; ++i) fs[i] = ph + i * phys_elems();
+  }
+  // u_hat_out = y0 + h/6 (k1 + 2 k2 + 2 k3 + k4) (spectral_solver.py:70-80),
+  // one RHS per stage; uhat_out may be uhat
+  void step_record(const void* uhat, void* uhat_out, void* rec, hipStream_t s) override {
+    check_adj();
+    ensure_adj();
     const T h = (T)d_.dt;
     const cx<T>* y0 = static_cast<const cx<T>*>(uhat);
-    cx<T>* out = static_cast<cx<T>*>(out_v);
+    const cx<T>* ys[3];
+    const T* fs[8];
+    record_views(rec, ys, fs);
+    cx<T>* y[3] = {const_cast<cx<T>*>(ys[0]), const_cast<cx<T>*>(ys[1]), const_cast<cx<T>*>(ys[2])};
+    T* f[8];
+    for (int i = 0; i < 8; ++i) f[i] = const_cast<T*>(fs[i]);
+    cx<T>*k = AS_[3], *acc = AS_[5];
+    rhs_fields(y0, k, f[0], f[1], s);
+    axpby(y0, k, y[0], (T)1, h / 2, s);
+    axpby(y0, k, acc, (T)1, h / 6, s);
+    rhs_fields(y[0], k, f[2], f[3], s);
+    axpby(y0, k, y[1], (T)1, h / 2, s);
+    axpby(acc, k, acc, (T)1, h / 3, s);
+    rhs_fields(y[1], k, f[4], f[5], s);
+    axpby(y0, k, y[2], (T)1, h, s);
+    axpby(acc, k, acc, (T)1, h / 3, s);
+    rhs_fields(y[2], k, f[6], f[7], s);
+    axpby(acc, k, static_cast<cx<T>*>(uhat_out), (T)1, h / 6, s);
+  }
+  void step_vjp_record(const void* uhat, const void* rec, const void* lam_in, void* out_v, double* dpow,
+                       hipStream_t s) override {
+    check_adj();
+    ensure_adj();
+    const cx<T>* yr[3];
+    const T* fs[8];
+    record_views(rec, yr, fs);
+    const cx<T>* ys[4] = {static_cast<const cx<T>*>(uhat), yr[0], yr[1], yr[2]};
+    vjp_stages(ys, fs, static_cast<const cx<T>*>(lam_in), static_cast<cx<T>*>(out_v), dpow, s);
+  }
+  // backward through the four stages at the stage states ys with their u, omega fs
+  void vjp_stages(const cx<T>* const (&ys)[4], const T* const (&fs)[8], const cx<T>* lam_in, cx<T>* out,
+                  double* dpow, hipStream_t s) {
+    const T h = (T)d_.dt;
     cx<T>* lam = AS_[6];
     QL_HIP(hipMemcpyAsync(lam, lam_in, spec_elems() * sizeof(cx<T>), hipMemcpyDeviceToDevice, s));
-    stage_states(y0, s);
     cx<T>*kb = AS_[3], *yb = AS_[4], *acc = AS_[5];
-    const cx<T>* ys[4] = {y0, AS_[0], AS_[1], AS_[2]};
     axpby(lam, nullptr, acc, (T)1, (T)0, s);
     // kbar_4 = h/6 l; kbar_3 = h/3 l + h ybar_3; kbar_2 = h/3 l + h/2 ybar_2; kbar_1 = h/6 l + h/2 ybar_1
     const T wl[4] = {h / 6, h / 3, h / 3, h / 6};  // for stages 3, 2, 1, 0
@@ -710,7 +762,7 @@ class EngineT final : public Engine {
     for (int q = 0; q < 4; ++q) {
       const int st = 3 - q;
       axpby(lam, q == 0 ? nullptr : yb, kb, wl[q], wb[q], s);
-      rhs_vjp_at(ys[st], kb, yb, s, dpow, SF_[2 * st], SF_[2 * st + 1]);
+      rhs_vjp_at(ys[st], kb, yb, s, dpow, fs[2 * st], fs[2 * st + 1]);
       axpby(acc, yb, st == 0 ? out : acc, (T)1, (T)1, s);
     }
   }

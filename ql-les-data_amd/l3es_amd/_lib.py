@@ -36,6 +36,7 @@ EXPORTS = (
     "qlles_rfftn_adjoint", "qlles_interp_hat_vjp", "qlles_plan_set_variant", "qlles_plan_get_variant",
     "qlles_plan_tune", "qlles_energy_spectrum", "qlles_sph_create", "qlles_sph_destroy", "qlles_sph_density",
     "qlles_sph_relax_acc", "qlles_sph_forces", "qlles_sph_relax_advect", "qlles_sph_mls_grid", "qlles_sph_dft_hat",
+    "qlles_step_record_bytes", "qlles_step_record", "qlles_step_vjp_record",
 )
 KINDS = ("x-pass", "y-inverse", "z-pass", "y-forward", "other")
 
@@ -117,6 +118,9 @@ def load():
         "qlles_sph_relax_advect": (ci, [vp, vp, ci, vp, ci, cd, cd, vp, vp]),
         "qlles_sph_mls_grid": (ci, [vp, vp, vp, ci, vp, vp]),
         "qlles_sph_dft_hat": (ci, [vp, vp, vp, ci, vp, vp]),
+        "qlles_step_record_bytes": (ctypes.c_size_t, [vp]),
+        "qlles_step_record": (ci, [vp, vp, vp, vp, vp]),
+        "qlles_step_vjp_record": (ci, [vp, vp, vp, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

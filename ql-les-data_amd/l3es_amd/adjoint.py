@@ -10,7 +10,10 @@ sum_x a.b for physical fields.
 
 ``integrate(step, u_hat0, nsteps, every)`` is a torch.autograd.Function: its forward takes nsteps fused
 RK4 steps and keeps a checkpoint every ``every`` steps; its backward recomputes each segment from its
-checkpoint and applies the step adjoint in reverse.  ``irfftn(u_hat, plan)`` is differentiable too.
+checkpoint and applies the step adjoint in reverse.  When a segment's step records fit in a quarter of
+the free device memory, the recompute takes recorded steps (qlles_step_record: stage by stage, keeping
+the stage states and their physical u, omega) and the adjoint reads them (qlles_step_vjp_record)
+instead of recomputing the stages.  ``irfftn(u_hat, plan)`` is differentiable too.
 """
 
 import torch
@@ -51,6 +54,60 @@ def _advance(step, uh):
     step.advance(None, uh, 1, None, use_u=False)
 
 
+def record_bytes(step):
+    """Bytes of one step record (0 where recorded steps are not available)."""
+    return int(_lib.load().qlles_step_record_bytes(step.plan.handle))
+
+
+def step_record(step, u_hat, record, out=None):
+    """One RK4 step of u_hat taken stage by stage, its stage states kept in ``record`` (a uint8 device
+    tensor of record_bytes(step)); returns the stepped spectrum (``out``, may be u_hat)."""
+    out = torch.empty_like(u_hat) if out is None else out
+    _lib.check(_lib.load().qlles_step_record(step.plan.handle, _lib.ptr(u_hat), _lib.ptr(out), _lib.ptr(record),
+                                             _lib.stream_ptr()))
+    return out
+
+
+def step_vjp_record(step, u_hat, record, lam, dpower=None):
+    """step_vjp at the u_hat ``record`` was made from, reading the record."""
+    out = torch.empty_like(u_hat)
+    _lib.check(_lib.load().qlles_step_vjp_record(step.plan.handle, _lib.ptr(u_hat), _lib.ptr(record),
+                                                 _lib.ptr(lam.contiguous()), _lib.ptr(out), _lib.ptr(dpower),
+                                                 _lib.stream_ptr()))
+    return out
+
+
+def _records(step, every, like):
+    """``every`` step records when they fit in a quarter of the free device memory, else None."""
+    rb = record_bytes(step)
+    if rb == 0:
+        return None
+    free, _ = torch.cuda.mem_get_info(like.device)
+    if every * rb > free // 4:
+        return None
+    return [torch.empty(rb, dtype=torch.uint8, device=like.device) for _ in range(every)]
+
+
+def _replay(step, ckpt, nsteps, records):
+    """The states y_0 .. y_nsteps of a segment from its checkpoint y_0; with records, step i is recorded
+    into records[i]."""
+    states = [ckpt]
+    uh = ckpt.clone()
+    for i in range(nsteps):
+        if records is not None:
+            uh = step_record(step, uh, records[i])
+        else:
+            _advance(step, uh)
+        states.append(uh.clone() if records is None else uh)
+    return states
+
+
+def _step_adjoint(step, states, records, i, lam, dpow):
+    if records is not None:
+        return step_vjp_record(step, states[i], records[i], lam, dpow)
+    return step_vjp(step, states[i], lam, dpow)
+
+
 class _Integrate(torch.autograd.Function):
     @staticmethod
     def forward(ctx, u_hat0, power, step, nsteps, every):
@@ -68,16 +125,14 @@ class _Integrate(torch.autograd.Function):
         step, nsteps, every = ctx.step, ctx.nsteps, ctx.every
         lam = grad.contiguous()
         dpow = torch.zeros(1, dtype=torch.float64, device=grad.device) if ctx.needs_input_grad[1] else None
+        records = _records(step, every, lam)
         for seg in reversed(range(len(ctx.ckpts))):
             start = seg * every
             stop = min(start + every, nsteps)
-            states = [ctx.ckpts[seg]]
-            uh = ctx.ckpts[seg].clone()
-            for _ in range(start, stop - 1):  # recompute the segment's step inputs
-                _advance(step, uh)
-                states.append(uh.clone())
-            for y in reversed(states):
-                lam = step_vjp(step, y, lam, dpow)
+            # the segment's step inputs (the last step is taken only when it is recorded)
+            states = _replay(step, ctx.ckpts[seg], stop - start - (records is None), records)
+            for i in reversed(range(stop - start)):
+                lam = _step_adjoint(step, states, records, i, lam, dpow)
         ctx.ckpts = None
         gp = None if dpow is None else dpow.reshape(()).to(torch.float64)
         return lam, gp, None, None, None
@@ -165,19 +220,15 @@ class _IntegrateParticles(torch.autograd.Function):
         lam = torch.zeros_like(ctx.ckpts[0]) if g_uh is None else g_uh.contiguous().clone()
         lr = torch.zeros_like(ctx.rs[0]) if g_r is None else g_r.to(torch.float64).contiguous().clone()
         dpow = torch.zeros(1, dtype=torch.float64, device=lam.device) if ctx.needs_input_grad[2] else None
+        records = _records(step, every, lam)
         for seg in reversed(range(len(ctx.ckpts))):
             start = seg * every
             stop = min(start + every, nsteps)
-            states = [ctx.ckpts[seg]]
-            uh = ctx.ckpts[seg].clone()
-            for _ in range(start, stop):  # u_hat_start .. u_hat_stop
-                _advance(step, uh)
-                states.append(uh.clone())
+            states = _replay(step, ctx.ckpts[seg], stop - start, records)  # u_hat_start .. u_hat_stop
             for n in reversed(range(start, stop)):
-                y_next, y = states[n - start + 1], states[n - start]
                 # r_{n+1} = r_n + dt u_r(u_hat_{n+1}, r_n)
-                lam, lr = interp_vjp(interp, y_next, ctx.rs[n], dt * lr, lam, lr.clone())
-                lam = step_vjp(step, y, lam, dpow)
+                lam, lr = interp_vjp(interp, states[n - start + 1], ctx.rs[n], dt * lr, lam, lr.clone())
+                lam = _step_adjoint(step, states, records, n - start, lam, dpow)
         ctx.ckpts = ctx.rs = None
         gp = None if dpow is None else dpow.reshape(()).to(torch.float64)
         return lam, lr, gp, None, None, None, None

@@ -256,3 +256,58 @@ def test_particle_loss_gradients_match_central_difference(gpu):
     an_p = float(p.grad)
     assert abs(fd_u - an_u) <= 1e-4 * abs(an_u), (fd_u, an_u)
     assert abs(fd_p - an_p) <= 1e-4 * abs(an_p) + 1e-14, (fd_p, an_p)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,tol", [("float64", 1e-12), ("float32", 2e-5)])
+@pytest.mark.parametrize("power", [None, 0.3])
+def test_recorded_step_and_its_adjoint(gpu, prec, tol, power):
+    """qlles_step_record equals the fused step to rounding, and qlles_step_vjp_record equals
+    qlles_step_vjp (and the power cotangent) at the same u_hat."""
+    from l3es_amd import adjoint as A
+
+    uh, step = _setup(16, prec, power=power)
+    rec = torch.empty(A.record_bytes(step), dtype=torch.uint8, device=uh.device)
+    nxt = A.step_record(step, uh, rec)
+    ref = uh.clone()
+    step.advance(None, ref, 1, None, use_u=False)
+    scale = float(torch.linalg.vector_norm(ref))
+    assert float(torch.linalg.vector_norm(nxt - ref)) < tol * scale
+    lam = _rand_hat(uh, 11)
+    dp_a = torch.zeros(1, dtype=torch.float64, device=uh.device)
+    dp_b = torch.zeros(1, dtype=torch.float64, device=uh.device)
+    a = A.step_vjp_record(step, uh, rec, lam, dp_a)
+    b = A.step_vjp(step, uh, lam, dp_b)
+    assert float(torch.linalg.vector_norm(a - b)) < tol * float(torch.linalg.vector_norm(b))
+    if power is not None:
+        assert abs(float(dp_a) - float(dp_b)) <= tol * abs(float(dp_b))
+    # in place (u_hat_out == u_hat) gives the same step
+    uh2 = uh.clone()
+    A.step_record(step, uh2, rec, out=uh2)
+    assert torch.equal(uh2, nxt)
+
+
+@pytest.mark.gpu
+def test_gradient_with_and_without_step_records(gpu, monkeypatch):
+    """The checkpointed gradient is the same whether the segments are replayed with recorded steps or
+    with fused steps and recomputed stages."""
+    from l3es_amd import adjoint as A
+    from l3es_amd.integrator import pos_init_cartesian
+
+    uh0, step = _setup(16, "float64", seed=7, power=0.3)
+    it = _interp(8, "float64")
+    r0 = torch.from_numpy(pos_init_cartesian(8, 3)).cuda()
+
+    def grads():
+        x = uh0.clone().requires_grad_(True)
+        p = torch.tensor(0.3, dtype=torch.float64, device=uh0.device, requires_grad=True)
+        uh, r = A.integrate_particles(step, it, x, r0, 4, 3, power=p)
+        loss = torch.mean(torch.sin(r) ** 2) + 1e-3 * torch.sum(uh.abs() ** 2)
+        loss.backward()
+        return x.grad.clone(), float(p.grad)
+
+    g1, p1 = grads()
+    monkeypatch.setattr(A, "_records", lambda step, every, like: None)
+    g2, p2 = grads()
+    assert float(torch.linalg.vector_norm(g1 - g2)) < 1e-10 * float(torch.linalg.vector_norm(g2))
+    assert abs(p1 - p2) <= 1e-10 * abs(p2)
