@@ -164,9 +164,16 @@ template <> struct XPlan<128> { using R = Radices<4, 4, 8>; static constexpr int
 #ifndef QL_NTL
 #define QL_NTL 1
 #endif
+// smallest field component (N^3 reals, MiB) that takes the streaming stores and
+// loads; below it the last-level cache holds enough of the data between
+// kernels (with no gate: 24 batched trajectories of 64^3-128^3 9440 -> 8840
+// trajectory-steps/s, the 128^3 adjoint forward 20.0 -> 22.7 ms)
+#ifndef QL_NT_MIN_MIB
+#define QL_NT_MIN_MIB 64
+#endif
 template <typename T, int N>
 struct NTBig {
-  static constexpr bool value = (size_t)N * N * N * sizeof(T) >= ((size_t)64 << 20);
+  static constexpr bool value = (size_t)N * N * N * sizeof(T) >= ((size_t)QL_NT_MIN_MIB << 20);
 };
 
 // V selects the tile width: 0 = QL_TIX columns, 1 = half of it (more
