@@ -450,3 +450,28 @@ def test_config_kolmogorov256_particles(gpu, prec, tf, tr):
     d = np.abs(host(r) - ro)
     d = np.minimum(d, 2 * np.pi - d)  # periodic distance
     assert d.max() < tr
+
+
+def test_simulate_driver_tgv(gpu, tmp_path):
+    """The simulation driver end to end (spectral_solver.py:233-367) on TGV 32^3 FP64: the set-up's
+    warm-up step plus t_final/dt + 1 steps against the oracle, the diagnostics CSV, and the spectrally
+    filtered .npy checkpoints (utils.py:141-166)."""
+    import csv as _csv
+    import os
+
+    from l3es_amd import simulate
+
+    out = simulate("TGV3D", N=32, nu=0.000625, t_final=0.1, dt=0.01, ckp_N=16, log_freq=5, ckp_freq=5,
+                   dst_path=str(tmp_path), precision="float64", verbose=False)
+    u, uh, step, _, _ = O.set_up_solver(32, 0.000625, 3, "TGV3D", 0.01, seed=42, warmup=True)
+    for _ in range(11):
+        u, uh, _ = step(u, uh)
+    assert rel(host(out["u"]), u) < 1e-12
+    rows = list(_csv.reader(open(os.path.join(tmp_path, "diagnostics.csv"))))
+    assert rows[0] == ["step", "time", "umax", "ekin", "dt_est", "e_inj"]
+    assert [int(r[0]) for r in rows[1:]] == [0, 5, 10]
+    assert abs(float(rows[-1][3]) - 0.5 * np.mean(np.sum(u * u, axis=0))) < 1e-12
+    ck = np.load(os.path.join(tmp_path, "sim", "u_16_00010.npy"))
+    assert ck.shape == (3, 16, 16, 16) and ck.dtype == np.float64
+    assert rel(ck, O.spectral_filtering(u, 16)) < 1e-12
+    assert np.load(os.path.join(tmp_path, "u_32_00000_burnin.npy")).shape == (3, 32, 32, 32)
