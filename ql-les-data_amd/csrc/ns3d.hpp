@@ -260,9 +260,13 @@ __device__ __forceinline__ float tpexp(float x) { return expf(x); }
 __device__ __forceinline__ double tpexp(double x) { return exp(x); }
 
 // which RK4 stages read / write the accumulator array (explicit viscous term:
-// stages 1-2 write, 2-3 read; integrating factor: stages 0-2 write, 1-3 read)
-template <int SG, bool VIF> struct AccIn { static constexpr bool value = (SG == 2 || SG == 3) || (VIF && SG == 1); };
-template <int SG, bool VIF> struct AccOut { static constexpr bool value = (SG == 1 || SG == 2) || (VIF && SG == 0); };
+// stage 1 writes, stage 3 reads; integrating factor: stages 0-2 write, 1-3 read)
+template <int SG, bool VIF> struct AccIn {
+  static constexpr bool value = VIF ? (SG >= 1 && SG <= 3) : SG == 3;
+};
+template <int SG, bool VIF> struct AccOut {
+  static constexpr bool value = VIF ? (SG >= 0 && SG <= 2) : SG == 1;
+};
 
 // SG: -1 = load the state and inverse-transform only, 0..3 = RK4 stage,
 // 4 = write du (rhs); VIF: integrating-factor viscous term.  Both are
@@ -405,21 +409,24 @@ k_ns3_x(NS3X<T> a) {
         if constexpr (SG == 4) {
           out = du[cc];
         } else if constexpr (!VIF) {
-          // spectral_solver.py:74-80.  The stage-0 accumulator u_n + dt/6 k1
-          // equals u_n + (u_1 - u_n)/3, so it is formed at stage 1 from the two
-          // arrays that stage reads anyway instead of being stored at stage 0
-          // (the rounded 1/3 then scales only the increment, not u_n).
+          // spectral_solver.py:74-80 with the accumulator stored once.  The
+          // stage-0 accumulator u_n + dt/6 k1 equals u_n + (u_1 - u_n)/3 and
+          // the stage-2 increment dt/3 k3 equals (u_3 - u_n)/3, so stage 1
+          // stores D = acc_2 - u_n/3 and stage 3 forms D + u_3/3 + dt/6 k4:
+          // stages 0 and 2 neither read nor write an accumulator.  The same
+          // rounded 1/3 multiplies u_n and u_3, so its rounding error scales
+          // only the increment u_3 - u_n, never u_n itself.
+          constexpr T third = (T)(1.0 / 3.0);
           if constexpr (SG == 0) {
             out = cadd(u3[cc], cscale(du[cc], c.cb));
           } else if constexpr (SG == 1) {
-            const cx<T> acc1 = cadd(n3[cc], cscale(csub(u3[cc], n3[cc]), (T)(1.0 / 3.0)));
-            acc_new = cadd(acc1, cscale(du[cc], c.ca));
+            const cx<T> acc1 = cadd(n3[cc], cscale(csub(u3[cc], n3[cc]), third));
+            acc_new = csub(cadd(acc1, cscale(du[cc], c.ca)), cscale(n3[cc], third));
             out = cadd(n3[cc], cscale(du[cc], c.cb));
           } else if constexpr (SG == 2) {
-            acc_new = cadd(c3[cc], cscale(du[cc], c.ca));
             out = cadd(n3[cc], cscale(du[cc], c.cb));
           } else {
-            out = cadd(c3[cc], cscale(du[cc], c.ca));
+            out = cadd(c3[cc], cadd(cscale(u3[cc], third), cscale(du[cc], c.ca)));
           }
         } else {
           const T dt = c.dt;
