@@ -700,14 +700,16 @@ __device__ __forceinline__ void c2r_prep(cx<T>* v, const cx<T>* pb, const cx<T>*
 }
 
 // r2c post-processing from the M-point forward FFT held in LDS (pb[0..M)):
-// X_k = (Z_k + Z*_{M-k})/2 + W_N^k (Z_k - Z*_{M-k})/(2i), k < KZD
+// X_k = h (Z_k + Z*_{M-k}) + W_N^k h (Z_k - Z*_{M-k})/i, k < KZD, where h is
+// 1/2 times whatever scale the caller folds into the output
 template <typename T, int M, int PAD = 0>
-__device__ __forceinline__ void r2c_post_store(cx<T>* dst, const cx<T>* pb, const cx<T>* s_tw, int j, int TP, int KZD) {
+__device__ __forceinline__ void r2c_post_store(cx<T>* dst, const cx<T>* pb, const cx<T>* s_tw, int j, int TP, int KZD,
+                                               T h) {
   for (int k = j; k < KZD; k += TP) {
     const cx<T> zk = pb[lds_at<PAD>(k)];
     const cx<T> zr = cconj(pb[lds_at<PAD>(k == 0 ? 0 : M - k)]);
-    const cx<T> e = cscale(cadd(zk, zr), (T)0.5);
-    const cx<T> o = crot<T, true>(cscale(csub(zk, zr), (T)0.5));
+    const cx<T> e = cscale(cadd(zk, zr), h);
+    const cx<T> o = crot<T, true>(cscale(csub(zk, zr), h));
     st_stream<QL_NT_Z && NTBig<T, 2 * M>::value>(&dst[k], cadd(e, cmul(s_tw[k], o)));
   }
 }
@@ -787,6 +789,7 @@ struct ZState {
     }
     if (ln < PWV) s_buf[(wrow + ln) * C::LSR + lds_at<ZP>(M)] = src[NEM];
   }
+  static __device__ __forceinline__ T inv_n3() { return (T)(1.0 / ((double)N * N * N)); }
   template <int F>
   __device__ __forceinline__ cx<T>* slot() {
     return pf[F % DEPTH];
@@ -803,15 +806,14 @@ struct ZState {
     c2r_prep<T, M, CI, ZP>(v, pb, s_tw, j);
     lds_sync<SYNC_WAVE>();
     CI::run(v, pb, 1, j, s_twi, 0);
-    const T scale = (T)1 / ((T)N * (T)N * (T)N);
-#pragma unroll
-    for (int s = 0; s < NO; ++s) v[s] = cscale(v[s], scale);
-    // (re, im) hold the (even, odd) z samples; u x omega accumulated per field
+    // (re, im) hold the (even, odd) z samples, still without the inverse
+    // transform's 1/N^3, which the r2c post-processing applies to u x omega
+    // (and the stores of u and omega apply to themselves)
     if constexpr (F >= 3) {
       if (a.w_out != nullptr) {
         cx<T>* dst = reinterpret_cast<cx<T>*>(a.w_out + ((size_t)(F - 3) * N * N + row) * N);
 #pragma unroll
-        for (int s = 0; s < NO; ++s) dst[CI::out_index(j, s)] = v[s];
+        for (int s = 0; s < NO; ++s) dst[CI::out_index(j, s)] = cscale(v[s], inv_n3());
       }
     }
     if constexpr (F < 3) {
@@ -878,13 +880,18 @@ __device__ __forceinline__ void ns3_z_body(const NS3Z<T>& a, cx<T>* s_tw, cx<T>*
   if constexpr (St::DEPTH >= 3) z.fetch(f0 + 2, z.template slot<f0 + 2>(), z.row0);
   z.template fields<f0>();
   if (a.u_out != nullptr) {
+    const T su = UIN ? (T)1 : St::inv_n3();
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
       cx<T>* dst = reinterpret_cast<cx<T>*>(a.u_out + ((size_t)f * N * N + z.row) * N);
 #pragma unroll
-      for (int s = 0; s < NO; ++s) dst[CI::out_index(z.j, s)] = z.uu[f][s];
+      for (int s = 0; s < NO; ++s) dst[CI::out_index(z.j, s)] = cscale(z.uu[f][s], su);
     }
   }
+  // 1/2 of the r2c post-processing times the 1/N^3 of each inverse-transformed
+  // factor of u x omega (u given in physical space carries its own)
+  const double n3 = (double)N * N * N;
+  const T h = (T)(UIN ? 0.5 / n3 : 0.5 / (n3 * n3));
   cx<T>* pb = s_buf + z.p * ZCfg<N, V>::LSR;
 #pragma unroll
   for (int cc = 0; cc < 3; ++cc) {
@@ -897,7 +904,7 @@ __device__ __forceinline__ void ns3_z_body(const NS3Z<T>& a, cx<T>* s_tw, cx<T>*
     for (int s = 0; s < CF::NOUT; ++s)
       if (CF::out_valid(z.j, s)) pb[lds_at<St::ZP>(CF::out_index(z.j, s))] = v[s];
     lds_sync<SYNC_WAVE>();
-    r2c_post_store<T, M, St::ZP>(a.nz + ((size_t)cc * N * N + z.row) * a.KZP, pb, s_tw, z.j, TP, a.KZD);
+    r2c_post_store<T, M, St::ZP>(a.nz + ((size_t)cc * N * N + z.row) * a.KZP, pb, s_tw, z.j, TP, a.KZD, h);
   }
 }
 
