@@ -544,6 +544,90 @@ def run_dataset(args):
     print(json.dumps(out), flush=True)
 
 
+STATS_CASES = (  # (name, dim, particles, frames): one trajectory of each dataset configuration
+    ("hit32_1", 3, 32 ** 3, 7501),   # hit32_1.yaml: 32^3 particles, t_final / dt + 1 frames
+    ("kolm64_1", 2, 64 ** 2, 18501),  # kolm64_1.yaml: 64^2 particles
+)
+
+
+def run_stats(args):
+    """Dataset statistics (gen_dataset.py:143-238): one pass of the trajectory-moment kernel
+    (qlles_stats_moments: periodic displacements, frame differences and their FP64 moments) over a
+    whole trajectory resident in HBM, at the frame counts and particle numbers of the two dataset
+    configurations.  The reference does this on the host (NumPy + jax_md on CPU)."""
+    import numpy as np
+    import torch
+
+    from l3es_amd import _lib
+    from oracle import dataset as ods
+
+    torch.cuda.set_device(0)
+    lib = _lib.load()
+    L = 2 * np.pi
+    g = torch.Generator(device="cuda").manual_seed(args.seed)
+    res = []
+    for name, dim, n, T in STATS_CASES:
+        r = torch.rand((1, n, dim), device="cuda", generator=g) * L
+        r = torch.remainder(r + torch.cumsum(torch.randn((T, n, dim), device="cuda", generator=g) * 0.01, 0), L)
+        u = torch.randn((T, n, dim), device="cuda", generator=g)
+        st = _lib.Stats(dim, L)
+        out = torch.empty(8 * dim, dtype=torch.float64, device="cuda")
+        stream = torch.cuda.current_stream()
+
+        def call():
+            _lib.check(lib.qlles_stats_moments(st.handle, _lib.ptr(r), _lib.ptr(u), T, n, None, _lib.ptr(out),
+                                               _lib.stream_ptr(stream)))
+
+        for _ in range(max(args.warmup, 1)):
+            call()
+        K = max(1, min(args.steps, 20))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(K):
+            call()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        sec = e0.elapsed_time(e1) / K * 1e-3
+        alg = 2.0 * T * n * dim * 4  # r and u read once
+        cpu = None
+        if not args.no_cpu:
+            Ts = 101  # bounded sample: the first 101 frames
+            rs, us = r[:Ts].cpu().numpy(), u[:Ts].cpu().numpy()
+            ods.trajectory_moments(rs, us, L)
+            c0, reps = time.perf_counter(), 0
+            while time.perf_counter() - c0 < args.cpu_seconds / len(STATS_CASES) or reps == 0:
+                ods.trajectory_moments(rs, us, L)
+                reps += 1
+            cpu = (Ts * n) / ((time.perf_counter() - c0) / reps)  # particle-frames/s
+        row = dict(name=name, dim=dim, particles=n, frames=T, us_per_pass=1e6 * sec,
+                   particle_frames_per_s=T * n / sec, alg_bytes_per_launch=alg, achieved_GBs=alg / sec * 1e-9,
+                   frac_of_peak=alg / sec * 1e-9 / 8000.0, cpu_particle_frames_per_s=cpu,
+                   speedup_vs_cpu=(T * n / sec / cpu) if cpu else None)
+        log(f"[bench] stats {name}: {T} frames x {n} particles in {1e6 * sec:.0f} us per pass "
+            f"({row['achieved_GBs']:.0f} GB/s), cpu {cpu}")
+        res.append(row)
+        st.close()
+        del r, u
+        torch.cuda.empty_cache()
+    h = res[0]
+    out = {
+        "metric": "dataset statistics: particle-frames/s of one trajectory-moment pass (hit32_1 trajectory)",
+        "value": h["particle_frames_per_s"], "unit": "particle-frames/s", "n_gpus": 1, "steps": max(1, min(args.steps, 20)),
+        "warmup": max(args.warmup, 1), "ms_per_step": h["us_per_pass"] * 1e-3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "gen_dataset.py compute_statistics_h5 moments of one trajectory resident in HBM: "
+                               "7501 frames x 32^3 particles (3-D) and 18501 x 64^2 (2-D), FP32 frames, FP64 sums"},
+        "roofline": {"bound": "hbm", "kernel": "k_traj_moments", "achieved": h["achieved_GBs"], "peak": 8000.0,
+                     "unit": "GB/s", "frac": h["frac_of_peak"], "traffic": None},
+        "cpu_baseline": {"value": h["cpu_particle_frames_per_s"], "unit": "particle-frames/s", "cores": 1,
+                         "kind": "port", "sample": "the oracle's trajectory_moments (NumPy, FP32 elements, FP64 sums) "
+                                               "on the first 101 frames of the same trajectory"},
+        "cases": res,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -566,10 +650,11 @@ def main():
     ap.add_argument("--copy-probe-mib", dest="copy_probe", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (counter calibration)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH/WRITE passes")
-    ap.add_argument("--workload", choices=("hit256", "batch", "adjoint", "relax", "dataset"), default="hit256",
+    ap.add_argument("--workload", choices=("hit256", "batch", "adjoint", "relax", "dataset", "stats"),
+                    default="hit256",
                     help="hit256: the headline 256^3 step (default); batch: 24 uneven batched trajectories; "
                          "adjoint: gradient through K checkpointed steps; relax: SPH particle relaxation; "
-                         "dataset: the dataset-generation loops end to end")
+                         "dataset: the dataset-generation loops end to end; stats: dataset statistics moments")
     ap.add_argument("--adj-N", dest="adj_N", type=int, default=128)
     ap.add_argument("--adj-K", dest="adj_K", type=int, default=50)
     ap.add_argument("--adj-every", dest="adj_every", type=int, default=10)
@@ -614,6 +699,9 @@ def main():
         return
     if args.workload == "dataset":
         run_dataset(args)
+        return
+    if args.workload == "stats":
+        run_stats(args)
         return
 
     import torch

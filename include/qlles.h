@@ -312,6 +312,26 @@ int qlles_sph_mls_grid(qlles_sph* sph, const double* r, const double* u_r, int n
  * the nodes m box/n. */
 int qlles_sph_dft_hat(qlles_sph* sph, const double* r, const double* u_r, int np, void* u_hat, void* stream);
 
+/* Dataset statistics (gen_dataset.py:143-239, compute_statistics_h5): the
+ * moments of one trajectory.  r and u are (T, np, dim) FP32, C order, on the
+ * device.  Per element, in FP32 as the reference computes them:
+ *   vel_t = mod(d + box/2, box) - box/2, d = r_{t+1} - r_t   (T-1 frames;
+ *           jax_md's periodic displacement, mod with the divisor's sign)
+ *   acc_t = vel_{t+1} - vel_t                                  (T-2 frames)
+ *   u_t                                                         (T frames)
+ *   au_t  = u_{t+1} - u_t                                      (T-1 frames)
+ * out (device, 8*dim FP64) receives, for the quantities q = vel, acc, u, au in
+ * that order, the sums of the values (out[q*dim + c]) and then the sums of
+ * their squared deviations from center (out[(4 + q)*dim + c]); center
+ * (device, 4*dim FP64, same order) may be NULL for zeros.  Sums are FP64 in a
+ * fixed order, so results are bitwise reproducible.  A context holds the
+ * per-workgroup partial sums and is used from one stream at a time. */
+typedef struct qlles_stats qlles_stats;
+int qlles_stats_create(int dim, double box, qlles_stats** out);
+int qlles_stats_destroy(qlles_stats* st);
+int qlles_stats_moments(qlles_stats* st, const float* r, const float* u, long long T, long long np,
+                        const double* center, double* out, void* stream);
+
 /* message of the last failed call on this thread */
 const char* qlles_last_error(void);
 

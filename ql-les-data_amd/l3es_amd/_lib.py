@@ -37,6 +37,7 @@ EXPORTS = (
     "qlles_plan_tune", "qlles_energy_spectrum", "qlles_sph_create", "qlles_sph_destroy", "qlles_sph_density",
     "qlles_sph_relax_acc", "qlles_sph_forces", "qlles_sph_relax_advect", "qlles_sph_mls_grid", "qlles_sph_dft_hat",
     "qlles_step_record_bytes", "qlles_step_record", "qlles_step_vjp_record",
+    "qlles_stats_create", "qlles_stats_destroy", "qlles_stats_moments",
 )
 KINDS = ("x-pass", "y-inverse", "z-pass", "y-forward", "other")
 
@@ -121,6 +122,9 @@ def load():
         "qlles_step_record_bytes": (ctypes.c_size_t, [vp]),
         "qlles_step_record": (ci, [vp, vp, vp, vp, vp]),
         "qlles_step_vjp_record": (ci, [vp, vp, vp, vp, vp, vp, vp]),
+        "qlles_stats_create": (ci, [ci, cd, ctypes.POINTER(vp)]),
+        "qlles_stats_destroy": (ci, [vp]),
+        "qlles_stats_moments": (ci, [vp, vp, vp, ctypes.c_longlong, ctypes.c_longlong, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -212,6 +216,28 @@ class Sph:
     def close(self):
         if getattr(self, "handle", None) is not None and _lib is not None:
             _lib.qlles_sph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Stats:
+    """Owner of a qlles_stats (trajectory moments for the dataset statistics)."""
+
+    def __init__(self, dim, box):
+        lib = load()
+        h = ctypes.c_void_p()
+        check(lib.qlles_stats_create(dim, float(box), ctypes.byref(h)))
+        self.handle = h
+        self.dim = dim
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and _lib is not None:
+            _lib.qlles_stats_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
