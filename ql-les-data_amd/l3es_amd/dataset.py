@@ -139,6 +139,25 @@ def compute_statistics(dst_dir, stats_every_nth=1, device=None, keep_resident=Tr
     dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
     lib = _lib.load()
     st = _lib.Stats(dim, float(box[0]))
+    try:
+        means, std = _two_loops(lib, st, dst_dir, stats_every_nth, dim, dev, keep_resident)
+    finally:
+        st.close()
+    for i, name in enumerate(("vel", "acc", "u", "au")):
+        metadata[f"{name}_mean"] = means[i].tolist()
+        metadata[f"{name}_std"] = std[i].tolist()
+    if not (metadata["write_every"] == 1 or stats_every_nth == 1):  # :229-232
+        raise AssertionError("Either keep the dataset without slicing and have sliced stats files, "
+                             "or slice the dataset and have unsliced stats.")
+    metadata["write_every"] = metadata["write_every"] * stats_every_nth
+    with open(os.path.join(dst_dir, f"metadata_every{stats_every_nth}.json"), "w") as fh:
+        json.dump(metadata, fh)
+    return metadata
+
+
+def _two_loops(lib, st, dst_dir, every, dim, dev, keep_resident):
+    """The mean and std loops of gen_dataset.py:168-211 over the train split;
+    returns (means, stds), each (4, dim) for vel, acc, u, au."""
     out = torch.empty(8 * dim, dtype=torch.float64, device=dev)
     free = torch.cuda.mem_get_info(dev)[0]
     resident, held = [], 0
@@ -147,7 +166,7 @@ def compute_statistics(dst_dir, stats_every_nth=1, device=None, keep_resident=Tr
         if resident:
             yield from resident
             return
-        for r, u in _train_trajectories(dst_dir, stats_every_nth):
+        for r, u in _train_trajectories(dst_dir, every):
             pair = (torch.from_numpy(r).to(dev), torch.from_numpy(u).to(dev))
             yield pair
 
@@ -179,18 +198,7 @@ def compute_statistics(dst_dir, stats_every_nth=1, device=None, keep_resident=Tr
     center[2:] = means[2:]
     c_dev = torch.from_numpy(center.reshape(-1)).to(dev)
     sq = np.stack([moments(r, u, c_dev)[1] for r, u in trajectories()]).mean(0) ** 0.5
-    std = np.where(sq < 1e-7, 1, sq)  # :215-223
-    for i, name in enumerate(("vel", "acc", "u", "au")):
-        metadata[f"{name}_mean"] = means[i].tolist()
-        metadata[f"{name}_std"] = std[i].tolist()
-    if not (metadata["write_every"] == 1 or stats_every_nth == 1):  # :229-232
-        raise AssertionError("Either keep the dataset without slicing and have sliced stats files, "
-                             "or slice the dataset and have unsliced stats.")
-    metadata["write_every"] = metadata["write_every"] * stats_every_nth
-    with open(os.path.join(dst_dir, f"metadata_every{stats_every_nth}.json"), "w") as fh:
-        json.dump(metadata, fh)
-    st.close()
-    return metadata
+    return means, np.where(sq < 1e-7, 1, sq)  # :215-223
 
 
 def main(argv=None):
