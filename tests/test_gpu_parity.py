@@ -137,19 +137,47 @@ def test_step_hit_unforced(gpu, N, prec):
     assert rel(uhg, uho) < 5 * tol(prec)
 
 
+def kat(name):
+    import json
+    import os
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "reference_kat.json")) as fh:
+        return json.load(fh)[name]
+
+
 def test_tgv_known_answer_gpu(gpu):
     """KAT (from_spectralDNS/main_numpy.py:136) on the HIP path in FP64."""
-    ug, _, _, _ = _run_gpu(64, 0.000625, 3, "TGV3D", 0.01, 10, "float64", forcing_type="none")
+    k = kat("tgv3d_numpy")
+    ug, _, _, _ = _run_gpu(k["N"], k["nu"], 3, k["case"], k["dt"], k["steps"], "float64", forcing_type="none")
     e = 0.5 * np.mean(np.sum(ug * ug, axis=0))
-    assert abs(e - 0.124953117517) < 5e-8
-    uo, _, _, _ = _run_oracle(64, 0.000625, 3, "TGV3D", 0.01, 10)
+    assert round(e - k["e_kin"], k["round_digits"]) == 0
+    uo, _, _, _ = _run_oracle(k["N"], k["nu"], 3, k["case"], k["dt"], k["steps"])
     assert rel(ug, uo) < 1e-12
 
 
 def test_tgv_fp32_energy(gpu):
-    ug, _, _, _ = _run_gpu(64, 0.000625, 3, "TGV3D", 0.01, 10, "float32", forcing_type="none")
+    k = kat("tgv3d_numpy")
+    ug, _, _, _ = _run_gpu(k["N"], k["nu"], 3, k["case"], k["dt"], k["steps"], "float32", forcing_type="none")
     e = 0.5 * np.mean(np.sum(ug.astype(np.float64) ** 2, axis=0))
-    assert abs(e - 0.124953117517) / 0.125 < 1e-6
+    assert abs(e - k["e_kin"]) / k["e_kin"] < 1e-6
+
+
+@pytest.mark.parametrize("prec", ["float64", "float32"])
+def test_simulate_validation_config(gpu, prec):
+    """configs/tgv3d_validate.yaml through the simulation driver: the reference asserts the
+    kinetic energy after its 12 steps to two decimals (spectral_solver.py:363-367); the FP64
+    run also matches the oracle's 12 steps."""
+    from l3es_amd import simulate
+
+    k = kat("tgv3d_validate")
+    out = simulate(k["case"], N=k["N"], nu=k["nu"], t_final=k["t_final"], dt=k["dt"], log_freq=5,
+                   precision=prec, write_checkpoints=False, verbose=False)
+    assert round(out["e_kin"] - k["e_kin"], k["round_digits"]) == 0
+    if prec == "float64":
+        u, uh, step, _, _ = O.set_up_solver(k["N"], k["nu"], 3, k["case"], k["dt"], warmup=False)
+        for _ in range(k["steps"]):
+            u, uh, _ = step(u, uh)
+        assert rel(host(out["u"]), u) < 1e-12
 
 
 @pytest.mark.parametrize("prec", ["float64", "float32"])

@@ -13,12 +13,21 @@ def rk4_poly(z):
     return 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24
 
 
+def kat(name):
+    import json
+    import os
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "reference_kat.json")) as fh:
+        return json.load(fh)[name]
+
+
 def test_tgv_known_answer():
     """from_spectralDNS/main_numpy.py:126-137: TGV 64^3, nu=6.25e-4, dt=0.01, 10 steps."""
-    u, uh, step, _, _ = O.set_up_solver(64, 0.000625, 3, "TGV3D", 0.01, warmup=False)
-    for _ in range(10):
+    k = kat("tgv3d_numpy")
+    u, uh, step, _, _ = O.set_up_solver(k["N"], k["nu"], 3, k["case"], k["dt"], warmup=False)
+    for _ in range(k["steps"]):
         u, uh, _ = step(u, uh)
-    assert round(O.kinetic_energy(u) - 0.124953117517, 7) == 0
+    assert round(O.kinetic_energy(u) - k["e_kin"], k["round_digits"]) == 0
 
 
 def _node_grid(N, dim=3):
