@@ -8,7 +8,7 @@ path (``ql-les-data_amd/l3es_amd``) never imports it.
 The oracle is a NumPy restatement of the reference algorithm
 (tumaer/ql-les-data, package ``l3es``), written from the mathematics and the
 reference's documented behaviour; each function cites the reference file:line
-it follows.  Functions that exist only for north-star modes (fixed-power HIT
+it follows.  Functions that exist only for modes this library adds (fixed-power HIT
 forcing, Kolmogorov forcing + drag, integrating-factor viscous term, the
 ``k^4 exp(-2(k/k0)^2)`` Rogallo spectrum) have no reference counterpart and are
 pinned by closed forms and invariants in ``tests/test_oracle.py``.

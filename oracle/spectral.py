@@ -72,7 +72,7 @@ def wavenumbers(N, dim):
 def rhs_wrapper(N, nu, fft_axes=(1, 2, 3), forcing=None, fft=None, viscous=True):
     """Rotational-form RHS (spectral_solver.py:27-62).
 
-    ``forcing`` (north-star modes, no reference counterpart) is a callable
+    ``forcing`` (modes this library adds, no reference counterpart) is a callable
     ``f(u_hat) -> f_hat`` added after the projection; ``viscous=False`` drops
     the explicit -nu k^2 u_hat term (used by the integrating-factor scheme).
     """
@@ -126,7 +126,7 @@ def rk4_wrapper(dt, rhs_fn, forcing_mask=None, e_kin_init=None, forcing_type="no
     """Classical RK4 (spectral_solver.py:65-120) with the ekin_tot rescaling
     forcing (:82-112).
 
-    ``viscous_mode="IF"`` (north-star, no reference counterpart) integrates
+    ``viscous_mode="IF"`` (an addition of this library, no reference counterpart) integrates
     the viscous term exactly with the factor exp(-nu k^2 h); then ``nl_fn`` is
     an RHS without the viscous term.
     """
@@ -217,7 +217,7 @@ def init_u_tgv2d(Nx, target_dim=2, rescale=1.0):
 def _spectrum_table(N, spectrum="k53", k0=4.0):
     """Per-mode energy E(k)/(4 pi k^2) on integer shells (init_fields.py:37-41).
 
-    ``spectrum="k4exp"`` is the north-star E(k) = k^4 exp(-2 (k/k0)^2).
+    ``spectrum="k4exp"`` is the added E(k) = k^4 exp(-2 (k/k0)^2).
     """
     nmax = round(np.sqrt(3 * N ** 2) / 2)
     E_k = np.zeros(nmax + 1) + EPS
@@ -331,7 +331,7 @@ def init_forcing_mask_hit(N, kf=3):
 
 
 def init_forcing_mask_2d(N, kf):
-    """2-D analogue of the HIT shell mask (north-star helper)."""
+    """2-D analogue of the HIT shell mask (helper of an added mode)."""
     k1 = np.fft.fftfreq(N, 1.0 / N)
     k2 = k1[: N // 2 + 1].copy()
     k2[-1] = -k2[-1]
@@ -341,7 +341,7 @@ def init_forcing_mask_2d(N, kf):
 
 
 def init_u_kolm_np(N, seed=42, k0=4.0, max_velocity=4.2):
-    """Project-defined seeded 2-D random field (north-star C2).
+    """Project-defined seeded 2-D random field (the 256^2 Kolmogorov configuration).
 
     The reference's init_u_kolm (init_fields.py:191-212) uses jax_cfd and the
     JAX PRNG, which cannot be reproduced here; this generator instead draws a
@@ -371,7 +371,7 @@ def init_u_kolm_np(N, seed=42, k0=4.0, max_velocity=4.2):
 
 
 def kolmogorov_forcing_hat(N, amp=1.0, kf=4):
-    """rfft2 of f_x = amp sin(kf y) on the node grid (north-star C2)."""
+    """rfft2 of f_x = amp sin(kf y) on the node grid (the 256^2 Kolmogorov configuration)."""
     y = np.arange(N) * 2 * np.pi / N
     f = np.zeros((3, N, N))
     f[0] = amp * np.sin(kf * y)[None, :]
@@ -380,7 +380,7 @@ def kolmogorov_forcing_hat(N, amp=1.0, kf=4):
 
 def make_forcing(kind, N, dim, u_hat_shape=None, kf=2, power=0.1, kolm_amp=1.0,
                  kolm_k=4, drag=0.0):
-    """North-star RHS forcing terms (no reference counterpart).
+    """Added RHS forcing terms (no reference counterpart).
 
     ``power``: f_hat = P / (2 E_f) u_hat on 0 < |k| <= kf, with E_f the
     Parseval energy of those modes, so <f.u> = P exactly.
@@ -557,7 +557,7 @@ def pos_init_cartesian(n, dim, L=2 * np.pi):
 
 
 def pos_init_jittered(n, dim, L=2 * np.pi, jitter=0.25, seed=0):
-    """Cell-centred lattice plus uniform jitter of +-jitter*dx (north-star C2)."""
+    """Cell-centred lattice plus uniform jitter of +-jitter*dx (the 256^2 Kolmogorov configuration)."""
     rng = np.random.default_rng(seed)
     r = pos_init_cartesian(n, dim, L)
     r = r + (rng.random(r.shape) * 2 - 1) * jitter * (L / n)

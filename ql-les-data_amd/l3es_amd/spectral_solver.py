@@ -9,7 +9,7 @@ Same factories and return values as the reference:
   set_up_solver(...) -> (u, u_hat, xyz_vis, dx, integrate_fn, L, fft_axes, e_kin_init, e_inj)  (:156-230)
   simulate(...)                                                          (:233-367)
 Arrays are torch tensors on the current HIP device; the arithmetic runs in
-libqlles.so.  Extra keyword arguments select the north-star modes:
+libqlles.so.  Extra keyword arguments select the modes this library adds:
 ``precision`` ("float64" | "float32"), ``viscous_mode`` ("explicit" | "IF"),
 ``forcing_type`` "power" / "kolmogorov", ``spectrum`` ("k53" | "k4exp").
 """
