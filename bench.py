@@ -544,6 +544,40 @@ def run_dataset(args):
     print(json.dumps(out), flush=True)
 
 
+def run_horizon(args):
+    """The FP32-vs-FP64 kinetic-energy trajectory of the headline configuration over a long horizon:
+    both precisions advance the same initial field (256^3 power-forced HIT) and record the FP64 Parseval
+    energy every --every steps, --samples times (untimed apart from the wall clock of each run)."""
+    import numpy as np
+    import torch
+
+    torch.cuda.set_device(0)
+    runs = {}
+    for prec in ("float32", "float64"):
+        t0 = time.perf_counter()
+        ke, final = energy_trajectory(args, prec, nsamp=args.samples, every=args.every)
+        torch.cuda.synchronize()
+        runs[prec] = (np.array(ke), final.cpu().numpy(), time.perf_counter() - t0)
+        log(f"[bench] horizon {prec}: {args.samples * args.every} steps in {runs[prec][2]:.1f}s")
+    k32, k64 = runs["float32"][0], runs["float64"][0]
+    rel = np.abs(k32 - k64) / np.abs(k64)
+    f32, f64 = runs["float32"][1], runs["float64"][1]
+    field = float(np.linalg.norm(f32 - f64) / np.linalg.norm(f64))
+    steps = args.samples * args.every
+    out = {
+        "metric": f"max relative FP32-vs-FP64 kinetic-energy error over a {steps}-step trajectory",
+        "value": float(rel.max()), "unit": "relative error", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": 1e3 * runs["float32"][2] / steps, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"3-D forced HIT {args.N}^3 (power {args.power} on 0<|k|<={args.kf}, nu {args.nu}, "
+                               f"dt {args.dt}, Rogallo k^4 exp(-2(k/4)^2) E0={args.e_kin}, seed {args.seed}), "
+                               "FP32 against FP64 from the same initial field; energies are FP64 Parseval sums"},
+        "every": args.every, "ke_fp32": k32.tolist(), "ke_fp64": k64.tolist(), "ke_rel_err": rel.tolist(),
+        "final_field_rel_l2_err": field, "wall_s": {p: runs[p][2] for p in runs},
+    }
+    print(json.dumps(out), flush=True)
+
+
 STATS_CASES = (  # (name, dim, particles, frames): one trajectory of each dataset configuration
     ("hit32_1", 3, 32 ** 3, 7501),   # hit32_1.yaml: 32^3 particles, t_final / dt + 1 frames
     ("kolm64_1", 2, 64 ** 2, 18501),  # kolm64_1.yaml: 64^2 particles
@@ -650,11 +684,14 @@ def main():
     ap.add_argument("--copy-probe-mib", dest="copy_probe", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (counter calibration)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH/WRITE passes")
-    ap.add_argument("--workload", choices=("hit256", "batch", "adjoint", "relax", "dataset", "stats"),
+    ap.add_argument("--workload", choices=("hit256", "batch", "adjoint", "relax", "dataset", "stats", "horizon"),
                     default="hit256",
                     help="hit256: the headline 256^3 step (default); batch: 24 uneven batched trajectories; "
                          "adjoint: gradient through K checkpointed steps; relax: SPH particle relaxation; "
-                         "dataset: the dataset-generation loops end to end; stats: dataset statistics moments")
+                         "dataset: the dataset-generation loops end to end; stats: dataset statistics moments; "
+                         "horizon: FP32-vs-FP64 energy trajectory over --samples x --every steps")
+    ap.add_argument("--samples", type=int, default=20)
+    ap.add_argument("--every", type=int, default=100)
     ap.add_argument("--adj-N", dest="adj_N", type=int, default=128)
     ap.add_argument("--adj-K", dest="adj_K", type=int, default=50)
     ap.add_argument("--adj-every", dest="adj_every", type=int, default=10)
@@ -702,6 +739,9 @@ def main():
         return
     if args.workload == "stats":
         run_stats(args)
+        return
+    if args.workload == "horizon":
+        run_horizon(args)
         return
 
     import torch
