@@ -149,10 +149,10 @@ class EngineT final : public Engine {
       const double nyv = (double)ncomp * N * (DIM == 3 ? KYD : 1) * KZD * 2.0 * sizeof(T);
       b += nyv;                                               // N^ (dealiased modes only)
       b += ncomp * cc;                                        // u_s
-      // the 3-D explicit scheme writes its accumulator at stage 1 only and
-      // reads it at stage 3 only
+      // the 3-D explicit scheme writes its accumulator (an increment) at
+      // stage 1 only and reads it, with u_n, at stage 3 only
       const bool full = DIM == 2 || c.visc_if != 0;
-      if (c.stage == 1 || c.stage == 2) b += ncomp * cc;  // u_n read
+      if (c.stage == 1 || c.stage == 2 || (!full && c.stage == 3)) b += ncomp * cc;  // u_n read
       if (full ? (c.stage >= 1 && c.stage <= 3) : c.stage == 3) b += ncomp * cc;  // acc read
       if (full ? (c.stage >= 0 && c.stage <= 2) : c.stage == 1) b += ncomp * cc;  // acc write
       b += ncomp * cc;                                        // u_{s+1} / du write

@@ -379,7 +379,7 @@ k_ns3_x(NS3X<T> a) {
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc) {
         u3[cc] = ld_stream<NTL>(&p_us[l_us.at(cc, row)]);
-        if constexpr (SG == 1 || SG == 2) n3[cc] = ld_stream<NTL>(&p_un[l_un.at(cc, row)]);
+        if constexpr (SG == 1 || SG == 2 || (!VIF && SG == 3)) n3[cc] = ld_stream<NTL>(&p_un[l_un.at(cc, row)]);
         if constexpr (AccIn<SG, VIF>::value) c3[cc] = ld_stream<NTL>(&p_acc[l_acc.at(cc, row)]);
       }
       const cx<T> n0 = keep_x ? w[0][s] : zero;
@@ -409,24 +409,25 @@ k_ns3_x(NS3X<T> a) {
         if constexpr (SG == 4) {
           out = du[cc];
         } else if constexpr (!VIF) {
-          // spectral_solver.py:74-80 with the accumulator stored once.  The
-          // stage-0 accumulator u_n + dt/6 k1 equals u_n + (u_1 - u_n)/3 and
-          // the stage-2 increment dt/3 k3 equals (u_3 - u_n)/3, so stage 1
-          // stores D = acc_2 - u_n/3 and stage 3 forms D + u_3/3 + dt/6 k4:
-          // stages 0 and 2 neither read nor write an accumulator.  The same
-          // rounded 1/3 multiplies u_n and u_3, so its rounding error scales
-          // only the increment u_3 - u_n, never u_n itself.
+          // spectral_solver.py:74-80 with the accumulator stored once, as an
+          // increment.  The reference's accumulator is u_n plus
+          // dt/6 k1 + dt/3 k2 + dt/3 k3, and dt/6 k1 = (u_1 - u_n)/3,
+          // dt/3 k3 = (u_3 - u_n)/3.  So stage 1 stores only
+          // G = (u_1 - u_n)/3 + dt/3 k2, stage 2 neither reads nor writes an
+          // accumulator, and stage 3 forms
+          // u_{n+1} = u_n + (G + (u_3 - u_n)/3 + dt/6 k4).  The increments
+          // are summed at their own scale and reach u_n in one rounding.
           constexpr T third = (T)(1.0 / 3.0);
           if constexpr (SG == 0) {
             out = cadd(u3[cc], cscale(du[cc], c.cb));
           } else if constexpr (SG == 1) {
-            const cx<T> acc1 = cadd(n3[cc], cscale(csub(u3[cc], n3[cc]), third));
-            acc_new = csub(cadd(acc1, cscale(du[cc], c.ca)), cscale(n3[cc], third));
+            acc_new = cadd(cscale(csub(u3[cc], n3[cc]), third), cscale(du[cc], c.ca));
             out = cadd(n3[cc], cscale(du[cc], c.cb));
           } else if constexpr (SG == 2) {
             out = cadd(n3[cc], cscale(du[cc], c.cb));
           } else {
-            out = cadd(c3[cc], cadd(cscale(u3[cc], third), cscale(du[cc], c.ca)));
+            const cx<T> inc = cadd(cadd(c3[cc], cscale(csub(u3[cc], n3[cc]), third)), cscale(du[cc], c.ca));
+            out = cadd(n3[cc], inc);
           }
         } else {
           const T dt = c.dt;
