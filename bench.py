@@ -578,6 +578,84 @@ def run_horizon(args):
     print(json.dumps(out), flush=True)
 
 
+def _time_steps(dim, N, n, precision, steps, warmup, backend="nufft", **kw):
+    """steps/s of the combined step (RK4 + filter + interpolation + advection of n^dim particles) and the
+    state after warm-up + steps; kw goes to set_up_solver."""
+    import torch
+
+    from l3es_amd import _lib, set_up_solver
+    from l3es_amd.integrator import pos_init_cartesian, spectral_interpolator_wrapper
+
+    _, uh, _, _, step, _, _, _, _ = set_up_solver(N, kw.pop("nu"), dim, kw.pop("case"), kw.pop("dt"), kw.pop("seed"),
+                                                  n, kw.pop("kf", None), precision=precision, warmup=False, **kw)
+    it = spectral_interpolator_wrapper(n, tuple(range(1, dim + 1)), backend=backend, precision=precision)
+    r = torch.from_numpy(pos_init_cartesian(n, dim)).cuda()
+    ur = torch.empty((r.shape[0], dim), dtype=step.dtype, device=r.device)
+    lib = _lib.load()
+
+    def advance(k):
+        _lib.check(lib.qlles_step_advect(step.plan.handle, it.it.handle, None, _lib.ptr(uh), int(k), _lib.ptr(r),
+                                         r.shape[0], _lib.ptr(ur), None, _lib.stream_ptr()))
+
+    advance(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    advance(steps)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    return steps / sec, uh.to(torch.complex128), r.clone()
+
+
+def run_configs(args):
+    """The two small configurations beside the headline one, FP32 and FP64 side by side:
+    TGV 64^3 (nu 1/1600; the dt = 1e-2 x 10-step known answer and dt = 1e-3 timing) and 2-D Kolmogorov
+    256^2 (sin 4y forcing, drag 0.1, nu 1e-3, dt 1e-3, 2000 steps, 64^2 NUFFT particles on the 64^2
+    filter).  Kinetic energies are FP64 Parseval sums; errors are FP32 against FP64 on the same seed."""
+    import numpy as np
+    import torch
+
+    from l3es_amd.utils import kinetic_energy_hat
+
+    torch.cuda.set_device(0)
+    cases = []
+    # TGV 64^3: the known answer (spectral_solver.py:363-367, main_numpy.py:136) and a timed run
+    tgv = dict(case="TGV3D", nu=0.000625, seed=42, forcing_type="none")
+    for prec in ("float64", "float32"):
+        _, uh, _ = _time_steps(3, 64, 32, prec, 10, 0, dt=0.01, **dict(tgv))
+        e_kat = kinetic_energy_hat(uh)[0]
+        sps, uh, _ = _time_steps(3, 64, 32, prec, 1000, 10, dt=0.001, **dict(tgv))
+        cases.append(dict(name="tgv64", precision=prec, steps=1000, steps_per_s=sps,
+                          e_kin_kat=e_kat, kat_err=abs(e_kat - 0.124953117517), e_kin=kinetic_energy_hat(uh)[0]))
+        log(f"[bench] configs tgv64 {prec}: {sps:.0f} steps/s, KAT |dE| {abs(e_kat - 0.124953117517):.1e}")
+    # Kolmogorov 256^2 with particles
+    kol = dict(case="Kolm", nu=1e-3, dt=1e-3, seed=args.seed, forcing_type="kolmogorov", drag=0.1, kolm_amp=1.0,
+               kolm_k=4)
+    K = 2000
+    res = {}
+    for prec in ("float64", "float32"):
+        sps, uh, r = _time_steps(2, 256, 64, prec, K, 0, **dict(kol))
+        res[prec] = (kinetic_energy_hat(uh)[0], uh, r)
+        cases.append(dict(name="kolm256", precision=prec, steps=K, steps_per_s=sps, e_kin=res[prec][0]))
+        log(f"[bench] configs kolm256 {prec}: {sps:.0f} steps/s, E={res[prec][0]:.9f}")
+    e64, e32 = res["float64"][0], res["float32"][0]
+    f64, f32 = res["float64"][1], res["float32"][1]
+    d = torch.remainder(res["float32"][2] - res["float64"][2] + np.pi, 2 * np.pi) - np.pi
+    kol_err = dict(ke_rel_err=abs(e32 - e64) / abs(e64),
+                   field_rel_l2_err=float(torch.linalg.vector_norm(f32 - f64) / torch.linalg.vector_norm(f64)),
+                   particle_max_abs_err=float(d.abs().max()))
+    k32 = [c for c in cases if c["name"] == "kolm256" and c["precision"] == "float32"][0]
+    out = {
+        "metric": "RK4 steps/s with particle advection, 2-D Kolmogorov 256^2 FP32 (and TGV 64^3)",
+        "value": k32["steps_per_s"], "unit": "steps/s", "n_gpus": 1, "steps": K, "warmup": 0,
+        "ms_per_step": 1e3 / k32["steps_per_s"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "2-D Kolmogorov 256^2 (nu 1e-3, sin 4y forcing, drag 0.1, dt 1e-3, 2000 steps, "
+                               "64^2 NUFFT particles) and TGV 64^3 (nu 1/1600, dt 1e-3; known answer at dt 1e-2)"},
+        "cases": cases, "kolm256_fp32_vs_fp64": kol_err,
+    }
+    print(json.dumps(out), flush=True)
+
+
 STATS_CASES = (  # (name, dim, particles, frames): one trajectory of each dataset configuration
     ("hit32_1", 3, 32 ** 3, 7501),   # hit32_1.yaml: 32^3 particles, t_final / dt + 1 frames
     ("kolm64_1", 2, 64 ** 2, 18501),  # kolm64_1.yaml: 64^2 particles
@@ -684,12 +762,14 @@ def main():
     ap.add_argument("--copy-probe-mib", dest="copy_probe", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (counter calibration)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH/WRITE passes")
-    ap.add_argument("--workload", choices=("hit256", "batch", "adjoint", "relax", "dataset", "stats", "horizon"),
+    ap.add_argument("--workload", choices=("hit256", "batch", "adjoint", "relax", "dataset", "stats", "horizon",
+                                           "configs"),
                     default="hit256",
                     help="hit256: the headline 256^3 step (default); batch: 24 uneven batched trajectories; "
                          "adjoint: gradient through K checkpointed steps; relax: SPH particle relaxation; "
                          "dataset: the dataset-generation loops end to end; stats: dataset statistics moments; "
-                         "horizon: FP32-vs-FP64 energy trajectory over --samples x --every steps")
+                         "horizon: FP32-vs-FP64 energy trajectory over --samples x --every steps; "
+                         "configs: TGV 64^3 and 2-D Kolmogorov 256^2")
     ap.add_argument("--samples", type=int, default=20)
     ap.add_argument("--every", type=int, default=100)
     ap.add_argument("--adj-N", dest="adj_N", type=int, default=128)
@@ -742,6 +822,9 @@ def main():
         return
     if args.workload == "horizon":
         run_horizon(args)
+        return
+    if args.workload == "configs":
+        run_configs(args)
         return
 
     import torch
