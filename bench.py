@@ -578,12 +578,13 @@ def run_horizon(args):
     print(json.dumps(out), flush=True)
 
 
-def _time_steps(dim, N, n, precision, steps, warmup, backend="nufft", **kw):
+def _time_steps(dim, N, n, precision, steps, warmup, backend="nufft", graph=False, **kw):
     """steps/s of the combined step (RK4 + filter + interpolation + advection of n^dim particles) and the
-    state after warm-up + steps; kw goes to set_up_solver."""
+    state after warm-up + steps; kw goes to set_up_solver.  graph=True advances the trajectory as a
+    one-member qlles_batch, i.e. by replaying a captured HIP graph of 64 steps (bit-identical results)."""
     import torch
 
-    from l3es_amd import _lib, set_up_solver
+    from l3es_amd import Batch, _lib, set_up_solver
     from l3es_amd.integrator import pos_init_cartesian, spectral_interpolator_wrapper
 
     _, uh, _, _, step, _, _, _, _ = set_up_solver(N, kw.pop("nu"), dim, kw.pop("case"), kw.pop("dt"), kw.pop("seed"),
@@ -593,9 +594,18 @@ def _time_steps(dim, N, n, precision, steps, warmup, backend="nufft", **kw):
     ur = torch.empty((r.shape[0], dim), dtype=step.dtype, device=r.device)
     lib = _lib.load()
 
-    def advance(k):
-        _lib.check(lib.qlles_step_advect(step.plan.handle, it.it.handle, None, _lib.ptr(uh), int(k), _lib.ptr(r),
-                                         r.shape[0], _lib.ptr(ur), None, _lib.stream_ptr()))
+    if graph:
+        b = Batch()
+        b.add(step, uh, it, r, ur)
+        b.configure(1, 64, single_graph=True)
+
+        def advance(k):
+            if k:
+                b.run(int(k))
+    else:
+        def advance(k):
+            _lib.check(lib.qlles_step_advect(step.plan.handle, it.it.handle, None, _lib.ptr(uh), int(k),
+                                             _lib.ptr(r), r.shape[0], _lib.ptr(ur), None, _lib.stream_ptr()))
 
     advance(warmup)
     torch.cuda.synchronize()
@@ -624,9 +634,11 @@ def run_configs(args):
         _, uh, _ = _time_steps(3, 64, 32, prec, 10, 0, dt=0.01, **dict(tgv))
         e_kat = kinetic_energy_hat(uh)[0]
         sps, uh, _ = _time_steps(3, 64, 32, prec, 1000, 10, dt=0.001, **dict(tgv))
-        cases.append(dict(name="tgv64", precision=prec, steps=1000, steps_per_s=sps,
+        sps_g, uh_g, _ = _time_steps(3, 64, 32, prec, 1024, 64, dt=0.001, graph=True, **dict(tgv))
+        cases.append(dict(name="tgv64", precision=prec, steps=1000, steps_per_s=sps, graph_steps_per_s=sps_g,
                           e_kin_kat=e_kat, kat_err=abs(e_kat - 0.124953117517), e_kin=kinetic_energy_hat(uh)[0]))
-        log(f"[bench] configs tgv64 {prec}: {sps:.0f} steps/s, KAT |dE| {abs(e_kat - 0.124953117517):.1e}")
+        log(f"[bench] configs tgv64 {prec}: {sps:.0f} steps/s eager, {sps_g:.0f} graph, "
+            f"KAT |dE| {abs(e_kat - 0.124953117517):.1e}")
     # Kolmogorov 256^2 with particles
     kol = dict(case="Kolm", nu=1e-3, dt=1e-3, seed=args.seed, forcing_type="kolmogorov", drag=0.1, kolm_amp=1.0,
                kolm_k=4)
@@ -635,8 +647,12 @@ def run_configs(args):
     for prec in ("float64", "float32"):
         sps, uh, r = _time_steps(2, 256, 64, prec, K, 0, **dict(kol))
         res[prec] = (kinetic_energy_hat(uh)[0], uh, r)
-        cases.append(dict(name="kolm256", precision=prec, steps=K, steps_per_s=sps, e_kin=res[prec][0]))
-        log(f"[bench] configs kolm256 {prec}: {sps:.0f} steps/s, E={res[prec][0]:.9f}")
+        sps_g, uh_g, r_g = _time_steps(2, 256, 64, prec, K, 0, graph=True, **dict(kol))
+        same = bool(torch.equal(uh_g, uh)) and bool(torch.equal(r_g, r))
+        cases.append(dict(name="kolm256", precision=prec, steps=K, steps_per_s=sps, graph_steps_per_s=sps_g,
+                          graph_bit_identical=same, e_kin=res[prec][0]))
+        log(f"[bench] configs kolm256 {prec}: {sps:.0f} steps/s eager, {sps_g:.0f} graph (identical {same}), "
+            f"E={res[prec][0]:.9f}")
     e64, e32 = res["float64"][0], res["float32"][0]
     f64, f32 = res["float64"][1], res["float32"][1]
     d = torch.remainder(res["float32"][2] - res["float64"][2] + np.pi, 2 * np.pi) - np.pi
