@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 #include <vector>
 
 #include "capi_common.hpp"
@@ -71,6 +72,7 @@ struct qlles_sph {
   // and the full spectrum [dim][n^dim], complex FP64
   void* dft_E = nullptr;
   void* dft_U = nullptr;
+  int dft_ps = 1;  // particle subsets of the point-cloud DFT sum
   size_t dft_E_elems = 0;
 };
 
@@ -660,53 +662,90 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// one block per row of the leading axes (kx, or kx, ky); lanes over the last
-// axis and over interleaved particle subsets, subsets added in order
-template <int DIM>
-__global__ __launch_bounds__(256) void k_dft_sum(const double2* __restrict__ E, const double* __restrict__ u, int np,
-                                                 int n, double2* __restrict__ U) {
-  __shared__ double2 red[DIM][256];
-  const int W = n <= 32 ? 32 : 64, S = 256 / W;
-  const int row = blockIdx.x;
-  int kl[DIM - 1];
-  for (int d = DIM - 2, rest = row; d >= 0; --d) {
-    kl[d] = rest % n;
-    rest /= n;
-  }
+// Tiled sum.  A workgroup owns DFT_RB rows of the leading axes (kx, or kx and
+// ky) times DFT_KW consecutive last-axis modes (one mode per thread) and one
+// of DFT_PS interleaved particle subsets.  Particles are staged DFT_TJ at a
+// time in LDS: per row the weights w_c = u_jc e_x(j) [e_y(j)], and the
+// last-axis phases e_z(j, k) of the workgroup's modes, so each phase-table
+// row is read from memory once per workgroup instead of once per mode row.
+// Each thread then adds w_c e_z over the tile in particle order; the ps
+// partial spectra are added in subset order by k_dft_reduce (reproducible).
+// ps is chosen per context so that the grid holds ~2048 workgroups.
+constexpr int DFT_THREADS = 256;
+constexpr int DFT_TJ = 32;      // particles per LDS tile
+constexpr int DFT_PS_MAX = 16;  // particle subsets (workgroups per mode block), at most
+
+template <int DIM, int KW>
+__global__ __launch_bounds__(DFT_THREADS) void k_dft_sum(const double2* __restrict__ E, const double* __restrict__ u,
+                                                        int np, int n, int rows, int ps, double2* __restrict__ Up) {
+  constexpr int RB = DFT_THREADS / KW;  // rows per workgroup
+  __shared__ double2 s_w[DFT_TJ][RB][DIM];
+  __shared__ double2 s_e[DFT_TJ][KW];
+  const int tid = threadIdx.x, rl = tid / KW, kl = tid % KW;
+  const int row = blockIdx.x * RB + rl, kz = blockIdx.y * KW + kl, sub = blockIdx.z;
   const size_t plane = (size_t)np * n;
-  for (int k0 = 0; k0 < n; k0 += W) {
-    const int kz = k0 + (int)threadIdx.x % W, sub = (int)threadIdx.x / W;
-    const bool act = kz < n;
-    double2 acc[DIM];
+  // particle subset `sub`: j = sub, sub + ps, ... (tiles of TJ consecutive members)
+  const int nsub = np > sub ? (np - sub + ps - 1) / ps : 0;
+  double2 acc[DIM];
 #pragma unroll
-    for (int c = 0; c < DIM; ++c) acc[c] = make_double2(0.0, 0.0);
-    if (act)
-      for (int j = sub; j < np; j += S) {
-        double2 e = E[(size_t)(DIM - 1) * plane + (size_t)j * n + kz];
-#pragma unroll
-        for (int d = 0; d < DIM - 1; ++d) e = cmul(E[(size_t)d * plane + (size_t)j * n + kl[d]], e);
+  for (int c = 0; c < DIM; ++c) acc[c] = make_double2(0.0, 0.0);
+  for (int t0 = 0; t0 < nsub; t0 += DFT_TJ) {
+    const int tn = nsub - t0 < DFT_TJ ? nsub - t0 : DFT_TJ;
+    __syncthreads();  // the previous tile is consumed
+    // weights: TJ x RB entries, one (particle, row) pair per thread and step
+    for (int q = tid; q < DFT_TJ * RB; q += DFT_THREADS) {
+      const int jt = q / RB, rr = q % RB, rw = blockIdx.x * RB + rr;
+      if (jt < tn && rw < rows) {
+        const int j = sub + (t0 + jt) * ps;
+        double2 e = make_double2(1.0, 0.0);
+        for (int d = DIM - 2, rest = rw; d >= 0; --d) {
+          e = cmul(E[(size_t)d * plane + (size_t)j * n + rest % n], e);
+          rest /= n;
+        }
 #pragma unroll
         for (int c = 0; c < DIM; ++c) {
           const double uc = u[(size_t)j * DIM + c];
-          acc[c].x += uc * e.x;
-          acc[c].y += uc * e.y;
+          s_w[jt][rr][c] = make_double2(uc * e.x, uc * e.y);
         }
       }
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) red[c][threadIdx.x] = acc[c];
-    __syncthreads();
-    if (act && sub == 0)
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) {
-        double2 sum = red[c][threadIdx.x];
-        for (int q = 1; q < S; ++q) {
-          sum.x += red[c][threadIdx.x + q * W].x;
-          sum.y += red[c][threadIdx.x + q * W].y;
-        }
-        U[((size_t)c * gridDim.x + row) * n + kz] = sum;
+    }
+    // last-axis phases of the workgroup's modes
+    for (int q = tid; q < DFT_TJ * KW; q += DFT_THREADS) {
+      const int jt = q / KW, kk = q % KW, k = blockIdx.y * KW + kk;
+      if (jt < tn && k < n) {
+        const int j = sub + (t0 + jt) * ps;
+        s_e[jt][kk] = E[(size_t)(DIM - 1) * plane + (size_t)j * n + k];
       }
+    }
     __syncthreads();
+    if (row < rows && kz < n) {
+      for (int jt = 0; jt < tn; ++jt) {
+        const double2 e = s_e[jt][kl];
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+          const double2 w = s_w[jt][rl][c];
+          acc[c].x += w.x * e.x - w.y * e.y;
+          acc[c].y += w.x * e.y + w.y * e.x;
+        }
+      }
+    }
   }
+  if (row < rows && kz < n)
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) Up[(((size_t)sub * DIM + c) * rows + row) * n + kz] = acc[c];
+}
+
+// U = the ps partial spectra added in subset order
+__global__ void k_dft_reduce(const double2* __restrict__ Up, size_t count, int ps, double2* __restrict__ U) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= count) return;
+  double2 s = Up[id];
+  for (int p = 1; p < ps; ++p) {
+    const double2 v = Up[(size_t)p * count + id];
+    s.x += v.x;
+    s.y += v.y;
+  }
+  U[id] = s;
 }
 
 // half spectrum of ifftn(U).real: H(k) = (U(k) + conj(U(-k mod n))) / 2 in
@@ -1005,13 +1044,32 @@ int qlles_sph_dft_hat(qlles_sph* S, const double* r, const double* u_r, int np, 
         QL_HIP(hipMalloc(&S->dft_E, e_elems * sizeof(double2)));
         S->dft_E_elems = e_elems;
       }
-      if (!S->dft_U) QL_HIP(hipMalloc(&S->dft_U, (size_t)DIM * rows * n * sizeof(double2)));
+      // U (DIM x rows x n) followed by the dft_ps partial spectra; the subset
+      // count fills the grid to ~2048 workgroups (n and dim are fixed per context)
+      const size_t u_elems = (size_t)DIM * rows * n;
+      const int kw = n <= 32 ? 32 : (n <= 64 ? 64 : 128);
+      const long long mode_blocks = (long long)((rows + DFT_THREADS / kw - 1) / (DFT_THREADS / kw)) * ((n + kw - 1) / kw);
+      if (!S->dft_U) {
+        S->dft_ps = (int)std::max<long long>(1, std::min<long long>(DFT_PS_MAX, 2048 / mode_blocks));
+        QL_HIP(hipMalloc(&S->dft_U, (S->dft_ps + 1) * u_elems * sizeof(double2)));
+      }
+      const int ps = S->dft_ps;
       double2* E = static_cast<double2*>(S->dft_E);
       double2* U = static_cast<double2*>(S->dft_U);
+      double2* Up = U + u_elems;
       if (np > 0)
         hipLaunchKernelGGL(k_dft_phase, dim3(blocks((long long)DIM * np * n, 256)), dim3(256), 0, s, r, np, DIM, n,
                            E);
-      hipLaunchKernelGGL((k_dft_sum<DIM>), dim3((unsigned)rows), dim3(256), 0, s, E, u_r, np, n, U);
+      auto sum = [&](auto kw) {
+        constexpr int KW = decltype(kw)::value;
+        const dim3 g((unsigned)((rows + DFT_THREADS / KW - 1) / (DFT_THREADS / KW)), (unsigned)((n + KW - 1) / KW),
+                     (unsigned)ps);
+        hipLaunchKernelGGL((k_dft_sum<DIM, KW>), g, dim3(DFT_THREADS), 0, s, E, u_r, np, n, (int)rows, ps, Up);
+      };
+      if (n <= 32) sum(std::integral_constant<int, 32>{});
+      else if (n <= 64) sum(std::integral_constant<int, 64>{});
+      else sum(std::integral_constant<int, 128>{});
+      hipLaunchKernelGGL(k_dft_reduce, dim3(blocks((long long)u_elems, 256)), dim3(256), 0, s, Up, u_elems, ps, U);
       hipLaunchKernelGGL((k_dft_half<DIM>), dim3(blocks(3LL * rows * (n / 2 + 1), 256)), dim3(256), 0, s, U, n,
                          static_cast<double2*>(u_hat));
       QL_LAUNCHED();
