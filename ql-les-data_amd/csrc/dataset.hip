@@ -41,12 +41,24 @@ constexpr int ST_TC = 256;  // frames per workgroup (at most, unless T needs mor
 #endif
 constexpr int ST_U = QL_ST_U;  // frames whose loads are in flight together
 
-// jnp.mod(d + half, box) - half in FP32 (fmod is exact; the sign fix-up adds
-// the divisor, as numpy and jax do)
+// jnp.mod(d + half, box) - half in FP32, box > 0 (fmod is exact; the sign
+// fix-up adds the divisor, as numpy and jax do).  For positions inside the
+// box, x = d + half lies in (-box, 2 box), where that mod is x, x + box or
+// x - box with the same rounding (x - box is exact there), so only other x
+// take the fmodf path.
 __device__ __forceinline__ float periodic_disp(float d, float box, float half) {
   const float x = d + half;
-  float m = fmodf(x, box);
-  if (m != 0.0f && ((m < 0.0f) != (box < 0.0f))) m += box;
+  float m;
+  if (x >= 0.0f && x < box) {
+    m = x;
+  } else if (x < 0.0f && x > -box) {
+    m = x + box;
+  } else if (x >= box && x < 2.0f * box) {
+    m = x - box;
+  } else {
+    m = fmodf(x, box);
+    if (m != 0.0f && m < 0.0f) m += box;
+  }
   return m - half;
 }
 
@@ -97,6 +109,12 @@ __global__ void __launch_bounds__(ST_THREADS) k_traj_moments(const float* __rest
       for (int i = 0; i < ST_U + 1; ++i)
 #pragma unroll
         for (int c = 0; c < DIM; ++c) Uf[i][c] = (tb + i < T) ? up[(tb + i) * fs + c] : 0.0f;
+      // the group's displacements, each formed once (V[i] = vel at frame tb + i)
+      float V[ST_U + 1][DIM];
+#pragma unroll
+      for (int i = 0; i < ST_U + 1; ++i)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) V[i][c] = periodic_disp(R[i + 1][c] - R[i][c], box, half);
 #pragma unroll
       for (int i = 0; i < ST_U; ++i) {
         const long long t = tb + i;
@@ -105,10 +123,9 @@ __global__ void __launch_bounds__(ST_THREADS) k_traj_moments(const float* __rest
           for (int c = 0; c < DIM; ++c) {
             add(2, c, Uf[i][c]);
             if (t + 1 < T) {
-              const float v0 = periodic_disp(R[i + 1][c] - R[i][c], box, half);
-              add(0, c, v0);
+              add(0, c, V[i][c]);
               add(3, c, Uf[i + 1][c] - Uf[i][c]);
-              if (t + 2 < T) add(1, c, periodic_disp(R[i + 2][c] - R[i + 1][c], box, half) - v0);
+              if (t + 2 < T) add(1, c, V[i + 1][c] - V[i][c]);
             }
           }
         }

@@ -70,6 +70,18 @@ def test_moments_match_oracle(gpu, T, n, dim, centred):
     assert close(got, ref, mag), np.max(np.abs(got - ref) / np.maximum(mag, 1e-300))
 
 
+def test_moments_of_unwrapped_jumps(gpu):
+    """Positions outside the box and frame-to-frame jumps of several box lengths: the
+    displacement's general path (fmod), not only the in-box shortcut."""
+    rng = np.random.default_rng(5)
+    T, n = 40, 500
+    r = (rng.random((1, n, 3)) * L + np.cumsum(rng.normal(size=(T, n, 3)) * 6.0, 0)).astype(np.float32)
+    u = rng.normal(size=(T, n, 3)).astype(np.float32)
+    got = gpu_moments(r, u, 3, gpu)
+    ref = ods.trajectory_moments(r, u, L)
+    assert close(got, ref, magnitudes(r, u, 3))
+
+
 def test_moments_empty_and_reproducible(gpu):
     r, u = trajectory(50, 700, 3, 9)
     a = gpu_moments(r, u, 3, gpu)
