@@ -191,6 +191,18 @@ __global__ void k_axpby(const cx<T>* __restrict__ x, const cx<T>* __restrict__ y
   }
 }
 
+// two axpbys sharing y, one pass: out1 = a1 x1 + b1 y, out2 = a2 x2 + b2 y
+// (an output may be its own x: each element is read before it is written)
+template <typename T>
+__global__ void k_axpby2(const cx<T>* x1, const cx<T>* x2, const cx<T>* __restrict__ y, cx<T>* out1, cx<T>* out2,
+                         T a1, T b1, T a2, T b2, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const cx<T> yi = y[i], v1 = x1[i], v2 = x2[i];
+    out1[i] = cadd(cscale(v1, a1), cscale(yi, b1));
+    out2[i] = cadd(cscale(v2, a2), cscale(yi, b2));
+  }
+}
+
 // out = scale * w(kz) * in, w = c (mode 0) or 1/c (mode 1)
 template <typename T>
 __global__ void k_cweight(const cx<T>* __restrict__ in, cx<T>* __restrict__ out, int N, size_t rows, int ncomp,

@@ -320,6 +320,12 @@ class EngineT final : public Engine {
     hipLaunchKernelGGL((k_axpby<T>), dim3(nblk(spec_elems())), dim3(256), 0, s, x, y, out, a, b, spec_elems());
     QL_LAUNCHED();
   }
+  void axpby2(const cx<T>* x1, const cx<T>* x2, const cx<T>* y, cx<T>* out1, cx<T>* out2, T a1, T b1, T a2, T b2,
+              hipStream_t s) {
+    hipLaunchKernelGGL((k_axpby2<T>), dim3(nblk(spec_elems())), dim3(256), 0, s, x1, x2, y, out1, out2, a1, b1, a2,
+                       b2, spec_elems());
+    QL_LAUNCHED();
+  }
   void cross2(const T* a, const T* b, const T* c, const T* d, T* out, hipStream_t s) {
     const size_t n = (size_t)N * N * N;
     hipLaunchKernelGGL((k_cross2<T>), dim3(nblk(n)), dim3(256), 0, s, a, b, c, d, out, n);
@@ -728,14 +734,11 @@ class EngineT final : public Engine {
     for (int i = 0; i < 8; ++i) f[i] = const_cast<T*>(fs[i]);
     cx<T>*k = AS_[3], *acc = AS_[5];
     rhs_fields(y0, k, f[0], f[1], s);
-    axpby(y0, k, y[0], (T)1, h / 2, s);
-    axpby(y0, k, acc, (T)1, h / 6, s);
+    axpby2(y0, y0, k, y[0], acc, (T)1, h / 2, (T)1, h / 6, s);
     rhs_fields(y[0], k, f[2], f[3], s);
-    axpby(y0, k, y[1], (T)1, h / 2, s);
-    axpby(acc, k, acc, (T)1, h / 3, s);
+    axpby2(y0, acc, k, y[1], acc, (T)1, h / 2, (T)1, h / 3, s);
     rhs_fields(y[1], k, f[4], f[5], s);
-    axpby(y0, k, y[2], (T)1, h, s);
-    axpby(acc, k, acc, (T)1, h / 3, s);
+    axpby2(y0, acc, k, y[2], acc, (T)1, h, (T)1, h / 3, s);
     rhs_fields(y[2], k, f[6], f[7], s);
     axpby(acc, k, static_cast<cx<T>*>(uhat_out), (T)1, h / 6, s);
   }
@@ -753,18 +756,22 @@ class EngineT final : public Engine {
   void vjp_stages(const cx<T>* const (&ys)[4], const T* const (&fs)[8], const cx<T>* lam_in, cx<T>* out,
                   double* dpow, hipStream_t s) {
     const T h = (T)d_.dt;
-    cx<T>* lam = AS_[6];
-    QL_HIP(hipMemcpyAsync(lam, lam_in, spec_elems() * sizeof(cx<T>), hipMemcpyDeviceToDevice, s));
+    // lam_in is only read, and read for the last time before out is written,
+    // so out may alias it
+    const cx<T>* lam = lam_in;
     cx<T>*kb = AS_[3], *yb = AS_[4], *acc = AS_[5];
-    axpby(lam, nullptr, acc, (T)1, (T)0, s);
     // kbar_4 = h/6 l; kbar_3 = h/3 l + h ybar_3; kbar_2 = h/3 l + h/2 ybar_2; kbar_1 = h/6 l + h/2 ybar_1
     const T wl[4] = {h / 6, h / 3, h / 3, h / 6};  // for stages 3, 2, 1, 0
     const T wb[4] = {0, h, h / 2, h / 2};
+    // acc = lam + ybar_3 + ybar_2 + ... in that order; each ybar is read once,
+    // for the accumulator and the next stage's kbar together
+    axpby(lam, nullptr, kb, wl[0], wb[0], s);
     for (int q = 0; q < 4; ++q) {
       const int st = 3 - q;
-      axpby(lam, q == 0 ? nullptr : yb, kb, wl[q], wb[q], s);
       rhs_vjp_at(ys[st], kb, yb, s, dpow, fs[2 * st], fs[2 * st + 1]);
-      axpby(acc, yb, st == 0 ? out : acc, (T)1, (T)1, s);
+      const cx<T>* acc_in = q == 0 ? lam : acc;
+      if (st == 0) axpby(acc_in, yb, out, (T)1, (T)1, s);
+      else axpby2(lam, acc_in, yb, kb, acc, wl[q + 1], wb[q + 1], (T)1, (T)1, s);
     }
   }
   void irfftn_adjoint(const void* g, void* out_v, hipStream_t s) override {
