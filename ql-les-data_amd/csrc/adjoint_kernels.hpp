@@ -128,6 +128,23 @@ __global__ void k_cross2(const T* __restrict__ a, const T* __restrict__ b, const
   }
 }
 
+// both products of the RHS adjoint in one pass: out1 = w x p, out2 = p x u
+template <typename T>
+__global__ void k_cross2x2(const T* __restrict__ w, const T* __restrict__ p, const T* __restrict__ u,
+                           T* __restrict__ out1, T* __restrict__ out2, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const T w0 = w[i], w1 = w[n + i], w2 = w[2 * n + i];
+    const T p0 = p[i], p1 = p[n + i], p2 = p[2 * n + i];
+    const T u0 = u[i], u1 = u[n + i], u2 = u[2 * n + i];
+    out1[i] = w1 * p2 - w2 * p1;
+    out1[n + i] = w2 * p0 - w0 * p2;
+    out1[2 * n + i] = w0 * p1 - w1 * p0;
+    out2[i] = p1 * u2 - p2 * u1;
+    out2[n + i] = p2 * u0 - p0 * u2;
+    out2[2 * n + i] = p0 * u1 - p1 * u0;
+  }
+}
+
 // Fixed-power forcing term F(y) = a(y) M y, a = P / (2 E_f(y) + eps),
 // E_f = 0.5 sum_S w |y_k|^2 / N^6 over the shell S = {0 < k^2 <= kf2} (w = 1 on
 // the kz = 0 plane, 2 elsewhere), as the fused x-pass evaluates it in FP64.

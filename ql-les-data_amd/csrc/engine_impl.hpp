@@ -371,11 +371,14 @@ class EngineT final : public Engine {
     }
     hipLaunchKernelGGL((k_pd<T>), dim3(nblk(rows_ * Nh)), dim3(256), 0, s, l, AS_[8], N, KZD, 1);
     QL_LAUNCHED();
-    to_phys(AS_[8], AP_[2], s);                       // psi = irfftn(P D l / c)
-    cross2(wp, AP_[2], nullptr, nullptr, AP_[3], s);  // w x psi
+    to_phys(AS_[8], AP_[2], s);  // psi = irfftn(P D l / c)
+    {
+      const size_t n = (size_t)N * N * N;  // w x psi and psi x u, psi read once
+      hipLaunchKernelGGL((k_cross2x2<T>), dim3(nblk(n)), dim3(256), 0, s, wp, AP_[2], up, AP_[3], AP_[4], n);
+      QL_LAUNCHED();
+    }
     to_spec(AP_[3], AS_[7], s);
-    cross2(AP_[2], up, nullptr, nullptr, AP_[3], s);  // psi x u
-    to_spec(AP_[3], AS_[8], s);
+    to_spec(AP_[4], AS_[8], s);
     hipLaunchKernelGGL((k_vjp_combine<T>), dim3(nblk(rows_ * Nh)), dim3(256), 0, s, AS_[7], AS_[8], l, out, N, KZD,
                        (T)d_.nu);
     QL_LAUNCHED();
