@@ -295,8 +295,12 @@ class EngineT final : public Engine {
   size_t phys_elems() const { return (size_t)3 * N * N * (DIM == 3 ? N : 1); }
   void ensure_adj() {
     if (AS_[0]) return;
-    for (int i = 0; i < NAS; ++i) AS_[i] = alloc<cx<T>>(spec_elems());
-    for (int i = 0; i < NAP; ++i) AP_[i] = alloc<T>(phys_elems());
+    // one block each, so consecutive scratch arrays are adjacent: the RHS
+    // adjoint transforms AP_[3..4] into AS_[7..8] as one 6-component batch
+    cx<T>* as = alloc<cx<T>>(NAS * spec_elems());
+    for (int i = 0; i < NAS; ++i) AS_[i] = as + i * spec_elems();
+    T* ap = alloc<T>(NAP * phys_elems());
+    for (int i = 0; i < NAP; ++i) AP_[i] = ap + i * phys_elems();
     for (int i = 0; i < 8; ++i) SF_[i] = alloc<T>(phys_elems());
   }
   void check_adj() const {
@@ -307,10 +311,10 @@ class EngineT final : public Engine {
   }
   static int nblk(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 8192); }
   void to_phys(const cx<T>* src, T* dst, hipStream_t s) { irfftn_into(src, Nh, AS_[9], dst, 3, s); }
-  void to_spec(const T* src, cx<T>* dst, hipStream_t s) {
-    r2c_contig<T>(N, src, dst, 3 * N * N, Nh, (T)1, twN_, s);
-    c2c_strided<T>(N, dst, dst, 3 * N, Nh, true, (T)1, twN_, s);
-    c2c_strided<T>(N, dst, dst, 3, N * Nh, true, (T)1, twN_, s);
+  void to_spec(const T* src, cx<T>* dst, hipStream_t s, int ncomp = 3) {
+    r2c_contig<T>(N, src, dst, ncomp * N * N, Nh, (T)1, twN_, s);
+    c2c_strided<T>(N, dst, dst, ncomp * N, Nh, true, (T)1, twN_, s);
+    c2c_strided<T>(N, dst, dst, ncomp, N * Nh, true, (T)1, twN_, s);
   }
   void curl(const cx<T>* src, cx<T>* dst, hipStream_t s) {
     hipLaunchKernelGGL((k_curl_hat<T>), dim3(nblk(rows_ * Nh)), dim3(256), 0, s, src, dst, N, 3, Nh, Nh);
@@ -377,8 +381,7 @@ class EngineT final : public Engine {
       hipLaunchKernelGGL((k_cross2x2<T>), dim3(nblk(n)), dim3(256), 0, s, wp, AP_[2], up, AP_[3], AP_[4], n);
       QL_LAUNCHED();
     }
-    to_spec(AP_[3], AS_[7], s);
-    to_spec(AP_[4], AS_[8], s);
+    to_spec(AP_[3], AS_[7], s, 6);  // AP_[4] follows AP_[3], AS_[8] follows AS_[7]
     hipLaunchKernelGGL((k_vjp_combine<T>), dim3(nblk(rows_ * Nh)), dim3(256), 0, s, AS_[7], AS_[8], l, out, N, KZD,
                        (T)d_.nu);
     QL_LAUNCHED();
