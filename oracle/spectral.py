@@ -345,8 +345,11 @@ def init_u_kolm_np(N, seed=42, k0=4.0, max_velocity=4.2):
 
     The reference's init_u_kolm (init_fields.py:191-212) uses jax_cfd and the
     JAX PRNG, which cannot be reproduced here; this generator instead draws a
-    Gaussian white-noise vorticity, shapes it to E(k) ~ k^4 exp(-2 (k/k0)^2)
-    (peak at k0) and inverts the stream function.  Returns (3, N, N, 1).
+    Gaussian white-noise vorticity and shapes it so that the shell enstrophy
+    (the vorticity spectrum) is Z(k) ~ k^4 exp(-2 (k/k0)^2), peaking at k0;
+    the energy spectrum Z(k)/k^2 then peaks at k0/sqrt(2).  It inverts the
+    stream function and scales max(|u|, |v|) to max_velocity.  Returns
+    (3, N, N, 1).
     """
     rng = np.random.default_rng(seed)
     xi = rng.standard_normal((N, N))
@@ -361,8 +364,8 @@ def init_u_kolm_np(N, seed=42, k0=4.0, max_velocity=4.2):
     psi_hat = w_hat / np.where(k2 == 0, 1.0, k2)
     u_hat = 1j * KY * psi_hat
     v_hat = -1j * KX * psi_hat
-    u = np.fft.irfftn(u_hat, s=(N, N))
-    v = np.fft.irfftn(v_hat, s=(N, N))
+    u = np.fft.irfftn(u_hat, s=(N, N), axes=(0, 1))
+    v = np.fft.irfftn(v_hat, s=(N, N), axes=(0, 1))
     s = max_velocity / max(np.abs(u).max(), np.abs(v).max())
     out = np.zeros((3, N, N, 1))
     out[0, :, :, 0] = u * s

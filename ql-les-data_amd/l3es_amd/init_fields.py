@@ -151,8 +151,9 @@ def kolm_vorticity_spectrum(N, seed=42, k0=4.0):
 
     The reference's init_u_kolm (init_fields.py:191-212) draws from jax_cfd with
     the JAX PRNG, which is not reproducible here; this project-defined
-    generator shapes a Gaussian white-noise vorticity to E(k) ~ k^4
-    exp(-2 (k/k0)^2) (peak at k0) and inverts the stream function.
+    generator shapes a Gaussian white-noise vorticity so that its shell
+    enstrophy is Z(k) ~ k^4 exp(-2 (k/k0)^2) (peak at k0; the energy
+    spectrum Z/k^2 peaks at k0/sqrt(2)) and inverts the stream function.
     Returns (u_hat, v_hat) in rfft2 layout (N, N/2+1).
     """
     rng = np.random.default_rng(seed)

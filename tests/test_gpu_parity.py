@@ -451,17 +451,21 @@ def test_config_tgv64_fp64_dt1e3(gpu):
 def test_config_kolmogorov256_particles(gpu, prec, tf, tr):
     """2-D Kolmogorov 256^2 (f_x = sin 4y, drag 0.1, nu = 1e-3, dt = 1e-3) with 64^2 jittered particles
     advected by the NUFFT (tol 1e-6): 20 combined-mode steps against the oracle's step + filter + DFT
-    interpolation + shift chain (combined.py:97,127,130,199)."""
+    interpolation + shift chain (combined.py:97,127,130,199).  The random-phase initial field and the
+    jittered lattice are the package's own (init_u_kolm, pos_init_jittered), checked against the oracle's."""
     from l3es_amd import _lib, set_up_solver
-    from l3es_amd.integrator import spectral_interpolator_wrapper
+    from l3es_amd.integrator import pos_init_jittered, spectral_interpolator_wrapper
 
     N, n, K, dt = 256, 64, 20, 1e-3
     kw = dict(forcing_type="kolmogorov", drag=0.1, kolm_amp=1.0, kolm_k=4)
     u0 = O.init_u_kolm_np(N, seed=3)
-    _, uh, _, _, step, _, _, _, _ = set_up_solver(N, 1e-3, 2, "Kolm", dt, 3, n, None, precision=prec,
-                                                  warmup=False, u_init=u0, **kw)
+    ug0, uh, _, _, step, _, _, _, _ = set_up_solver(N, 1e-3, 2, "Kolm", dt, 3, n, None, precision=prec,
+                                                    warmup=False, **kw)
+    assert ug0.shape == (3, N, N, 1)
+    assert rel(host(ug0), u0) < (1e-13 if prec == "float64" else 1e-7)
     it = spectral_interpolator_wrapper(n, (1, 2), backend="nufft", precision=prec, tol=1e-6)
-    r0 = O.pos_init_jittered(n, 2, seed=5)
+    r0 = pos_init_jittered(n, 2, seed=5)
+    np.testing.assert_array_equal(r0, O.pos_init_jittered(n, 2, seed=5))
     r = dev(r0)
     ur = torch.empty((r.shape[0], 2), dtype=tdt(prec), device=r.device)
     lib = _lib.load()
