@@ -69,11 +69,19 @@ typedef struct qlles_desc {
 } qlles_desc;
 
 /* Plan = rhs_wrapper + rk4_wrapper + custom_jit_integrate
- * (spectral_solver.py:27-48, :65-71, :130-153): validates the descriptor,
- * builds twiddle tables and allocates the device workspace. */
+ * (spectral_solver.py:27-48, :65-71, :130-153): validates the descriptor
+ * and builds twiddle tables.  Device workspace is allocated on first use:
+ * the reduction partials at creation, the transform scratch by the first
+ * qlles_irfftn / qlles_filter / qlles_energy_spectrum / qlles_dissipation,
+ * and the step workspace (1.85 GiB at 256^3 FP32) by the first qlles_step,
+ * qlles_step_advect, qlles_rhs, qlles_plan_tune, tangent or adjoint call.
+ * A caller that captures a plan's calls into a HIP graph makes one such call
+ * eagerly first (qlles_batch_add does this for its members), so that no
+ * allocation happens inside the capture.  A plan used only for qlles_rfftn,
+ * qlles_diagnostics or qlles_energy_hat holds a few KiB. */
 int qlles_plan_create(const qlles_desc* desc, qlles_plan** plan);
 int qlles_plan_destroy(qlles_plan* plan);
-/* bytes of device workspace held by the plan */
+/* bytes of device workspace the plan holds now */
 size_t qlles_plan_workspace_bytes(const qlles_plan* plan);
 
 /* Kernel variants (no reference counterpart; the reference leaves FFT

@@ -197,10 +197,11 @@ __global__ void __launch_bounds__(256) k_force_tangent(const cx<T>* __restrict__
   }
 }
 
-// out = alpha x + beta y (complex arrays of n entries; y may be nullptr)
+// out = alpha x + beta y (complex arrays of n entries; y may be nullptr).
+// out may be x or y (in-place updates): each element is read before it is
+// written, so none of the pointers is __restrict__.
 template <typename T>
-__global__ void k_axpby(const cx<T>* __restrict__ x, const cx<T>* __restrict__ y, cx<T>* __restrict__ out, T alpha,
-                        T beta, size_t n) {
+__global__ void k_axpby(const cx<T>* x, const cx<T>* y, cx<T>* out, T alpha, T beta, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     cx<T> v = cscale(x[i], alpha);
     if (y != nullptr) v = cadd(v, cscale(y[i], beta));

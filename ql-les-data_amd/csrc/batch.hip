@@ -215,6 +215,7 @@ int qlles_batch_add(qlles_batch* b, qlles_plan* p, qlles_interp* it, void* u_hat
   return guarded([&] {
     const int N = p->desc.N, dim = p->desc.dim;
     const size_t rows = dim == 3 ? (size_t)N * N : (size_t)N;
+    p->eng->prepare();  // the step workspace exists before any capture
     qlles_batch::Member x{p, it, u_hat, it ? r : nullptr, it ? np : 0, it ? u_r : nullptr,
                           3 * rows * (N / 2 + 1) * 2 * elem_bytes(p),
                           it ? (size_t)np * dim * sizeof(double) : 0, it ? (size_t)np * dim * elem_bytes(p) : 0};

@@ -23,6 +23,9 @@ class Engine {
  public:
   virtual ~Engine() {}
   virtual size_t workspace_bytes() const = 0;
+  // allocate the step workspace now (it is otherwise allocated by the first
+  // step, RHS, tangent or adjoint call): done before a graph capture
+  virtual void prepare() = 0;
   virtual void step(void* u_out, void* uhat, const void* u_in, int nsteps, double* e_inj, hipStream_t s) = 0;
   virtual void rhs(const void* u, const void* uhat, int rk, void* du, hipStream_t s) = 0;
   virtual void rfftn(const void* u, void* uhat, hipStream_t s) = 0;

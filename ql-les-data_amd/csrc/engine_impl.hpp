@@ -24,7 +24,9 @@ class EngineT final : public Engine {
   int LDM;
   size_t qs(int nf) const { return (size_t)nf * N * N * LDM; }
   size_t rows_;  // N*N (3-D) or N (2-D)
+  size_t fsz_;   // rows_ * LD: one padded spectral component
   size_t bytes_ = 0;
+  bool step_ready_ = false;
   std::vector<void*> allocs_;
   // P: u_n in the padded row stride LD (the caller's spectrum has rows of
   // N/2+1, which are not cache-line aligned), refreshed by the state load of
@@ -294,6 +296,7 @@ class EngineT final : public Engine {
   size_t spec_elems() const { return (size_t)3 * rows_ * Nh; }
   size_t phys_elems() const { return (size_t)3 * N * N * (DIM == 3 ? N : 1); }
   void ensure_adj() {
+    ensure_step();
     if (AS_[0]) return;
     // one block each, so consecutive scratch arrays are adjacent: the RHS
     // adjoint transforms AP_[3..4] into AS_[7..8] as one 6-component batch
@@ -426,6 +429,35 @@ class EngineT final : public Engine {
     state_fields(AS_[2], SF_[6], SF_[7], s);
   }
 
+  // The plan allocates its large buffers on first use: Y6 (transform
+  // scratch) for the inverse transform, filter and spectrum, X5 as well for
+  // the dissipation, and the whole step workspace for the first step, RHS,
+  // tangent or adjoint.  A plan used only for rfftn, diagnostics or energies
+  // holds the reduction partials alone.
+  void ensure_y6() {
+    if (!Y6) Y6 = alloc<cx<T>>((DIM == 3 ? 6 : 3) * fsz_);
+  }
+  void ensure_x5() {
+    if (!X5) X5 = alloc<cx<T>>((DIM == 3 ? 5 : 3) * fsz_);
+  }
+  void ensure_step() {
+    if (step_ready_) return;
+    ensure_x5();
+    ensure_y6();
+    const int nc = DIM == 3 ? 3 : 2;
+    if (DIM == 3) {
+      P = alloc<cx<T>>(3 * fsz_);
+      NZ = alloc<cx<T>>((size_t)3 * N * N * KZP);
+      NY = alloc<cx<T>>((size_t)3 * N * KYD * KZP);
+    } else {
+      NY = alloc<cx<T>>((size_t)2 * N * KZP);
+    }
+    S1a = alloc<cx<T>>(nc * fsz_);
+    S1b = alloc<cx<T>>(nc * fsz_);
+    S2 = alloc<cx<T>>(nc * fsz_);
+    step_ready_ = true;
+  }
+
  public:
   explicit EngineT(const Desc& d) : d_(d) {
     KZD = dealias_count(N);
@@ -434,26 +466,11 @@ class EngineT final : public Engine {
     LD = round_up(Nh, 16);
     LDM = N / 2;
     rows_ = DIM == 3 ? (size_t)N * N : (size_t)N;
-    const size_t fsz = rows_ * LD;
-    if (DIM == 3) {
-      P = alloc<cx<T>>(3 * fsz);
-      S1a = alloc<cx<T>>(3 * fsz);
-      S1b = alloc<cx<T>>(3 * fsz);
-      S2 = alloc<cx<T>>(3 * fsz);
-      X5 = alloc<cx<T>>(5 * fsz);
-      Y6 = alloc<cx<T>>(6 * fsz);
-      NZ = alloc<cx<T>>((size_t)3 * N * N * KZP);
-      NY = alloc<cx<T>>((size_t)3 * N * KYD * KZP);
+    fsz_ = rows_ * LD;
+    if (DIM == 3)
       nxblk_ = std::max(xtiles_v<0>(), xtiles_v<HAS_V ? 1 : 0>()) * N;  // partials capacity over variants
-    } else {
-      S1a = alloc<cx<T>>(2 * fsz);
-      S1b = alloc<cx<T>>(2 * fsz);
-      S2 = alloc<cx<T>>(2 * fsz);
-      X5 = alloc<cx<T>>(3 * fsz);
-      Y6 = alloc<cx<T>>(3 * fsz);  // scratch for transforms / diagnostics
-      NY = alloc<cx<T>>((size_t)2 * N * KZP);
+    else
       nxblk_ = (Nh + XCfg<N, HAS_V ? 1 : 0>::TIX - 1) / XCfg<N, HAS_V ? 1 : 0>::TIX;
-    }
     partials_ = alloc<double>(std::max(RED_BLOCKS * 4, nxblk_ * 2));
     alpha_ = alloc<double>(8);
     dscr_ = alloc<double>(8);
@@ -465,8 +482,10 @@ class EngineT final : public Engine {
     for (void* p : allocs_) (void)hipFree(p);
   }
   size_t workspace_bytes() const override { return bytes_; }
+  void prepare() override { ensure_step(); }
 
   void step(void* u_out_v, void* uhat, const void* u_in_v, int nsteps, double* e_inj, hipStream_t s) override {
+    ensure_step();
     cx<T>* S0 = static_cast<cx<T>*>(uhat);
     T* u_out = static_cast<T*>(u_out_v);
     const T* u_in = static_cast<const T*>(u_in_v);
@@ -564,6 +583,7 @@ class EngineT final : public Engine {
   }
 
   void rhs(const void* u_v, const void* uhat_v, int rk, void* du_v, hipStream_t s) override {
+    ensure_step();
     cx<T>* uh = const_cast<cx<T>*>(static_cast<const cx<T>*>(uhat_v));
     cx<T>* du = static_cast<cx<T>*>(du_v);
     const T* u = (rk == 0) ? static_cast<const T*>(u_v) : nullptr;
@@ -621,6 +641,7 @@ class EngineT final : public Engine {
   }
 
   void irfftn(const void* uhat_v, void* u_v, hipStream_t s) override {
+    ensure_y6();
     irfftn_into(static_cast<const cx<T>*>(uhat_v), Nh, Y6, static_cast<T*>(u_v), 3, s);
   }
 
@@ -629,6 +650,8 @@ class EngineT final : public Engine {
   }
 
   void dissipation(const void* uhat_v, double* out, hipStream_t s) override {
+    ensure_y6();
+    ensure_x5();
     const cx<T>* uh = static_cast<const cx<T>*>(uhat_v);
     const size_t total = rows_ * Nh;
     const int nb = (int)std::min<size_t>((total + 255) / 256, 4096);
@@ -650,6 +673,7 @@ class EngineT final : public Engine {
   }
 
   void spectrum(const void* u_v, double mul_fac, double* ek, hipStream_t s) override {
+    ensure_y6();
     cx<T>* vh = Y6;  // [3][rows][Nh] scratch
     rfftn(u_v, vh, s);
     constexpr int NBLK = 256;
@@ -804,6 +828,7 @@ class EngineT final : public Engine {
     const size_t rows = DIM == 3 ? (size_t)n * n : (size_t)n;
     const size_t total = rows * nh * ncomp;
     const int nb = (int)std::min<size_t>((total + 255) / 256, 8192);
+    ensure_y6();
     cx<T>* g = Y6;  // [ncomp][rows][nh]
     hipLaunchKernelGGL((k_window<T>), dim3(nb), dim3(256), 0, s, static_cast<const cx<T>*>(uhat_v), N, DIM, Nh, g,
                        n, nh);

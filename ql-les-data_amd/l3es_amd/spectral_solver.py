@@ -133,9 +133,10 @@ def diagnostics(u, plan):
 def comp_dt(u, dx, nu, cfl=1.0, plan=None):
     """CFL estimate (spectral_solver.py:123-127), FP64 reduction on the GPU."""
     if plan is None:
-        N = u.shape[1]
-        dim = 3 if u.shape[-1] == N else 2
-        plan = _lib.Plan(dim, N, u.dtype, 0.0, 1.0)
+        from .utils import helper_plan, physical_field
+
+        dim, N, u = physical_field(u)
+        plan = helper_plan(dim, N, u.dtype)
     d = diagnostics(u, plan).cpu().numpy()
     umax = d[2:5]
     return float(cfl / (umax / dx + nu / dx ** 2).min())
@@ -207,7 +208,9 @@ def set_up_solver(N, nu, dim, case, dt, seed, ckp_N, kf, e_kin_target=None, forc
     else:
         raise ValueError(f"unknown case {case!r}")
     u = u.to(dtype).contiguous()
-    plan0 = _lib.Plan(dim, N, dtype, nu, dt)
+    from .utils import helper_plan
+
+    plan0 = helper_plan(dim, N, dtype)  # rfftn and diagnostics only: no step workspace
     e_kin_init = float(diagnostics(u, plan0)[0])  # spectral_solver.py:200
     if forcing_type != "none" and e_kin_target is not None:  # :201-203
         u = (u * np.sqrt(e_kin_target / (e_kin_init + EPS))).contiguous()

@@ -1,0 +1,133 @@
+"""The batched configuration at its size: 24 trajectories of uneven lengths on one GPU, 2-D Kolmogorov
+at 64^2, 128^2, 256^2 and 512^2 (32^2 NUFFT particles each) and 3-D power-forced HIT at 64^3, 96^3 and
+128^3 (16^3 particles each), advanced by qlles_batch after its autotuner picked the launch shape.
+
+No reference counterpart (the reference runs one trajectory per process, slurm_*.sh:5).  Pins:
+  - every FP32 member is bit-identical to the same steps taken alone (qlles_step_advect);
+  - the worst FP32 kinetic-energy error of the shortest member of each grid size against the FP64
+    oracle's trajectory from the same seed stays within 1e-5;
+  - an FP64 batch with one member per grid size equals the oracle's step + filter + DFT interpolation +
+    shift chain (combined.py:97,127,130,199) to round-off.
+"""
+
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(2, 64), (2, 128), (2, 256), (2, 512), (3, 64), (3, 96), (3, 128)]
+KOLM = dict(forcing_type="kolmogorov", drag=0.1, kolm_amp=1.0, kolm_k=4)
+HIT = dict(e_kin_target=0.5, forcing_type="power", power=0.1, spectrum="k4exp")
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def cfg(i):
+    dim, N = SIZES[i % len(SIZES)]
+    if dim == 2:
+        return dim, N, 1e-3, "Kolm", None, dict(KOLM), 32
+    return dim, N, 5e-4, "HIT", 2, dict(HIT), 16
+
+
+def make(i, prec):
+    from l3es_amd import set_up_solver
+    from l3es_amd.integrator import pos_init_jittered, spectral_interpolator_wrapper
+
+    dim, N, dt, case, kf, kw, n = cfg(i)
+    _, uh, _, _, step, _, _, _, _ = set_up_solver(N, 1e-3, dim, case, dt, 100 + i, n, kf, precision=prec,
+                                                  warmup=False, **kw)
+    it = spectral_interpolator_wrapper(n, tuple(range(1, dim + 1)), backend="nufft", precision=prec, tol=1e-6)
+    r = torch.from_numpy(pos_init_jittered(n, dim, seed=i)).cuda()
+    return step, uh, it, r
+
+
+def oracle_run(i, nsteps, particles=None):
+    dim, N, dt, case, kf, kw, n = cfg(i)
+    fft = O.fft_backend(workers=min(16, len(os.sched_getaffinity(0))))
+    u, uh, step, _, _ = O.set_up_solver(N, 1e-3, dim, case, dt, seed=100 + i, kf=kf, fft=fft, warmup=False, **kw)
+    r = particles
+    for _ in range(nsteps):
+        u, uh, _ = step(u, uh)
+        if r is not None:
+            uf = O.spectral_filtering(u if dim == 3 else u[:2, :, :, 0], n)
+            r = O.shift_fn(r, dt * O.interpolate_dft(uf, r))
+    return u, uh, r
+
+
+def _advect_solo(step, it, uh, r, k):
+    from l3es_amd import _lib
+
+    ur = torch.empty((r.shape[0], it.dim), dtype=it.dtype, device=r.device)
+    _lib.check(_lib.load().qlles_step_advect(step.plan.handle, it.it.handle, None, _lib.ptr(uh), int(k), _lib.ptr(r),
+                                             r.shape[0], _lib.ptr(ur), None, _lib.stream_ptr()))
+    return ur
+
+
+def test_batch24_bit_identical_to_solo_and_energy_vs_oracle(gpu):
+    from l3es_amd import Batch
+    from l3es_amd.utils import kinetic_energy_hat
+
+    M = 24
+    lengths = [6 + 2 * (i % 4) for i in range(M)]
+    mem = [make(i, "float32") for i in range(M)]
+    print("[batch24] 24 FP32 members set up", flush=True)
+    solo = [(uh.clone(), r.clone()) for (_, uh, _, r) in mem]
+    b = Batch()
+    urs = []
+    for step, uh, it, r in mem:
+        ur = torch.empty((r.shape[0], it.dim), dtype=it.dtype, device=r.device)
+        urs.append(ur)
+        b.add(step, uh, it, r, ur)
+    cfg_t = b.tune(trial_steps=8)  # on scratch copies: the states do not move
+    print(f"[batch24] tuned {cfg_t}", flush=True)
+    b.run(lengths)
+    torch.cuda.synchronize()
+    for i, ((step, uh_b, it, r_b), (uh_s, r_s)) in enumerate(zip(mem, solo)):
+        ur_s = _advect_solo(step, it, uh_s, r_s, lengths[i])
+        torch.cuda.synchronize()
+        assert np.array_equal(host(uh_s), host(uh_b)), f"member {i}: spectrum differs from its solo run"
+        assert np.array_equal(host(r_s), host(r_b)), f"member {i}: positions differ"
+        assert np.array_equal(host(ur_s), host(urs[i])), f"member {i}: particle velocities differ"
+    print("[batch24] every member bit-identical to its solo run", flush=True)
+    worst = 0.0
+    for k in range(len(SIZES)):
+        i = min(range(k, M, len(SIZES)), key=lambda j: lengths[j])  # shortest member of this size
+        _, uho, _ = oracle_run(i, lengths[i])
+        dim, N = SIZES[k]
+        e_o = O.kinetic_energy_hat(uho, N, dim)
+        e_g = kinetic_energy_hat(mem[i][1])[0]
+        err = abs(e_g - e_o) / abs(e_o)
+        worst = max(worst, err)
+        print(f"[batch24] member {i} ({dim}-D {N}, {lengths[i]} steps): FP32 energy vs FP64 oracle {err:.2e}",
+              flush=True)
+    assert worst < 1e-5, worst
+
+
+def test_batch_fp64_one_member_per_size_matches_oracle(gpu):
+    from l3es_amd import Batch
+
+    lengths = [1, 2, 3, 1, 2, 1, 2]
+    mem = [make(i, "float64") for i in range(len(SIZES))]
+    r0 = [host(r).copy() for (_, _, _, r) in mem]
+    b = Batch()
+    for step, uh, it, r in mem:
+        b.add(step, uh, it, r)
+    b.configure(lanes=4, chunk=2)  # whole chunks by graph replay, the remainders eagerly
+    b.run(lengths)
+    torch.cuda.synchronize()
+    for i, (step, uh, it, r) in enumerate(mem):
+        dim, N = SIZES[i]
+        _, uho, ro = oracle_run(i, lengths[i], r0[i])
+        err = np.abs(host(uh) - uho).max() / np.abs(uho).max()
+        assert err < 1e-12, (dim, N, err)
+        d = np.abs(host(r) - ro)
+        d = np.minimum(d, 2 * np.pi - d)
+        assert d.max() < 1e-7, (dim, N, d.max())  # 3 steps x dt x the NUFFT tolerance x |u|
+        print(f"[batch64] {dim}-D {N}: spectrum {err:.1e}, positions {d.max():.1e}", flush=True)

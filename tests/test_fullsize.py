@@ -58,6 +58,78 @@ def test_hit256_fp64_step_matches_oracle(gpu):
     assert abs(e[0] - float(eo)) <= 1e-12 * abs(float(eo))
 
 
+NP, SAMPLE = 32, 2048
+
+
+def test_hit256_combined_step_with_particles_fp64(gpu):
+    """The combined-mode loop at its size (combined.py:97,127,130,199): one qlles_step_advect of the
+    256^3 power-forced field with 32^3 jittered particles, filtered to the 32^3 window and interpolated
+    by the NUFFT (tol 1e-6), against the oracle's step, spectral_filtering, exact DFT interpolation and
+    shift.  The oracle interpolates a seeded sample of 2048 of the particles."""
+    from l3es_amd import _lib, set_up_solver
+    from l3es_amd.integrator import pos_init_jittered, spectral_interpolator_wrapper
+
+    print("[fullsize] FP64 256^3 step + 32^3-particle advection", flush=True)
+    _, uh, _, _, step, _, _, _, _ = set_up_solver(N, NU, 3, "HIT", DT, 42, NP, KF, e_kin_target=E0,
+                                                  forcing_type="power", power=P, spectrum="k4exp",
+                                                  precision="float64", warmup=False)
+    it = spectral_interpolator_wrapper(NP, (1, 2, 3), backend="nufft", precision="float64", tol=1e-6)
+    r0 = pos_init_jittered(NP, 3, seed=1)
+    r = torch.from_numpy(r0).cuda()
+    ur = torch.empty((NP ** 3, 3), dtype=torch.float64, device=r.device)
+    _lib.check(_lib.load().qlles_step_advect(step.plan.handle, it.it.handle, None, _lib.ptr(uh), 1, _lib.ptr(r),
+                                             NP ** 3, _lib.ptr(ur), None, _lib.stream_ptr()))
+    torch.cuda.synchronize()
+    uhg, rg, urg = host(uh), host(r), host(ur)
+    del uh, step
+    torch.cuda.empty_cache()
+    fft = O.fft_backend(workers=min(16, len(os.sched_getaffinity(0))))
+    uo, uho, so, _, _ = O.set_up_solver(N, NU, 3, "HIT", DT, seed=42, kf=KF, e_kin_target=E0,
+                                        forcing_type="power", power=P, spectrum="k4exp", fft=fft, warmup=False)
+    uo, uho, _ = so(uo, uho)
+    assert rel(uhg, uho) < 1e-12
+    pick = np.random.default_rng(3).choice(NP ** 3, SAMPLE, replace=False)
+    uro = O.interpolate_dft(O.spectral_filtering(uo, NP), r0[pick])
+    err = np.linalg.norm(urg[pick] - uro) / np.linalg.norm(uro)
+    assert err < 1e-6, err  # the NUFFT tolerance
+    ro = O.shift_fn(r0[pick], DT * uro)
+    d = np.abs(rg[pick] - ro)
+    d = np.minimum(d, 2 * np.pi - d)
+    assert d.max() < DT * 1e-4 * np.abs(uro).max()
+    # the particles not sampled moved by dt u_r too
+    d_all = (rg - r0 - DT * urg + np.pi) % (2 * np.pi) - np.pi
+    assert np.abs(d_all).max() < 1e-12
+
+
+def test_hit256_fp32_energy_trajectory_1000_steps(gpu):
+    """FP32 against FP64 over 1000 steps (t = 0.5) of the forced configuration: the FP64 Parseval kinetic
+    energy of the two runs, sampled every 50 steps, agrees within 1e-5 relative."""
+    from l3es_amd import set_up_solver
+    from l3es_amd.utils import kinetic_energy_hat
+
+    runs = {}
+    for prec in ("float64", "float32"):
+        _, uh, _, _, step, _, _, _, _ = set_up_solver(N, NU, 3, "HIT", DT, 42, 32, KF, e_kin_target=E0,
+                                                      forcing_type="power", power=P, spectrum="k4exp",
+                                                      precision=prec, warmup=False)
+        runs[prec] = (uh, step)
+    e = {p: [kinetic_energy_hat(runs[p][0])[0]] for p in runs}
+    for blk in range(20):
+        for prec, (uh, step) in runs.items():
+            step.advance(None, uh, 50, None, use_u=False)
+            e[prec].append(kinetic_energy_hat(uh)[0])
+        print(f"[fullsize] energy trajectory: {50 * (blk + 1)} steps", flush=True)
+    e64, e32 = np.array(e["float64"]), np.array(e["float32"])
+    assert np.all(np.isfinite(e32))
+    err = np.abs(e32 / e64 - 1).max()
+    print(f"[fullsize] 1000-step FP32-vs-FP64 energy: max rel diff {err:.2e}", flush=True)
+    assert err < 1e-5, err
+    # one step workspace per solver, allocated by its first step; rhs_fn was never called
+    ws = runs["float32"][1].plan.workspace_bytes
+    assert ws <= 1.9 * 2 ** 30, ws / 2 ** 30
+    assert runs["float32"][1].rhs.plan.workspace_bytes < 2 ** 20
+
+
 def test_hit256_fp32_against_fp64_and_divergence(gpu):
     u64, uh64, _ = gpu_run("float64", 10)
     a64, h64 = host(u64), host(uh64)
