@@ -340,6 +340,23 @@ int qlles_stats_destroy(qlles_stats* st);
 int qlles_stats_moments(qlles_stats* st, const float* r, const float* u, long long T, long long np,
                         const double* center, double* out, void* stream);
 
+/* qlles_stats_moments without a context and without allocating: no
+ * counterpart in the reference.  It is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  dim and box are those of qlles_stats_create; the other
+ * arguments, the launches and the summation order are those of
+ * qlles_stats_moments, so the results are bitwise equal.  The per-workgroup
+ * partial sums live in a static device array of QLLES_STATS_SCRATCH_DOUBLES
+ * doubles (8 MiB) that each GPU the library runs on holds until the process
+ * ends; a trajectory whose partials need more (8 dim doubles per workgroup
+ * of 256 particles x up to 256 frames: 3840 workgroups for 7501 frames of
+ * 32^3 particles) is refused with QLLES_ERR_ARG.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams,
+ * other threads) race and give wrong sums. */
+#define QLLES_STATS_SCRATCH_DOUBLES (1 << 20)
+int qlles_stats_moments_noalloc(int dim, double box, const float* r, const float* u, long long T, long long np,
+                                const double* center, double* out, void* stream);
+
 /* message of the last failed call on this thread */
 const char* qlles_last_error(void);
 

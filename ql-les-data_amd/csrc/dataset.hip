@@ -25,11 +25,19 @@
 // scheduling.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 
 #include "capi_common.hpp"
 
 using qlapi::fail;
 using qlapi::guarded;
+
+// Partial sums of qlles_stats_moments_noalloc: one array per GPU for the
+// life of the process (8 MiB).  In host code `scratch` names a host-side
+// stand-in, not a device address; the device address comes from
+// hipGetSymbolAddress, once per device, and only that address is passed on.
+static __device__ __align__(256) double scratch[QLLES_STATS_SCRATCH_DOUBLES];
 
 namespace {
 
@@ -166,6 +174,60 @@ __global__ void __launch_bounds__(ST_THREADS) k_traj_moments_final(const double*
   }
 }
 
+// Launch shape of one trajectory: gx workgroups of 256 particles times gy
+// groups of tc frames.  Frames per workgroup: up to ST_TC, fewer while that
+// leaves under ~4096 workgroups (16 per CU), more if T would overflow grid.y.
+// Both entry points use it, so they sum in the same order.
+struct StatsGrid {
+  long long gx, gy, tc, nb;
+};
+StatsGrid stats_grid(long long T, long long np) {
+  StatsGrid g{};
+  g.gx = (np + ST_THREADS - 1) / ST_THREADS;
+  g.tc = std::min<long long>(ST_TC, std::max<long long>(ST_TC / 4, (T * g.gx + 4095) / 4096));
+  g.tc = std::max<long long>(g.tc, (T + 65534) / 65535);
+  g.gy = (T + g.tc - 1) / g.tc;
+  if (g.gx > 0x7fffffffLL) throw std::invalid_argument("np too large");
+  g.nb = g.gx * g.gy;
+  return g;
+}
+
+void launch_moments(int dim, double box_d, const float* r, const float* u, long long T, long long np,
+                    const double* center, double* partials, double* out, bool empty, hipStream_t s) {
+  long long nb = 0;
+  if (!empty) {
+    const StatsGrid g = stats_grid(T, np);
+    nb = g.nb;
+    const dim3 grid((unsigned)g.gx, (unsigned)g.gy);
+    const float box = (float)box_d;
+    if (dim == 3)
+      hipLaunchKernelGGL(k_traj_moments<3>, grid, dim3(ST_THREADS), 0, s, r, u, T, np, g.tc, box, center, partials);
+    else
+      hipLaunchKernelGGL(k_traj_moments<2>, grid, dim3(ST_THREADS), 0, s, r, u, T, np, g.tc, box, center, partials);
+    QL_LAUNCHED();
+  }
+  hipLaunchKernelGGL(k_traj_moments_final, dim3(8 * dim), dim3(ST_THREADS), 0, s, partials, nb, out);
+  QL_LAUNCHED();
+}
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+
+double* scratch_on_current_device() {
+  int dev = -1;
+  QL_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) throw std::invalid_argument("device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    QL_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw ql::HipError("hipGetSymbolAddress returned a null address for the scratch array");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
 }  // namespace
 
 struct qlles_stats {
@@ -209,17 +271,8 @@ int qlles_stats_moments(qlles_stats* S, const float* r, const float* u, long lon
   const int dim = S->dim, nv = 8 * dim;
   hipStream_t s = (hipStream_t)stream;
   return guarded([&] {
-    long long nb = 0;
     if (!empty) {
-      // frames per workgroup: up to ST_TC, fewer while that leaves under
-      // ~4096 workgroups (16 per CU), more if T would overflow grid.y
-      const long long gx = (np + ST_THREADS - 1) / ST_THREADS;
-      long long tc = std::min<long long>(ST_TC, std::max<long long>(ST_TC / 4, (T * gx + 4095) / 4096));
-      tc = std::max<long long>(tc, (T + 65534) / 65535);
-      const long long gy = (T + tc - 1) / tc;
-      if (gx > 0x7fffffffLL) throw std::invalid_argument("np too large");
-      nb = gx * gy;
-      const size_t need = (size_t)nb * nv;
+      const size_t need = (size_t)stats_grid(T, np).nb * nv;
       if (need > S->cap) {
         if (S->partials) {
           QL_HIP(hipStreamSynchronize(s));
@@ -230,18 +283,32 @@ int qlles_stats_moments(qlles_stats* S, const float* r, const float* u, long lon
         QL_HIP(hipMalloc(&S->partials, need * sizeof(double)));
         S->cap = need;
       }
-      const dim3 grid((unsigned)gx, (unsigned)gy);
-      const float box = (float)S->box;
-      if (dim == 3)
-        hipLaunchKernelGGL(k_traj_moments<3>, grid, dim3(ST_THREADS), 0, s, r, u, T, np, tc, box, center,
-                           S->partials);
-      else
-        hipLaunchKernelGGL(k_traj_moments<2>, grid, dim3(ST_THREADS), 0, s, r, u, T, np, tc, box, center,
-                           S->partials);
-      QL_LAUNCHED();
     }
-    hipLaunchKernelGGL(k_traj_moments_final, dim3(nv), dim3(ST_THREADS), 0, s, S->partials, nb, out);
-    QL_LAUNCHED();
+    launch_moments(dim, S->box, r, u, T, np, center, S->partials, out, empty, s);
+  });
+}
+
+int qlles_stats_moments_noalloc(int dim, double box, const float* r, const float* u, long long T, long long np,
+                                const double* center, double* out, void* stream) {
+  if (!out) return fail(QLLES_ERR_ARG, "null argument");
+  if (dim != 2 && dim != 3) return fail(QLLES_ERR_SHAPE, "dim must be 2 or 3");
+  if (!(box > 0.0) || !std::isfinite(box)) return fail(QLLES_ERR_ARG, "box side must be positive");
+  if (T < 0 || np < 0) return fail(QLLES_ERR_ARG, "T and np must be >= 0");
+  const bool empty = T == 0 || np == 0;
+  if (!empty && (!r || !u)) return fail(QLLES_ERR_ARG, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  return guarded([&] {
+    if (!empty) {
+      const long long need = stats_grid(T, np).nb * 8 * dim;
+      if (need > (long long)QLLES_STATS_SCRATCH_DOUBLES)
+        throw std::invalid_argument("trajectory too large for qlles_stats_moments_noalloc: its partial sums need " +
+                                    std::to_string(need) + " doubles, the scratch array holds " +
+                                    std::to_string((long long)QLLES_STATS_SCRATCH_DOUBLES) +
+                                    " (use qlles_stats_moments)");
+    }
+    // every partial the final pass reads is written by this call's first
+    // kernel, so nothing needs clearing
+    launch_moments(dim, box, r, u, T, np, center, scratch_on_current_device(), out, empty, s);
   });
 }
 

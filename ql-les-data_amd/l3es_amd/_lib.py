@@ -37,7 +37,7 @@ EXPORTS = (
     "qlles_plan_tune", "qlles_energy_spectrum", "qlles_sph_create", "qlles_sph_destroy", "qlles_sph_density",
     "qlles_sph_relax_acc", "qlles_sph_forces", "qlles_sph_relax_advect", "qlles_sph_mls_grid", "qlles_sph_dft_hat",
     "qlles_step_record_bytes", "qlles_step_record", "qlles_step_vjp_record",
-    "qlles_stats_create", "qlles_stats_destroy", "qlles_stats_moments",
+    "qlles_stats_create", "qlles_stats_destroy", "qlles_stats_moments", "qlles_stats_moments_noalloc",
 )
 KINDS = ("x-pass", "y-inverse", "z-pass", "y-forward", "other")
 
@@ -125,6 +125,7 @@ def load():
         "qlles_stats_create": (ci, [ci, cd, ctypes.POINTER(vp)]),
         "qlles_stats_destroy": (ci, [vp]),
         "qlles_stats_moments": (ci, [vp, vp, vp, ctypes.c_longlong, ctypes.c_longlong, vp, vp, vp]),
+        "qlles_stats_moments_noalloc": (ci, [ci, cd, vp, vp, ctypes.c_longlong, ctypes.c_longlong, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
