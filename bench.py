@@ -35,7 +35,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_gpu(args, precision, steps, warmup, profile):
+def run_gpu(args, precision, steps, warmup, profile, repeats=1):
     import numpy as np
     import torch
 
@@ -63,11 +63,16 @@ def run_gpu(args, precision, steps, warmup, profile):
 
     advance(warmup)
     torch.cuda.synchronize()
-    # timed region: no per-kernel events (they cost a few per cent between launches)
-    t0 = time.perf_counter()
-    advance(steps)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
+    # timed regions: `repeats` times exactly `steps` steps, no per-kernel events (they cost a few per
+    # cent between launches); the reported rate is their median
+    rates = []
+    for _ in range(max(1, repeats)):
+        t0 = time.perf_counter()
+        advance(steps)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        rates.append(steps / (t1 - t0))
+    nufft = nufft_metrics(it, uh, r, N, precision, args.tol) if profile else None
     prof = None
     if profile:
         # a second timed region of the same length with HIP events around every launch
@@ -82,15 +87,69 @@ def run_gpu(args, precision, steps, warmup, profile):
         _lib.check(lib.qlles_profile_read(step.plan.handle, ms_a, cnt_a, by_a))
         _lib.check(lib.qlles_profile(step.plan.handle, 0))
         prof = {_lib.KINDS[k]: dict(ms=ms_a[k], count=cnt_a[k], bytes=by_a[k]) for k in range(5) if cnt_a[k]}
-    sps = steps / (t1 - t0)
+    sps = float(np.median(rates))
     u_fin = torch.empty((3, N, N, N), dtype=step.dtype, device=uh.device)
     _lib.check(lib.qlles_irfftn(step.plan.handle, _lib.ptr(uh), _lib.ptr(u_fin), _lib.stream_ptr()))
     e_fin = float(diagnostics(u_fin, step.plan)[0])
     finite = bool(torch.isfinite(ur).all()) and bool(np.isfinite(e_fin))
-    log(f"[bench] {precision}: {steps} steps in {t1 - t0:.3f}s -> {sps:.1f} steps/s, E={e_fin:.9f}")
+    log(f"[bench] {precision}: {len(rates)} x {steps} steps -> median {sps:.1f} steps/s "
+        f"(min {min(rates):.1f}, max {max(rates):.1f}), E={e_fin:.9f}")
     del uh, u_fin, step, it
     torch.cuda.empty_cache()
-    return sps, e_fin, prof, finite
+    reps = dict(n=len(rates), median=sps, min=min(rates), max=max(rates), steps_each=steps)
+    return sps, e_fin, prof, finite, reps, nufft
+
+
+def kernel_table(prof, traffic):
+    """Per-kernel-kind averages from the HIP-event profile and the roofline object of the dominant
+    kind (the most time).  traffic: (bytes per launch, detail) from the counter passes, or None."""
+    kernels = {}
+    for k, v in prof.items():
+        gbs = v["bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 else 0.0
+        kernels[k] = dict(avg_us=1e3 * v["ms"] / v["count"], launches=v["count"],
+                          alg_bytes_per_launch=v["bytes"] / v["count"], achieved_GBs=gbs,
+                          frac_of_peak=gbs / HBM_PEAK_GBS, share_of_time=0.0)
+    tot = sum(v["ms"] for v in prof.values())
+    for k, v in prof.items():
+        kernels[k]["share_of_time"] = v["ms"] / tot
+    dom = max(prof, key=lambda k: prof[k]["ms"])
+    a = kernels[dom]["achieved_GBs"]
+    tr, tinfo = traffic if traffic is not None else (None, "not measured for this run")
+    roof = dict(bound="hbm", kernel=dom, achieved=a, peak=HBM_PEAK_GBS, unit="GB/s", frac=a / HBM_PEAK_GBS,
+                traffic=tr, traffic_unit="bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+                traffic_detail=tinfo, alg_bytes_per_launch=kernels[dom]["alg_bytes_per_launch"],
+                avg_us=kernels[dom]["avg_us"])
+    return kernels, roof
+
+
+def nufft_metrics(it, uh, r, N, precision, tol, calls=50):
+    """Particles/s of the fused filter + type-2 NUFFT interpolation (qlles_interp_hat, no shift) on the
+    step's stream, timed with HIP events over `calls` calls, and its relative L2 error against the exact
+    direct-sum interpolation (the DFT backend, FP64) of the same filtered spectrum."""
+    import torch
+
+    from l3es_amd.integrator import spectral_interpolator_wrapper
+
+    n = it.n
+    ur = torch.empty((r.shape[0], 3), dtype=it.dtype, device=r.device)
+    stream = torch.cuda.current_stream()
+    it.from_hat(uh, r, 0.0, ur)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(calls):
+        it.from_hat(uh, r, 0.0, ur)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) * 1e-3 / calls
+    exact = spectral_interpolator_wrapper(n, (1, 2, 3), backend="dft", precision="float64")
+    ref = exact.from_hat(uh.to(torch.complex128), r, 0.0)
+    err = float(torch.linalg.vector_norm(ur.to(torch.float64) - ref) / torch.linalg.vector_norm(ref))
+    log(f"[bench] nufft {precision}: {r.shape[0]} particles on the {n}^3 window in {1e6 * sec:.1f} us "
+        f"({r.shape[0] / sec:.3e} particles/s), rel L2 err vs exact DFT {err:.2e}")
+    return dict(particles=int(r.shape[0]), window=n, us_per_call=1e6 * sec, particles_per_s=r.shape[0] / sec,
+                tol=tol, rel_l2_err_vs_exact_dft=err,
+                note="filter + type-2 NUFFT from the N^3 spectrum (qlles_interp_hat without the shift)")
 
 
 def _pmc_pass(counter, child_args, timeout):
@@ -325,6 +384,18 @@ def energy_trajectory(args, precision, nsamp=10, every=10):
     return out, final
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(args):
     """The FP64 NumPy/SciPy restatement (oracle) timed on this host's cores, bounded sample."""
     import numpy as np
@@ -347,7 +418,7 @@ def cpu_baseline(args):
     dt = time.perf_counter() - t0
     log(f"[bench] cpu: {n} steps in {dt:.1f}s")
     del u, uh, np
-    return dict(value=n / dt, unit="steps/s", cores=cores, kind="port",
+    return dict(value=n / dt, unit="steps/s", cores=cores, kind="port", cpu_model=cpu_model(),
                 sample=f"{n} RK4 step(s) of the FP64 oracle (NumPy + scipy.fft, workers={cores}) on the same "
                        f"{N}^3 power-forced HIT field after 1 warm-up step; particles excluded")
 
@@ -761,6 +832,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--repeats", type=int, default=5, help="timed regions of --steps steps; value = their median")
     ap.add_argument("--N", type=int, default=256)
     ap.add_argument("--ckp-N", dest="ckp_N", type=int, default=32)
     ap.add_argument("--nu", type=float, default=1e-3)
@@ -846,18 +918,24 @@ def main():
     import torch
 
     torch.cuda.set_device(0)
-    sps32, e32, prof, ok32 = run_gpu(args, "float32", args.steps, args.warmup, profile=True)
-    horizon = args.warmup + 2 * args.steps
+    sps32, e32, prof, ok32, reps32, nufft = run_gpu(args, "float32", args.steps, args.warmup, profile=True,
+                                                    repeats=args.repeats)
+    # the FP32 run advanced warmup + repeats x steps (timed) + steps (profiled)
+    horizon = args.warmup + (args.repeats + 1) * args.steps
     fp64 = None
     ke_err = None
     if not args.no_fp64:
         k64 = max(10, args.steps // 2)
-        # the FP64 run covers the same horizon (warmup + steps) for the energy comparison
-        # the FP32 run advanced warmup + steps (timed) + steps (profiled): same horizon in FP64
-        sps64, e64, _, ok64 = run_gpu(args, "float64", k64, horizon - k64, profile=False)
+        # the FP64 run covers the same horizon for the energy comparison: its warm-up, 3 timed repeats
+        # and one profiled region of k64 steps each
+        sps64, e64, prof64, ok64, reps64, nufft64 = run_gpu(args, "float64", k64, horizon - 4 * k64,
+                                                            profile=True, repeats=3)
         ke_err = abs(e32 - e64) / abs(e64)
+        k64s, roof64 = kernel_table(prof64, None)
         fp64 = dict(steps_per_s=sps64, ms_per_step=1000.0 / sps64, speedup_fp32_over_fp64=sps32 / sps64,
-                    e_kin=e64, finite=ok64)
+                    e_kin=e64, finite=ok64, repeats=reps64, roofline=roof64, kernels=k64s, nufft=nufft64,
+                    alg_bytes_per_step=sum(v["bytes"] for v in prof64.values()) / k64)
+        fp64["alg_frac"] = fp64["alg_bytes_per_step"] * sps64 / (HBM_PEAK_GBS * 1e9)
         t32, f32 = energy_trajectory(args, "float32")
         t64, f64 = energy_trajectory(args, "float64")
         traj_err = max(abs(a - b) / abs(b) for a, b in zip(t32, t64))
@@ -874,24 +952,11 @@ def main():
     C = 2 * s * N * N * (N // 2 + 1)
     R = s * N ** 3
     ledger = 211 * C + 3 * R  # algorithmic bytes per RK4 step, 3 axis passes per transform
-    kernels = {}
-    roof = None
+    kernels, roof = {}, None
     if prof:
-        for k, v in prof.items():
-            gbs = v["bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 else 0.0
-            kernels[k] = dict(avg_us=1e3 * v["ms"] / v["count"], launches=v["count"],
-                              alg_bytes_per_launch=v["bytes"] / v["count"], achieved_GBs=gbs,
-                              frac_of_peak=gbs / HBM_PEAK_GBS, share_of_time=0.0)
-        tot = sum(v["ms"] for v in prof.values())
-        for k, v in prof.items():
-            kernels[k]["share_of_time"] = v["ms"] / tot
-        dom = max(prof, key=lambda k: prof[k]["ms"])
-        a = kernels[dom]["achieved_GBs"]
         traffic, tinfo = (None, "skipped") if args.no_traffic else hbm_traffic(args, "k_ns3_x<")
-        roof = dict(bound="hbm", kernel=dom, achieved=a, peak=HBM_PEAK_GBS, unit="GB/s", frac=a / HBM_PEAK_GBS,
-                    traffic=traffic, traffic_unit="bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-                    traffic_detail=tinfo,
-                    alg_bytes_per_launch=kernels[dom]["alg_bytes_per_launch"], avg_us=kernels[dom]["avg_us"])
+        kernels, roof = kernel_table(prof, (traffic, tinfo))
+    alg_step = (sum(v["bytes"] for v in prof.values()) / args.steps) if prof else None
     cpu = None if args.no_cpu else cpu_baseline(args)
     out = {
         "metric": METRIC,
@@ -912,8 +977,13 @@ def main():
                    "particles": args.ckp_N ** 3, "nufft_tol": args.tol, "fft_precision": "f32",
                    "reductions": "f64", "positions": "f64"},
         "roofline": roof,
+        "repeats": reps32,
         "step_hbm": {"ledger_bytes_per_step": ledger, "ledger_frac": ledger * sps32 / (HBM_PEAK_GBS * 1e9),
-                     "alg_bytes_per_step": (sum(v["bytes"] for v in prof.values()) / args.steps) if prof else None},
+                     "alg_bytes_per_step": alg_step,
+                     "alg_frac": (alg_step * sps32 / (HBM_PEAK_GBS * 1e9)) if alg_step else None,
+                     "note": "ledger: the 211C + 3R bytes of a minimum-pass pipeline; alg: the bytes these "
+                             "kernels must move (fused x-passes, dealiased N-hat, one accumulator)"},
+        "nufft": nufft,
         "kernels": kernels,
         "fp64": fp64,
         "ke_rel_err_fp32_vs_fp64": ke_err,

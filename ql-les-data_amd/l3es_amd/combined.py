@@ -5,7 +5,8 @@ the ckp_N-filtered velocity at the particles, then r <- (r + dt u_r) mod 2 pi.
 Without relaxation the filter, interpolation and shift are one fused GPU
 call.  With relax_dt_factor > 0 the velocity is interpolated at r, then one
 fused call relaxes it and shifts r (combined.py:132-142, :199).  The density
-column rho_max comes from comp_rho(r) (combined.py:145-149).  Plotting is not
+column rho_max comes from comp_rho(r) (combined.py:145-149).  At the burn-in step the
+full-resolution field is written as u_{N}_{step:05d}_burnin.npy (combined.py:89-90).  Plotting is not
 part of this library; particle frames are written as .npz (no h5py).
 """
 
@@ -18,7 +19,7 @@ import torch
 
 from .integrator import set_up_integrator
 from .relax import relax_advect
-from .spectral_solver import diagnostics, rk4_wrapper, rhs_wrapper, set_up_solver
+from .spectral_solver import diagnostics, rk4_wrapper, rhs_wrapper, set_up_solver, write_u
 
 
 def combined(case, N=256, dim=3, nu=0.000625, t_final=0.1, t_burnin=0, dt=0.01, u_ref=4.0, ckp_N=64,
@@ -49,6 +50,8 @@ def combined(case, N=256, dim=3, nu=0.000625, t_final=0.1, t_burnin=0, dt=0.01, 
     e_acc = 0.0
     rows = []
     for i in range(tstep_max + 1):
+        if i == burnin_steps and dst_path:  # combined.py:89-90: the full-resolution field before step i
+            write_u(u_hat, i, dst_path, integrate_fn.plan, N, suffix="_burnin", u=u)
         if i == burnin_steps and case == "HIT":
             integrate_fn = rk4_wrapper(dt, rhs_wrapper(N, nu, fft_axes, precision), None, e_kin_init, "none",
                                        viscous_mode=viscous_mode)

@@ -239,16 +239,21 @@ def set_up_solver(N, nu, dim, case, dt, seed, ckp_N, kf, e_kin_target=None, forc
     return u, u_hat, xyz_vis, dx, integrate_fn, L, fft_axes, e_kin_init, e_inj
 
 
-def write_u(u_hat, i, dst_path, plan, ckp_N, suffix=""):
+def write_u(u_hat, i, dst_path, plan, ckp_N, suffix="", u=None):
     """Spectrally filtered velocity checkpoint u_{ckp_N}_{i:05d}{suffix}.npy (utils.py:141-166):
-    (3, n, n, n) in 3-D, (2, n, n) in 2-D, float64 like the reference's x64 arrays."""
+    (3, n, n, n) in 3-D, (2, n, n) in 2-D, float64 like the reference's x64 arrays.  With
+    ckp_N == N the reference writes the physical field itself (:161); so does this when ``u``
+    (the physical field of u_hat) is given."""
     from .utils import spectral_filtering_hat
 
     if ckp_N > plan.N or ckp_N % 2 or plan.N % 2:
         raise ValueError("ckp_N must be even and <= N (utils.py:157-158)")
     os.makedirs(dst_path, exist_ok=True)
-    u_lres = spectral_filtering_hat(u_hat, ckp_N, plan)
-    arr = u_lres.double().cpu().numpy()
+    if u is not None and ckp_N == plan.N:
+        arr = u.double().cpu().numpy()
+        arr = arr if plan.dim == 3 else arr[:2, :, :, 0]
+    else:
+        arr = spectral_filtering_hat(u_hat, ckp_N, plan).double().cpu().numpy()
     if plan.dim == 2:
         arr = arr[:2]
     np.save(os.path.join(dst_path, f"u_{ckp_N}_{i:05d}{suffix}.npy"), arr)
@@ -300,13 +305,39 @@ def simulate(case="TGV3D", N=64, dim=3, nu=0.000625, t_final=0.1, t_burnin=0.0, 
                       f"dt_est = {dt_est:.5f}, E_inj = {e_inj:.3f}")
         if write_checkpoints and dst_path:
             if i == burnin_steps:
-                write_u(u_hat, i, dst_path, integrate_fn.plan, N, suffix="_burnin")
+                write_u(u_hat, i, dst_path, integrate_fn.plan, N, suffix="_burnin", u=u)
             if i % ckp_freq == 0 and i >= burnin_steps:
-                write_u(u_hat, i, dst_ckp, integrate_fn.plan, ckp_N)
+                write_u(u_hat, i, dst_ckp, integrate_fn.plan, ckp_N, u=u)
+    stats = None
+    if case == "HIT" and forcing_type != "none":  # spectral_solver.py:344-358
+        stats = hit_statistics(e_inj_acc, tstep_max, kf, N, nu, t_final, rows[-1][3])
+        if verbose:
+            print(f"Eddy turnover time =            {stats['eddy_turnover_time']:.3f}")
+            print(f"Number of eddy turnover times = {stats['eddy_turnover_times']:.3f}")
+            print(f"Average energy injected =       {stats['e_inj_mean']:.3f}")
+            print(f"eta=                            {stats['eta']:.3f}")
+            print(f"kmax*eta =                      {stats['kmax_eta']:.3f}")
+            print(f"Re_lambda =                     {stats['re_lambda']:.3f}")
     t_tot = time() - t0
     if verbose:
         print(f"t_tot = {t_tot:.3f}, t_sim = {t_sim:.3f}")
     e_kin = float(diagnostics(u, integrate_fn.plan)[0])
     if t_final == 0.1 and case == "TGV3D":  # spectral_solver.py:363-367
         assert round(e_kin - 0.124953117517, 2) == 0, f"e_kin = {e_kin}"
-    return dict(u=u, u_hat=u_hat, e_kin=e_kin, rows=rows, e_inj_acc=e_inj_acc, t_sim=t_sim)
+    return dict(u=u, u_hat=u_hat, e_kin=e_kin, rows=rows, e_inj_acc=e_inj_acc, t_sim=t_sim, hit_stats=stats)
+
+
+def hit_statistics(e_inj_acc, steps, kf, N, nu, t_final, e_kin):
+    """End-of-run statistics of forced HIT (spectral_solver.py:344-358), host arithmetic.
+
+    ``e_inj_acc`` is the injection rate summed over the loop, ``steps`` the last loop index
+    (the reference divides by it), ``e_kin`` the last logged kinetic energy.  Returns the mean
+    injection rate eps, the eddy turnover time 1 / (eps kf^2)^(1/3), the number of turnovers
+    in t_final, the Kolmogorov scale eta = (nu^3 / eps)^(1/4), kmax eta with kmax = sqrt(2) N / 3,
+    and Re_lambda = sqrt(20 E^2 / (3 nu eps)).
+    """
+    eps = e_inj_acc / steps
+    t_e = 1.0 / (eps * kf ** 2) ** (1.0 / 3.0)
+    eta = (nu ** 3 / eps) ** 0.25
+    return dict(e_inj_mean=eps, eddy_turnover_time=t_e, eddy_turnover_times=t_final / t_e, eta=eta,
+                kmax_eta=np.sqrt(2) * N / 3 * eta, re_lambda=float(np.sqrt(20 * e_kin ** 2 / (3 * nu * eps))))

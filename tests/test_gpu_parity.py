@@ -507,3 +507,63 @@ def test_simulate_driver_tgv(gpu, tmp_path):
     assert ck.shape == (3, 16, 16, 16) and ck.dtype == np.float64
     assert rel(ck, O.spectral_filtering(u, 16)) < 1e-12
     assert np.load(os.path.join(tmp_path, "u_32_00000_burnin.npy")).shape == (3, 32, 32, 32)
+
+
+def _oracle_hit_burnin(N, nu, dt, kf, e0, forced_steps, total_steps, log_freq):
+    """The reference's forced-HIT driver loop (spectral_solver.py:300-320): the set-up's warm-up step,
+    forced steps until the burn-in step, then steps without forcing; the injection rates summed, the
+    kinetic energy at the logged steps and the state after each step."""
+    u, uh, step, e_init, _ = O.set_up_solver(N, nu, 3, "HIT", dt, seed=42, kf=kf, e_kin_target=e0,
+                                             forcing_type="ekin_tot", warmup=True)
+    free = O.rk4_wrapper(dt, O.rhs_wrapper(N, nu), None, e_init, "none")
+    e_acc, e_log, states = 0.0, [], []
+    for i in range(total_steps):
+        u, uh, ei = (step if i < forced_steps else free)(u, uh)
+        e_acc += ei
+        if i % log_freq == 0:
+            e_log.append(O.kinetic_energy(u))
+        states.append(u)
+    return e_acc, e_log, states
+
+
+def test_simulate_hit_statistics_and_burnin(gpu, tmp_path):
+    """simulate's forced-HIT branch end to end: the forcing switched off at the burn-in step
+    (spectral_solver.py:305-309), the injection rates summed (:316), the full-resolution _burnin snapshot
+    after the burn-in step (:331-332) and the end-of-run statistics (:344-358) from them."""
+    import os
+
+    from l3es_amd import simulate
+    from l3es_amd.spectral_solver import hit_statistics
+
+    N, nu, dt, kf, e0 = 16, 0.01, 0.01, 3, 1.25
+    out = simulate("HIT", N=N, nu=nu, t_final=0.05, t_burnin=0.02, dt=dt, ckp_N=8, forcing_type="ekin_tot",
+                   e_kin_target=e0, kf=kf, log_freq=2, ckp_freq=1, dst_path=str(tmp_path), precision="float64",
+                   verbose=False)
+    e_acc, e_log, states = _oracle_hit_burnin(N, nu, dt, kf, e0, 2, 6, 2)
+    assert e_acc > 0
+    assert abs(out["e_inj_acc"] - e_acc) <= 1e-10 * e_acc
+    assert rel(host(out["u"]), states[-1]) < 1e-12
+    snap = np.load(os.path.join(tmp_path, "u_16_00002_burnin.npy"))
+    assert rel(snap, states[2]) < 1e-12
+    want = hit_statistics(e_acc, 5, kf, N, nu, 0.05, e_log[-1])
+    for k, v in want.items():
+        assert abs(out["hit_stats"][k] - v) <= 1e-10 * abs(v), k
+
+
+def test_combined_burnin_snapshot(gpu, tmp_path):
+    """combined writes the full-resolution field at the burn-in step before taking it
+    (combined.py:89-90), then advances without forcing."""
+    import os
+
+    from l3es_amd.combined import combined
+
+    N, nu, dt, kf, e0 = 16, 0.01, 0.01, 3, 1.25
+    out = combined("HIT", N=N, dim=3, nu=nu, t_final=0.04, t_burnin=0.02, dt=dt, ckp_N=8, interp_backend="nufft",
+                   forcing_type="ekin_tot", e_kin_target=e0, kf=kf, dst_path=str(tmp_path), precision="float64",
+                   log_freq=100, ckp_freq=100, verbose=False)
+    e_acc, _, states = _oracle_hit_burnin(N, nu, dt, kf, e0, 2, 5, 100)
+    snap = np.load(os.path.join(tmp_path, "u_16_00002_burnin.npy"))
+    assert snap.shape == (3, N, N, N)
+    assert rel(snap, states[1]) < 1e-12  # the state before step 2
+    assert rel(host(out["u"]), states[-1]) < 1e-12
+    assert abs(out["e_inj_acc"] - e_acc) <= 1e-10 * e_acc

@@ -60,3 +60,20 @@ def test_helpers_refuse_bad_spectra(fn):
             utils.comp_dissipation_rate(bad, 0.01)
         else:
             utils.kinetic_energy_hat(bad)
+
+
+def test_hit_statistics_closed_form():
+    """spectral_solver.py:344-358 with numbers chosen so every quantity is exact: mean injection
+    eps = 0.25 and kf = 2 give eps kf^2 = 1, a turnover time of 1; nu = 0.25 gives eta =
+    (nu^3 / eps)^(1/4) = 0.5; N = 48 gives kmax = 16 sqrt(2); E = 0.75 gives Re_lambda = sqrt(60)."""
+    import numpy as np
+
+    from l3es_amd.spectral_solver import hit_statistics
+
+    s = hit_statistics(e_inj_acc=2.5, steps=10, kf=2, N=48, nu=0.25, t_final=3.0, e_kin=0.75)
+    assert abs(s["e_inj_mean"] - 0.25) < 1e-15
+    assert abs(s["eddy_turnover_time"] - 1.0) < 1e-15
+    assert abs(s["eddy_turnover_times"] - 3.0) < 1e-14
+    assert abs(s["eta"] - 0.5) < 1e-15
+    assert abs(s["kmax_eta"] - 8 * np.sqrt(2)) < 1e-13
+    assert abs(s["re_lambda"] - np.sqrt(60)) < 1e-13
