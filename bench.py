@@ -866,14 +866,17 @@ def main():
     ap.add_argument("--adj-reps", dest="adj_reps", type=int, default=3)
     ap.add_argument("--adj-n", dest="adj_n", type=int, default=16)
     ap.add_argument("--batch-members", dest="batch_members", type=int, default=24)
-    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-child", action="store_true",
+                    help="run only the solver workload (no JSON line): the child of the counter passes, and the "
+                         "command profiled by rocprofv3 for the kernel traces")
+    ap.add_argument("--child-precision", dest="child_precision", choices=("float32", "float64"), default="float32")
     args = ap.parse_args()
 
-    if args.pmc_child:  # the FP32 solver workload alone, run under rocprofv3 by hbm_traffic()
+    if args.pmc_child:  # the solver workload alone (FP32 unless --child-precision), run under rocprofv3
         import torch
 
         torch.cuda.set_device(0)
-        run_gpu(args, "float32", args.steps, args.warmup, profile=False)
+        run_gpu(args, args.child_precision, args.steps, args.warmup, profile=False)
         return
 
     if args.copy_probe:

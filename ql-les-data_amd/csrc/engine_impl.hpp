@@ -190,7 +190,8 @@ class EngineT final : public Engine {
     prof_begin(s, &e0);
     with_zv([&](auto v) {
       constexpr int V = decltype(v)::value;
-      hipLaunchKernelGGL((k_ns3_z<T, N, V>), dim3(N * N / ZCfg<N, V>::P), dim3(ZCfg<N, V>::THREADS), 0, s, a);
+      hipLaunchKernelGGL((k_ns3_z<T, N, V>), dim3(N * N / Z3Cfg<T, N, V>::P), dim3(Z3Cfg<T, N, V>::THREADS), 0, s,
+                         a);
     });
     QL_LAUNCHED();
     const double R = (double)N * N * N * sizeof(T);

@@ -666,10 +666,19 @@ template <> struct ZPlan<64> { using R = Radices<4, 4, 4>; };
 template <int M, int V> struct ZPlanV { using R = typename ZPlan<M>::R; };
 template <int M> struct ZPlanV<M, 1> { using R = typename Plan<M>::R; };
 
-template <int N, int V = 0>
+// The 3-D z-pass in FP32 at N = 256 (M = 128) runs radices 8, 4, 4 forward
+// (4, 4, 8 inverse) on 16 lanes per pencil: every pass keeps all lanes busy,
+// where the 32-lane plan leaves half of them idle in its radix-8 pass.  That
+// pass is bound by vector-ALU issue, so the busier lanes pay for the lower
+// occupancy (2 waves per SIMD at 206 registers against 4): 146.6 -> 137.8 us
+// per launch.  FP64 (256 registers, 1 wave per SIMD) keeps the 32-lane plan.
+template <int M, int V, int SEL> struct ZPlanS { using R = typename ZPlanV<M, V>::R; };
+template <> struct ZPlanS<128, 0, 1> { using R = Radices<8, 4, 4>; };
+
+template <int N, int V = 0, int SEL = 0>
 struct ZCfg {
   static constexpr int M = N / 2;
-  using R = typename ZPlanV<M, V>::R;
+  using R = typename ZPlanS<M, V, SEL>::R;
   static constexpr int TP = M / R::at(0);  // lanes per pencil = last-pass butterflies of the c2r
   static constexpr int P0 = QL_ZTHREADS / TP;
   static constexpr int P = (N <= 64) ? (P0 > 8 ? 8 : P0) : (P0 > 16 ? 16 : P0);
@@ -721,9 +730,13 @@ __device__ __forceinline__ void r2c_post_store(cx<T>* dst, const cx<T>* pb, cons
 // the three products stay live.
 // state of one z-pass workgroup; the prefetch ring is two named buffers so
 // every index is a compile-time constant (no private-memory arrays)
+// plan selector of the 3-D z-pass (SEL of ZCfg): 1 in FP32
+template <typename T> struct ZSel { static constexpr int value = sizeof(T) == 4 ? 1 : 0; };
+template <typename T, int N, int V> using Z3Cfg = ZCfg<N, V, ZSel<T>::value>;
+
 template <typename T, int N, int V = 0>
 struct ZState {
-  using C = ZCfg<N, V>;
+  using C = Z3Cfg<T, N, V>;
   static constexpr int M = C::M, TP = C::TP, P = C::P, Nh = M + 1, NT = C::THREADS;
   static constexpr int ZP = C::ZPAD;
   // every pencil's TP lanes lie in one wavefront, and each wavefront loads
@@ -893,7 +906,7 @@ __device__ __forceinline__ void ns3_z_body(const NS3Z<T>& a, cx<T>* s_tw, cx<T>*
   // factor of u x omega (u given in physical space carries its own)
   const double n3 = (double)N * N * N;
   const T h = (T)(UIN ? 0.5 / n3 : 0.5 / (n3 * n3));
-  cx<T>* pb = s_buf + z.p * ZCfg<N, V>::LSR;
+  cx<T>* pb = s_buf + z.p * Z3Cfg<T, N, V>::LSR;
 #pragma unroll
   for (int cc = 0; cc < 3; ++cc) {
     cx<T> v[CF::MAXE];
@@ -910,13 +923,13 @@ __device__ __forceinline__ void ns3_z_body(const NS3Z<T>& a, cx<T>* s_tw, cx<T>*
 }
 
 template <typename T, int N, int V = 0>
-__global__ void __launch_bounds__((ZCfg<N, V>::THREADS)) QL_ATTR_Z
+__global__ void __launch_bounds__((Z3Cfg<T, N, V>::THREADS)) QL_ATTR_Z
 k_ns3_z(NS3Z<T> a) {
   using St = ZState<T, N, V>;
   __shared__ cx<T> s_tw[N];
   __shared__ cx<T> s_twi[St::CI::TWL > 0 ? St::CI::TWL : 1];
   __shared__ cx<T> s_twf[St::CF::TWL > 0 ? St::CF::TWL : 1];
-  __shared__ cx<T> s_buf[ZCfg<N, V>::P * ZCfg<N, V>::LSR];
+  __shared__ cx<T> s_buf[Z3Cfg<T, N, V>::P * Z3Cfg<T, N, V>::LSR];
   // the register prefetch ring is indexed by compile-time field numbers only
   if (a.u_in != nullptr) ns3_z_body<T, N, true, V>(a, s_tw, s_twi, s_twf, s_buf);
   else ns3_z_body<T, N, false, V>(a, s_tw, s_twi, s_twf, s_buf);

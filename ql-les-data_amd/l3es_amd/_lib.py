@@ -11,7 +11,8 @@ import os
 import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "lib", "libqlles.so"))
+# QLLES_LIB names another build of the same library (e.g. one compiled with other tuning macros)
+LIB_PATH = os.environ.get("QLLES_LIB") or os.path.normpath(os.path.join(_HERE, "..", "lib", "libqlles.so"))
 
 QLLES_OK = 0
 QLLES_ERR_ARG = 1
