@@ -47,6 +47,14 @@
 #ifndef QL_ZDEPTH32
 #define QL_ZDEPTH32 2
 #endif
+#ifndef QL_ZDEPTH64
+#define QL_ZDEPTH64 3
+#endif
+// FP64 z-pass at N = 256: 2 waves per SIMD requested explicitly, which lets
+// it keep its registers (204-206 VGPRs) instead of spilling to private memory
+#ifndef QL_Z64_WPE
+#define QL_Z64_WPE 2
+#endif
 #ifndef QL_WPE_YI
 #define QL_WPE_YI 0
 #endif
@@ -73,6 +81,12 @@ template <typename T, int N> struct WpeX { static constexpr int value = (sizeof(
 #endif
 #if QL_WPE_Z > 0
 #define QL_ATTR_Z QL_WPE_ATTR_(QL_WPE_Z)
+#elif QL_Z64_WPE > 0
+template <typename T, int N> struct ZWpe {
+  static constexpr bool on = sizeof(T) == 8 && N >= 256;
+  static constexpr int lo = on ? QL_Z64_WPE : 1, hi = on ? QL_Z64_WPE : 8;
+};
+#define QL_ATTR_Z __attribute__((amdgpu_waves_per_eu(ZWpe<T, N>::lo, ZWpe<T, N>::hi)))
 #else
 #define QL_ATTR_Z
 #endif
@@ -752,9 +766,10 @@ struct ZState {
   // lanes 0..PWV-1 from the contiguous Nyquist plane)
   static constexpr int NEM = (PWV * M + WL - 1) / WL;
   static constexpr int NE = NEM + 1;
-  // fields in flight: two in FP32; one in FP64, whose larger registers would
-  // otherwise push the ring into private memory
-  static constexpr int DEPTH = sizeof(T) == 4 ? QL_ZDEPTH32 : 1;
+  // fields in flight: two in FP32; three in FP64 at N = 256 (1/2/3 measured
+  // at 357.5, 354.1 and 346.7 us per launch: FP64 waits on memory more), one
+  // at the smaller grids, where more would spill
+  static constexpr int DEPTH = sizeof(T) == 4 ? QL_ZDEPTH32 : (N >= 256 ? QL_ZDEPTH64 : 1);
   static constexpr bool NTL = QL_NTL && NTBig<T, N>::value;
   const NS3Z<T>& a;
   cx<T>* s_tw;   // flat N-point table (c2r / r2c pre- and post-processing)
