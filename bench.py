@@ -332,6 +332,8 @@ def run_adjoint(args):
         t2 = time.perf_counter()
         return float(loss.detach()), t1 - t0, t2 - t1, x.grad, float(p.grad)
 
+    free, _ = torch.cuda.mem_get_info()
+    recorded = A.record_bytes(step) * K <= free // 4  # the integrator's own rule (adjoint._records)
     grad_eval()  # warm-up (workspace allocation)
     reps = [grad_eval() for _ in range(args.adj_reps)]
     fwd = min(r[1] for r in reps)
@@ -356,6 +358,8 @@ def run_adjoint(args):
                                f"kf={args.kf}); target = seeded perturbation of r_K; gradients with respect to "
                                f"the initial spectrum and P; checkpoints every {every} steps",
                    "N": N, "K": K, "checkpoint_every": every, "particles": n ** 3},
+        "mode": ("every step recorded in the forward pass (qlles_step_record), recorded adjoints in the backward"
+                 if recorded else f"checkpoints every {every} steps, segments replayed in the backward"),
         "forward_ms": 1e3 * fwd, "backward_ms": 1e3 * bwd, "backward_over_forward": bwd / fwd,
         "loss": reps[-1][0], "dL_dpower": gpow, "grad_finite": bool(torch.isfinite(grad).all()),
         "dot_product_rel_residual": dp,

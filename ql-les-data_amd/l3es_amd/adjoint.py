@@ -8,12 +8,14 @@ adjoint also returns the cotangent of the injection power).  They run in libqlle
 over the stored half-spectrum, which is torch's convention for gradients of complex tensors, and
 sum_x a.b for physical fields.
 
-``integrate(step, u_hat0, nsteps, every)`` is a torch.autograd.Function: its forward takes nsteps fused
-RK4 steps and keeps a checkpoint every ``every`` steps; its backward recomputes each segment from its
-checkpoint and applies the step adjoint in reverse.  When a segment's step records fit in a quarter of
-the free device memory, the recompute takes recorded steps (qlles_step_record: stage by stage, keeping
-the stage states and their physical u, omega) and the adjoint reads them (qlles_step_vjp_record)
-instead of recomputing the stages.  ``irfftn(u_hat, plan)`` is differentiable too.
+``integrate(step, u_hat0, nsteps, every)`` is a torch.autograd.Function.  When the records of all nsteps
+steps fit in a quarter of the free device memory (qlles_step_record: a step taken stage by stage that
+keeps its stage states and their physical u, omega; 278 MB per step at 128^3 FP32, so 50 steps take
+14 GB of the 288 GB), its forward records every step and its backward applies the recorded step
+adjoints (qlles_step_vjp_record) in reverse, with no recomputation.  Otherwise the forward takes nsteps
+fused RK4 steps and keeps a checkpoint every ``every`` steps, and the backward recomputes each segment
+from its checkpoint (with recorded steps when a segment's records fit) before applying its adjoints.
+The recorded step equals the fused one to rounding.  ``irfftn(u_hat, plan)`` is differentiable too.
 """
 
 import torch
@@ -108,16 +110,34 @@ def _step_adjoint(step, states, records, i, lam, dpow):
     return step_vjp(step, states[i], lam, dpow)
 
 
+def _recorded_forward(step, uh, nsteps, records, after=None):
+    """nsteps recorded steps from uh: the step inputs u_hat_0 .. u_hat_nsteps (new tensors); after(n, u_hat)
+    runs after step n (the particle advection)."""
+    states = [uh]
+    for n in range(nsteps):
+        uh = step_record(step, uh, records[n])
+        states.append(uh)
+        if after is not None:
+            after(n, uh)
+    return states
+
+
 class _Integrate(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, u_hat0, power, step, nsteps, every):
+    def forward(ctx, u_hat0, power, step, nsteps, every, grads):
         uh = u_hat0.detach().contiguous().clone()
+        ctx.step, ctx.nsteps, ctx.every = step, nsteps, every
+        ctx.records = _records(step, nsteps, uh) if grads else None  # every step recorded when all fit
+        if ctx.records is not None:
+            ctx.states = _recorded_forward(step, uh, nsteps, ctx.records)
+            ctx.ckpts = None
+            return ctx.states[-1].clone()
         ckpts = []
         for n in range(nsteps):
             if n % every == 0:
                 ckpts.append(uh.clone())
             _advance(step, uh)
-        ctx.step, ctx.nsteps, ctx.every, ctx.ckpts = step, nsteps, every, ckpts
+        ctx.ckpts = ckpts
         return uh
 
     @staticmethod
@@ -125,6 +145,12 @@ class _Integrate(torch.autograd.Function):
         step, nsteps, every = ctx.step, ctx.nsteps, ctx.every
         lam = grad.contiguous()
         dpow = torch.zeros(1, dtype=torch.float64, device=grad.device) if ctx.needs_input_grad[1] else None
+        if ctx.records is not None:
+            for i in reversed(range(nsteps)):
+                lam = step_vjp_record(step, ctx.states[i], ctx.records[i], lam, dpow)
+            ctx.states = ctx.records = None
+            gp = None if dpow is None else dpow.reshape(()).to(torch.float64)
+            return lam, gp, None, None, None, None
         records = _records(step, every, lam)
         for seg in reversed(range(len(ctx.ckpts))):
             start = seg * every
@@ -135,7 +161,7 @@ class _Integrate(torch.autograd.Function):
                 lam = _step_adjoint(step, states, records, i, lam, dpow)
         ctx.ckpts = None
         gp = None if dpow is None else dpow.reshape(()).to(torch.float64)
-        return lam, gp, None, None, None
+        return lam, gp, None, None, None, None
 
 
 def integrate(step, u_hat0, nsteps, every=10, power=None):
@@ -149,7 +175,8 @@ def integrate(step, u_hat0, nsteps, every=10, power=None):
         power = torch.zeros((), dtype=torch.float64, device=u_hat0.device)
     elif abs(float(power.detach()) - float(step.plan.desc.power)) > 1e-12 * max(1.0, abs(float(power.detach()))):
         raise ValueError("power must equal the plan's injection power (create a plan per value)")
-    return _Integrate.apply(u_hat0, power, step, int(nsteps), int(every))
+    grads = torch.is_grad_enabled() and (u_hat0.requires_grad or power.requires_grad)
+    return _Integrate.apply(u_hat0, power, step, int(nsteps), int(every), grads)
 
 
 class _Irfftn(torch.autograd.Function):
@@ -199,27 +226,45 @@ def interp_vjp(interp, u_hat, r, u_bar, lam=None, r_bar=None):
 
 class _IntegrateParticles(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, u_hat0, r0, power, step, interp, nsteps, every):
+    def forward(ctx, u_hat0, r0, power, step, interp, nsteps, every, grads):
         dt = float(step.plan.desc.dt)
         uh = u_hat0.detach().contiguous().clone()
         r = r0.detach().to(torch.float64).contiguous().clone()
-        ckpts, rs = [], [r.clone()]
+        rs = [r.clone()]
+        ctx.step, ctx.interp, ctx.nsteps, ctx.every, ctx.dt = step, interp, nsteps, every, dt
+        ctx.records = _records(step, nsteps, uh) if grads else None  # every step recorded when all fit
+
+        def advect(n, u_hat):
+            interp.from_hat(u_hat, r, dt)  # r <- (r + dt u_r) mod 2 pi, in place
+            rs.append(r.clone())
+
+        if ctx.records is not None:
+            ctx.states = _recorded_forward(step, uh, nsteps, ctx.records, advect)
+            ctx.ckpts, ctx.rs = None, rs
+            return ctx.states[-1].clone(), r
+        ckpts = []
         for n in range(nsteps):
             if n % every == 0:
                 ckpts.append(uh.clone())
             _advance(step, uh)
-            interp.from_hat(uh, r, dt)  # r <- (r + dt u_r) mod 2 pi, in place
-            rs.append(r.clone())
-        ctx.step, ctx.interp, ctx.nsteps, ctx.every, ctx.dt = step, interp, nsteps, every, dt
+            advect(n, uh)
         ctx.ckpts, ctx.rs = ckpts, rs
         return uh, r
 
     @staticmethod
     def backward(ctx, g_uh, g_r):
         step, interp, nsteps, every, dt = ctx.step, ctx.interp, ctx.nsteps, ctx.every, ctx.dt
-        lam = torch.zeros_like(ctx.ckpts[0]) if g_uh is None else g_uh.contiguous().clone()
+        like = ctx.states[0] if ctx.records is not None else ctx.ckpts[0]
+        lam = torch.zeros_like(like) if g_uh is None else g_uh.contiguous().clone()
         lr = torch.zeros_like(ctx.rs[0]) if g_r is None else g_r.to(torch.float64).contiguous().clone()
         dpow = torch.zeros(1, dtype=torch.float64, device=lam.device) if ctx.needs_input_grad[2] else None
+        if ctx.records is not None:
+            for n in reversed(range(nsteps)):
+                lam, lr = interp_vjp(interp, ctx.states[n + 1], ctx.rs[n], dt * lr, lam, lr.clone())
+                lam = step_vjp_record(step, ctx.states[n], ctx.records[n], lam, dpow)
+            ctx.states = ctx.records = ctx.rs = None
+            gp = None if dpow is None else dpow.reshape(()).to(torch.float64)
+            return lam, lr, gp, None, None, None, None, None
         records = _records(step, every, lam)
         for seg in reversed(range(len(ctx.ckpts))):
             start = seg * every
@@ -231,7 +276,7 @@ class _IntegrateParticles(torch.autograd.Function):
                 lam = _step_adjoint(step, states, records, n - start, lam, dpow)
         ctx.ckpts = ctx.rs = None
         gp = None if dpow is None else dpow.reshape(()).to(torch.float64)
-        return lam, lr, gp, None, None, None, None
+        return lam, lr, gp, None, None, None, None, None
 
 
 def integrate_particles(step, interp, u_hat0, r0, nsteps, every=10, power=None):
@@ -243,7 +288,8 @@ def integrate_particles(step, interp, u_hat0, r0, nsteps, every=10, power=None):
         power = torch.zeros((), dtype=torch.float64, device=u_hat0.device)
     elif abs(float(power.detach()) - float(step.plan.desc.power)) > 1e-12 * max(1.0, abs(float(power.detach()))):
         raise ValueError("power must equal the plan's injection power (create a plan per value)")
-    return _IntegrateParticles.apply(u_hat0, r0, power, step, interp, int(nsteps), int(every))
+    grads = torch.is_grad_enabled() and (u_hat0.requires_grad or r0.requires_grad or power.requires_grad)
+    return _IntegrateParticles.apply(u_hat0, r0, power, step, interp, int(nsteps), int(every), grads)
 
 
 def periodic_diff(a, b):

@@ -289,8 +289,9 @@ def test_recorded_step_and_its_adjoint(gpu, prec, tol, power):
 
 @pytest.mark.gpu
 def test_gradient_with_and_without_step_records(gpu, monkeypatch):
-    """The checkpointed gradient is the same whether the segments are replayed with recorded steps or
-    with fused steps and recomputed stages."""
+    """The gradient is the same whether every step is recorded in the forward pass, or the forward
+    checkpoints and the backward replays each segment with recorded steps, or with fused steps and
+    recomputed stages."""
     from l3es_amd import adjoint as A
     from l3es_amd.integrator import pos_init_cartesian
 
@@ -306,8 +307,13 @@ def test_gradient_with_and_without_step_records(gpu, monkeypatch):
         loss.backward()
         return x.grad.clone(), float(p.grad)
 
-    g1, p1 = grads()
+    g1, p1 = grads()  # every step recorded in the forward pass
+    real = A._records
+    # only a segment's records fit: the forward checkpoints, the backward replays each segment recorded
+    monkeypatch.setattr(A, "_records", lambda step, n, like: None if n > 3 else real(step, n, like))
+    g3, p3 = grads()
     monkeypatch.setattr(A, "_records", lambda step, every, like: None)
-    g2, p2 = grads()
-    assert float(torch.linalg.vector_norm(g1 - g2)) < 1e-10 * float(torch.linalg.vector_norm(g2))
-    assert abs(p1 - p2) <= 1e-10 * abs(p2)
+    g2, p2 = grads()  # no records: fused replay, recomputed stages
+    for g, p in ((g1, p1), (g3, p3)):
+        assert float(torch.linalg.vector_norm(g - g2)) < 1e-10 * float(torch.linalg.vector_norm(g2))
+        assert abs(p - p2) <= 1e-10 * abs(p2)
