@@ -169,12 +169,12 @@ int qlles_rhs_jvp(qlles_plan* plan, const void* u_hat, const void* v, void* out,
 int qlles_rhs_vjp(qlles_plan* plan, const void* u_hat, const void* lam, void* out, double* dpower, void* stream);
 int qlles_step_jvp(qlles_plan* plan, const void* u_hat, const void* v, void* out, void* stream);
 int qlles_step_vjp(qlles_plan* plan, const void* u_hat, const void* lam, void* out, double* dpower, void* stream);
-/* Recorded steps for the recompute half of a checkpointed adjoint (no
- * reference counterpart).  qlles_step_record takes one classical RK4 step
- * u_hat_out = step(u_hat) stage by stage (four RHS evaluations;
- * spectral_solver.py:70-80), equal to qlles_step to rounding, and keeps its
- * three stage states and the physical u and omega of all four stage states in
- * `record` (device memory of qlles_step_record_bytes; u_hat_out may be u_hat).
+/* Recorded steps for the adjoint of a multi-step integration (no reference
+ * counterpart).  qlles_step_record takes one step u_hat_out = step(u_hat)
+ * with the kernels of qlles_step (bit-identical to it; classical RK4,
+ * spectral_solver.py:70-80) and keeps its three stage states and the
+ * physical u and omega of all four stage states in `record` (device memory
+ * of qlles_step_record_bytes; u_hat_out may be u_hat).
  * qlles_step_vjp_record is qlles_step_vjp at that u_hat, reading the record
  * instead of recomputing the stages.  Same scope as qlles_step_vjp. */
 size_t qlles_step_record_bytes(const qlles_plan* plan);

@@ -750,28 +750,42 @@ class EngineT final : public Engine {
     const T* ph = reinterpret_cast<const T*>(sp + 3 * spec_elems());
     for (int i = 0; i < 8; ++i) fs[i] = ph + i * phys_elems();
   }
-  // u_hat_out = y0 + h/6 (k1 + 2 k2 + 2 k3 + k4) (spectral_solver.py:70-80),
-  // one RHS per stage; uhat_out may be uhat
+  // One fused RK4 step (the kernels and arithmetic of step()) that also
+  // keeps what the step adjoint reads: the stage states y1, y2, y3 go to the
+  // record instead of the internal stage buffers, and each stage's z-pass
+  // writes the physical u and omega of its state.  uhat_out may be uhat: the
+  // state is copied into P by the first launch and read from there.
   void step_record(const void* uhat, void* uhat_out, void* rec, hipStream_t s) override {
     check_adj();
     ensure_adj();
-    const T h = (T)d_.dt;
-    const cx<T>* y0 = static_cast<const cx<T>*>(uhat);
+    cx<T>* y0 = const_cast<cx<T>*>(static_cast<const cx<T>*>(uhat));
     const cx<T>* ys[3];
     const T* fs[8];
     record_views(rec, ys, fs);
-    cx<T>* y[3] = {const_cast<cx<T>*>(ys[0]), const_cast<cx<T>*>(ys[1]), const_cast<cx<T>*>(ys[2])};
-    T* f[8];
-    for (int i = 0; i < 8; ++i) f[i] = const_cast<T*>(fs[i]);
-    cx<T>*k = AS_[3], *acc = AS_[5];
-    rhs_fields(y0, k, f[0], f[1], s);
-    axpby2(y0, y0, k, y[0], acc, (T)1, h / 2, (T)1, h / 6, s);
-    rhs_fields(y[0], k, f[2], f[3], s);
-    axpby2(y0, acc, k, y[1], acc, (T)1, h / 2, (T)1, h / 3, s);
-    rhs_fields(y[1], k, f[4], f[5], s);
-    axpby2(y0, acc, k, y[2], acc, (T)1, h, (T)1, h / 3, s);
-    rhs_fields(y[2], k, f[6], f[7], s);
-    axpby(acc, k, static_cast<cx<T>*>(uhat_out), (T)1, h / 6, s);
+    NS3X<T> a = xargs3(y0, Nh, 0);
+    a.c = ctl(-1);
+    a.us = y0;
+    a.ld_us = Nh;
+    a.copy_dst = P;
+    a.ld_copy = LDM;
+    a.q_copy = qs(3);
+    a.do_inverse = 1;
+    x3(a, s);
+    for (int st = 0; st < 4; ++st) {
+      yinv3(6, s);
+      z3(nullptr, const_cast<T*>(fs[2 * st]), s, const_cast<T*>(fs[2 * st + 1]));
+      yfwd3(s);
+      a = xargs3(P, LDM, qs(3));
+      a.c = ctl(st);
+      a.us = st == 0 ? P : const_cast<cx<T>*>(ys[st - 1]);
+      a.ld_us = st == 0 ? LDM : Nh;
+      a.q_us = st == 0 ? qs(3) : 0;
+      a.uout = st < 3 ? const_cast<cx<T>*>(ys[st]) : static_cast<cx<T>*>(uhat_out);
+      a.ld_uout = Nh;
+      a.q_uout = 0;
+      a.do_inverse = st < 3;
+      x3(a, s);
+    }
   }
   void step_vjp_record(const void* uhat, const void* rec, const void* lam_in, void* out_v, double* dpow,
                        hipStream_t s) override {

@@ -65,7 +65,7 @@ def test_adjoint128_fp32_dot_products_and_records(gpu):
     nxt = A.step_record(step, uh, rec)
     ref = uh.clone()
     step.advance(None, ref, 1, None, use_u=False)
-    assert float(torch.linalg.vector_norm(nxt - ref)) < 2e-6 * float(torch.linalg.vector_norm(ref))
+    assert torch.equal(nxt, ref)  # the recorded step is the fused step
     dpr = torch.zeros(1, dtype=torch.float64, device=uh.device)
     vr = A.step_vjp_record(step, uh, rec, w, dpr)
     assert float(torch.linalg.vector_norm(vr - vw)) < 2e-5 * float(torch.linalg.vector_norm(vw))
