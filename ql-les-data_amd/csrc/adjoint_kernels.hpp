@@ -128,6 +128,107 @@ __global__ void k_cross2(const T* __restrict__ a, const T* __restrict__ b, const
   }
 }
 
+// The z-axis middle of the RHS adjoint, one pencil (x, y) per TP lanes:
+//   psi   = c2r_z of the three rows of `in` (the y- and x-inverse-transformed
+//           P D l / c, rows of N/2+1), without its 1/N^3 (folded into h)
+//   out1  = r2c_z(w x psi), out2 = r2c_z(psi x u)     (six rows of N/2+1)
+// with w, u the recorded physical omega and u of the state.  It replaces
+// c2r_contig + k_cross2x2 + r2c_contig: psi and the products never go to
+// HBM.  The inverse plan is the forward plan reversed, so psi's register
+// order is the forward transform's input order and the products are formed
+// in registers (each complex slot holds the even and odd z samples, hence
+// the products are taken per part).
+template <int N>
+struct AdjZCfg {
+  static constexpr int M = N / 2;
+  using R = typename Plan<M>::R;
+  static constexpr int TP = Plan<M>::TP;
+  static constexpr int P = (256 / TP) > 64 ? 64 : (256 / TP);
+  static constexpr int THREADS = P * TP;
+  static constexpr int LSR = M + 1 + LPAD;
+};
+
+template <typename T, int N>
+__global__ void __launch_bounds__(AdjZCfg<N>::THREADS)
+k_adj_z(const cx<T>* __restrict__ in, const T* __restrict__ w, const T* __restrict__ u, cx<T>* __restrict__ out,
+        const cx<T>* __restrict__ tw, T h) {
+  using C = AdjZCfg<N>;
+  constexpr int M = C::M, Nh = M + 1, TP = C::TP;
+  using CI = PFFT<T, false, M, TP, Reverse<typename C::R>>;
+  using CF = PFFT<T, true, M, TP, typename C::R>;
+  static_assert(CI::NOUT == CF::NIN, "inverse output order must be the forward input order");
+  constexpr int NO = CI::NOUT;
+  constexpr size_t rows = (size_t)N * N;
+  __shared__ cx<T> s_tw[N];
+  __shared__ cx<T> s_buf[C::P * C::LSR];
+  const int tid = threadIdx.x, p = tid / TP, j = tid % TP;
+  stage_table(s_tw, tw, N);
+  const size_t row = (size_t)blockIdx.x * C::P + p;  // rows is a multiple of P
+  cx<T>* pb = s_buf + p * C::LSR;
+  cx<T> psi[3][NO];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    __syncthreads();  // the previous transform's LDS scratch is free (and the table staged)
+    const cx<T>* src = in + (c * rows + row) * Nh;
+    for (int k = j; k <= M; k += TP) pb[k] = src[k];
+    __syncthreads();
+    cx<T> v[CI::MAXE];
+#pragma unroll
+    for (int s = 0; s < CI::NIN; ++s) {
+      cx<T> z = {0, 0};
+      if (CI::in_valid(j, s)) {
+        const int n = CI::in_index(j, s);
+        cx<T> xn = pb[n], xr = pb[M - n];
+        if (n == 0) {
+          xn.im = 0;
+          xr.im = 0;
+        }
+        xr = cconj(xr);
+        z = cadd(cadd(xn, xr), cmuli(cmulc(csub(xn, xr), s_tw[n])));
+      }
+      v[s] = z;
+    }
+    __syncthreads();
+    CI::run(v, pb, 1, j, s_tw, 2);
+#pragma unroll
+    for (int s = 0; s < NO; ++s) psi[c][s] = v[s];
+  }
+  // physical rows of omega / u as (even, odd) pairs in psi's register order
+  auto ld = [&](const T* f, int c, int s) {
+    return reinterpret_cast<const cx<T>*>(f + (c * rows + row) * N)[CI::out_index(j, s)];
+  };
+#pragma unroll
+  for (int o = 0; o < 6; ++o) {
+    const T* f = o < 3 ? w : u;
+    const int a1 = (o % 3 + 1) % 3, a2 = (o % 3 + 2) % 3;  // (x, y, z) -> (y, z), (z, x), (x, y)
+    cx<T> v[CF::MAXE];
+#pragma unroll
+    for (int s = 0; s < NO; ++s) {
+      const cx<T> f1 = ld(f, a1, s), f2 = ld(f, a2, s);
+      if (o < 3) {  // (w x psi)_o = w_a1 psi_a2 - w_a2 psi_a1
+        v[s] = {f1.re * psi[a2][s].re - f2.re * psi[a1][s].re, f1.im * psi[a2][s].im - f2.im * psi[a1][s].im};
+      } else {  // (psi x u)_o = psi_a1 u_a2 - psi_a2 u_a1
+        v[s] = {psi[a1][s].re * f2.re - psi[a2][s].re * f1.re, psi[a1][s].im * f2.im - psi[a2][s].im * f1.im};
+      }
+    }
+    __syncthreads();
+    CF::run(v, pb, 1, j, s_tw, 2);
+#pragma unroll
+    for (int s = 0; s < CF::NOUT; ++s)
+      if (CF::out_valid(j, s)) pb[CF::out_index(j, s)] = v[s];
+    __syncthreads();
+    cx<T>* dst = out + (o * rows + row) * Nh;
+    for (int k = j; k <= M; k += TP) {
+      const cx<T> zk = pb[k == M ? 0 : k];
+      const cx<T> zr = cconj(pb[k == 0 ? 0 : M - k]);
+      const cx<T> e = cscale(cadd(zk, zr), h);
+      const cx<T> od = crot<T, true>(cscale(csub(zk, zr), h));  // (zk - zr) / (2i), times 2h
+      const cx<T> wk = k == M ? cx<T>{-1, 0} : s_tw[k];
+      dst[k] = cadd(e, cmul(wk, od));
+    }
+  }
+}
+
 // both products of the RHS adjoint in one pass: out1 = w x p, out2 = p x u
 template <typename T>
 __global__ void k_cross2x2(const T* __restrict__ w, const T* __restrict__ p, const T* __restrict__ u,

@@ -379,13 +379,21 @@ class EngineT final : public Engine {
     }
     hipLaunchKernelGGL((k_pd<T>), dim3(nblk(rows_ * Nh)), dim3(256), 0, s, l, AS_[8], N, KZD, 1);
     QL_LAUNCHED();
-    to_phys(AS_[8], AP_[2], s);  // psi = irfftn(P D l / c)
+    // psi = irfftn(P D l / c) along x and y; along z, fused with both products
+    // (w x psi, psi x u) and their forward transforms along z (k_adj_z)
+    c2c_strided<T>(N, AS_[8], AS_[9], 3, N * Nh, false, (T)1, twN_, s);
+    c2c_strided<T>(N, AS_[9], AS_[9], 3 * N, Nh, false, (T)1, twN_, s);
     {
-      const size_t n = (size_t)N * N * N;  // w x psi and psi x u, psi read once
-      hipLaunchKernelGGL((k_cross2x2<T>), dim3(nblk(n)), dim3(256), 0, s, wp, AP_[2], up, AP_[3], AP_[4], n);
+      using ZC = AdjZCfg<N>;
+      static_assert((N * N) % ZC::P == 0, "pencils per workgroup must divide N^2");
+      const T h = (T)(0.5 / ((double)N * N * N));  // r2c's 1/2 times the inverse's 1/N^3
+      hipLaunchKernelGGL((k_adj_z<T, N>), dim3(N * N / ZC::P), dim3(ZC::THREADS), 0, s, AS_[9], wp, up, AS_[7],
+                         twN_, h);
       QL_LAUNCHED();
     }
-    to_spec(AP_[3], AS_[7], s, 6);  // AP_[4] follows AP_[3], AS_[8] follows AS_[7]
+    // the six products' y and x forward transforms (AS_[8] follows AS_[7])
+    c2c_strided<T>(N, AS_[7], AS_[7], 6 * N, Nh, true, (T)1, twN_, s);
+    c2c_strided<T>(N, AS_[7], AS_[7], 6, N * Nh, true, (T)1, twN_, s);
     hipLaunchKernelGGL((k_vjp_combine<T>), dim3(nblk(rows_ * Nh)), dim3(256), 0, s, AS_[7], AS_[8], l, out, N, KZD,
                        (T)d_.nu);
     QL_LAUNCHED();
