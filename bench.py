@@ -323,7 +323,7 @@ def run_adjoint(args):
         x = uh0.clone().requires_grad_(True)
         p = torch.tensor(args.power, dtype=torch.float64, device=uh0.device, requires_grad=True)
         t0 = time.perf_counter()
-        _, r = A.integrate_particles(step, it, x, r0, K, every, power=p)
+        _, r = A.integrate_particles(step, it, x, r0, K, every, power=p, record=not args.adj_checkpoint)
         loss = torch.mean(A.periodic_diff(r, target) ** 2)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
@@ -333,7 +333,7 @@ def run_adjoint(args):
         return float(loss.detach()), t1 - t0, t2 - t1, x.grad, float(p.grad)
 
     free, _ = torch.cuda.mem_get_info()
-    recorded = A.record_bytes(step) * K <= free // 4  # the integrator's own rule (adjoint._records)
+    recorded = not args.adj_checkpoint and A.record_bytes(step) * K <= free // 4  # adjoint._records' rule
     grad_eval()  # warm-up (workspace allocation)
     reps = [grad_eval() for _ in range(args.adj_reps)]
     fwd = min(r[1] for r in reps)
@@ -869,6 +869,8 @@ def main():
     ap.add_argument("--adj-every", dest="adj_every", type=int, default=10)
     ap.add_argument("--adj-reps", dest="adj_reps", type=int, default=3)
     ap.add_argument("--adj-n", dest="adj_n", type=int, default=16)
+    ap.add_argument("--adj-checkpoint", dest="adj_checkpoint", action="store_true",
+                    help="adjoint workload: checkpoint every --adj-every steps instead of recording every step")
     ap.add_argument("--batch-members", dest="batch_members", type=int, default=24)
     ap.add_argument("--pmc-child", action="store_true",
                     help="run only the solver workload (no JSON line): the child of the counter passes, and the "

@@ -258,10 +258,13 @@ __global__ void k_cross2x2(const T* __restrict__ w, const T* __restrict__ p, con
 template <typename T>
 __global__ void __launch_bounds__(256) k_force_tangent(const cx<T>* __restrict__ y, const cx<T>* __restrict__ x,
                                                        cx<T>* __restrict__ out, int N, int kf2, double power,
-                                                       int mode, double* dpow) {
+                                                       int mode, double* dpow, int ldy) {
   __shared__ double s_red[3 * 256];
   const int Nh = N / 2 + 1;
   const size_t cs = (size_t)N * N * Nh;
+  // y may have rows of ldy entries (ldy = N/2: the internal split layout, whose
+  // kz < N/2 part holds every shell mode, kf < N/2)
+  const size_t cy_s = (size_t)N * N * ldy;
   int kf = 0;
   while ((kf + 1) * (kf + 1) <= kf2) ++kf;
   const int nx = 2 * kf + 1, nz = kf + 1, cnt = nx * nx * nz;
@@ -271,10 +274,11 @@ __global__ void __launch_bounds__(256) k_force_tangent(const cx<T>* __restrict__
     const int kx = q / (nx * nz) - kf, ky = (q / nz) % nx - kf, kz = q % nz;
     const int k2 = kx * kx + ky * ky + kz * kz;
     if (k2 == 0 || k2 > kf2) continue;
-    const size_t m = ((size_t)((kx + N) % N) * N + (ky + N) % N) * Nh + kz;
+    const size_t r = (size_t)((kx + N) % N) * N + (ky + N) % N;
+    const size_t m = r * Nh + kz, my = r * ldy + kz;
     const double w = kz == 0 ? 1.0 : 2.0;
     for (int c = 0; c < 3; ++c) {
-      const cx<T> a = y[c * cs + m], b = x[c * cs + m];
+      const cx<T> a = y[c * cy_s + my], b = x[c * cs + m];
       const double re = (double)a.re * b.re + (double)a.im * b.im;
       acc[0] += w * ((double)a.re * a.re + (double)a.im * a.im);
       acc[1] += w * re;
@@ -290,11 +294,12 @@ __global__ void __launch_bounds__(256) k_force_tangent(const cx<T>* __restrict__
     const int kx = q / (nx * nz) - kf, ky = (q / nz) % nx - kf, kz = q % nz;
     const int k2 = kx * kx + ky * ky + kz * kz;
     if (k2 == 0 || k2 > kf2) continue;
-    const size_t m = ((size_t)((kx + N) % N) * N + (ky + N) % N) * Nh + kz;
+    const size_t r = (size_t)((kx + N) % N) * N + (ky + N) % N;
+    const size_t m = r * Nh + kz, my = r * ldy + kz;
     const double w = kz == 0 ? 1.0 : 2.0;
     const T cy = mode == 0 ? (T)(-b * acc[1] / n6) : (T)(-b * acc[2] * w / n6);
     for (int c = 0; c < 3; ++c)
-      out[c * cs + m] = cadd(out[c * cs + m], cadd(cscale(x[c * cs + m], (T)a), cscale(y[c * cs + m], cy)));
+      out[c * cs + m] = cadd(out[c * cs + m], cadd(cscale(x[c * cs + m], (T)a), cscale(y[c * cy_s + my], cy)));
   }
 }
 

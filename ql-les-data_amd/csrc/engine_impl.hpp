@@ -364,14 +364,15 @@ class EngineT final : public Engine {
     QL_LAUNCHED();
     if (d_.forcing == QLLES_FORCING_POWER) {
       hipLaunchKernelGGL((k_force_tangent<T>), dim3(1), dim3(256), 0, s, y, d, out, N, d_.kf * d_.kf, d_.power, 0,
-                         nullptr);
+                         nullptr, Nh);
       QL_LAUNCHED();
     }
   }
   // out = R'(y)^T l  (temporaries AS_[7..9], AP_[0..4]); with fixed-power
   // forcing, *dpow (device, optional) accumulates <l, dR/dP>
+  // y has rows of ldy entries (Nh; N/2 for the split layout of a step record)
   void rhs_vjp_at(const cx<T>* y, const cx<T>* l, cx<T>* out, hipStream_t s, double* dpow = nullptr,
-                  const T* up = nullptr, const T* wp = nullptr) {
+                  const T* up = nullptr, const T* wp = nullptr, int ldy = Nh) {
     if (up == nullptr) {
       state_fields(y, AP_[0], AP_[1], s);
       up = AP_[0];
@@ -399,7 +400,7 @@ class EngineT final : public Engine {
     QL_LAUNCHED();
     if (d_.forcing == QLLES_FORCING_POWER) {
       hipLaunchKernelGGL((k_force_tangent<T>), dim3(1), dim3(256), 0, s, y, l, out, N, d_.kf * d_.kf, d_.power, 1,
-                         dpow);
+                         dpow, ldy);
       QL_LAUNCHED();
     }
   }
@@ -745,10 +746,12 @@ class EngineT final : public Engine {
     stage_states(y0, s);
     const cx<T>* ys[4] = {y0, AS_[0], AS_[1], AS_[2]};
     const T* fs[8] = {SF_[0], SF_[1], SF_[2], SF_[3], SF_[4], SF_[5], SF_[6], SF_[7]};
-    vjp_stages(ys, fs, static_cast<const cx<T>*>(lam_in), static_cast<cx<T>*>(out_v), dpow, s);
+    vjp_stages(ys, fs, static_cast<const cx<T>*>(lam_in), static_cast<cx<T>*>(out_v), dpow, s, Nh);
   }
-  // a step record: the stage states y1, y2, y3 (spectra), then u and omega of
-  // y0..y3 (physical), in the plan dtype
+  // a step record: the stage states y1, y2, y3 (spectra in the split layout of
+  // the step's internal fields: [3][N][N][N/2] rows, then the [3][N][N] plane
+  // of kz = N/2, 3 N^2 (N/2+1) entries each), then u and omega of y0..y3
+  // (physical), in the plan dtype
   size_t record_bytes() const override {
     return DIM == 3 ? 3 * spec_elems() * sizeof(cx<T>) + 8 * phys_elems() * sizeof(T) : 0;
   }
@@ -786,11 +789,11 @@ class EngineT final : public Engine {
       a = xargs3(P, LDM, qs(3));
       a.c = ctl(st);
       a.us = st == 0 ? P : const_cast<cx<T>*>(ys[st - 1]);
-      a.ld_us = st == 0 ? LDM : Nh;
-      a.q_us = st == 0 ? qs(3) : 0;
+      a.ld_us = LDM;
+      a.q_us = qs(3);
       a.uout = st < 3 ? const_cast<cx<T>*>(ys[st]) : static_cast<cx<T>*>(uhat_out);
-      a.ld_uout = Nh;
-      a.q_uout = 0;
+      a.ld_uout = st < 3 ? LDM : Nh;
+      a.q_uout = st < 3 ? qs(3) : 0;
       a.do_inverse = st < 3;
       x3(a, s);
     }
@@ -803,11 +806,12 @@ class EngineT final : public Engine {
     const T* fs[8];
     record_views(rec, yr, fs);
     const cx<T>* ys[4] = {static_cast<const cx<T>*>(uhat), yr[0], yr[1], yr[2]};
-    vjp_stages(ys, fs, static_cast<const cx<T>*>(lam_in), static_cast<cx<T>*>(out_v), dpow, s);
+    vjp_stages(ys, fs, static_cast<const cx<T>*>(lam_in), static_cast<cx<T>*>(out_v), dpow, s, LDM);
   }
   // backward through the four stages at the stage states ys with their u, omega fs
+  // ys[1..3] have rows of ld_st entries (ys[0] is the caller's state, rows of Nh)
   void vjp_stages(const cx<T>* const (&ys)[4], const T* const (&fs)[8], const cx<T>* lam_in, cx<T>* out,
-                  double* dpow, hipStream_t s) {
+                  double* dpow, hipStream_t s, int ld_st) {
     const T h = (T)d_.dt;
     // lam_in is only read, and read for the last time before out is written,
     // so out may alias it
@@ -821,7 +825,7 @@ class EngineT final : public Engine {
     axpby(lam, nullptr, kb, wl[0], wb[0], s);
     for (int q = 0; q < 4; ++q) {
       const int st = 3 - q;
-      rhs_vjp_at(ys[st], kb, yb, s, dpow, fs[2 * st], fs[2 * st + 1]);
+      rhs_vjp_at(ys[st], kb, yb, s, dpow, fs[2 * st], fs[2 * st + 1], st == 0 ? Nh : ld_st);
       const cx<T>* acc_in = q == 0 ? lam : acc;
       if (st == 0) axpby(acc_in, yb, out, (T)1, (T)1, s);
       else axpby2(lam, acc_in, yb, kb, acc, wl[q + 1], wb[q + 1], (T)1, (T)1, s);

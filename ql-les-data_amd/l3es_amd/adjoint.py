@@ -164,8 +164,9 @@ class _Integrate(torch.autograd.Function):
         return lam, gp, None, None, None, None
 
 
-def integrate(step, u_hat0, nsteps, every=10, power=None):
-    """nsteps RK4 steps of u_hat0, differentiable with checkpoints every ``every`` steps.
+def integrate(step, u_hat0, nsteps, every=10, power=None, record=True):
+    """nsteps RK4 steps of u_hat0, differentiable: every step recorded when all records fit (and
+    ``record``), else checkpoints every ``every`` steps.
 
     ``power``: optional 0-d float64 tensor equal to the plan's injection power; when it requires grad,
     backward returns dL/dP for a fixed-power plan (the forward uses the plan's own value)."""
@@ -175,7 +176,7 @@ def integrate(step, u_hat0, nsteps, every=10, power=None):
         power = torch.zeros((), dtype=torch.float64, device=u_hat0.device)
     elif abs(float(power.detach()) - float(step.plan.desc.power)) > 1e-12 * max(1.0, abs(float(power.detach()))):
         raise ValueError("power must equal the plan's injection power (create a plan per value)")
-    grads = torch.is_grad_enabled() and (u_hat0.requires_grad or power.requires_grad)
+    grads = record and torch.is_grad_enabled() and (u_hat0.requires_grad or power.requires_grad)
     return _Integrate.apply(u_hat0, power, step, int(nsteps), int(every), grads)
 
 
@@ -279,16 +280,17 @@ class _IntegrateParticles(torch.autograd.Function):
         return lam, lr, gp, None, None, None, None, None
 
 
-def integrate_particles(step, interp, u_hat0, r0, nsteps, every=10, power=None):
+def integrate_particles(step, interp, u_hat0, r0, nsteps, every=10, power=None, record=True):
     """The combined-mode loop for nsteps: returns (u_hat_K, r_K), differentiable with respect to u_hat0,
-    r0 and (fixed-power plans) the injection power, with checkpoints every ``every`` steps."""
+    r0 and (fixed-power plans) the injection power; every step recorded when all records fit (and
+    ``record``), else checkpoints every ``every`` steps."""
     if nsteps < 1 or every < 1:
         raise ValueError("need nsteps >= 1 and every >= 1")
     if power is None:
         power = torch.zeros((), dtype=torch.float64, device=u_hat0.device)
     elif abs(float(power.detach()) - float(step.plan.desc.power)) > 1e-12 * max(1.0, abs(float(power.detach()))):
         raise ValueError("power must equal the plan's injection power (create a plan per value)")
-    grads = torch.is_grad_enabled() and (u_hat0.requires_grad or r0.requires_grad or power.requires_grad)
+    grads = record and torch.is_grad_enabled() and (u_hat0.requires_grad or r0.requires_grad or power.requires_grad)
     return _IntegrateParticles.apply(u_hat0, r0, power, step, interp, int(nsteps), int(every), grads)
 
 
