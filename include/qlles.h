@@ -244,9 +244,12 @@ int qlles_batch_destroy(qlles_batch* b);
 int qlles_batch_add(qlles_batch* b, qlles_plan* plan, qlles_interp* it, void* u_hat, double* r, int np, void* u_r,
                     int* index);
 /* manual configuration: 1 <= lanes <= 16 streams, 1 <= chunk <= 1000 steps
- * per graph; single_graph != 0 captures every member's chunk into ONE graph
+ * per graph.  single_graph = 1 captures every member's chunk into ONE graph
  * (parallel branches over the lanes), launched once per chunk while every
- * member still needs a whole chunk */
+ * member still needs a whole chunk; single_graph = 2 captures every member's
+ * whole run (its own step count, uneven lengths included) into ONE graph, so
+ * each qlles_batch_run is a single graph launch (captured again when the
+ * step counts change; chunk is unused) */
 int qlles_batch_configure(qlles_batch* b, int lanes, int chunk, int single_graph);
 /* autotuner: for each member, its plan's kernel variants (qlles_plan_tune);
  * then single_graph in {0,1} x lanes in {1,2,4,8,16} x chunk in {1,4,16,64},

@@ -30,8 +30,10 @@ struct qlles_batch {
   std::vector<Member> m;
   int lanes = 1;  // streams the members are spread over (member i -> stream i % lanes)
   int chunk = 1;  // steps per graph launch
-  int single = 0;  // 1: one graph holds every member's chunk as parallel branches
+  int single = 0;  // 1: one graph holds every member's chunk as parallel branches;
+                   // 2: one graph holds every member's whole run (its own step count)
   hipGraphExec_t all = nullptr;
+  std::vector<int> all_n;  // mode 2: the step counts `all` was captured for
   std::vector<hipStream_t> streams;
   std::vector<hipEvent_t> join;
   hipEvent_t fork = nullptr;
@@ -51,6 +53,7 @@ void drop_graphs(qlles_batch* b) {
     (void)hipGraphExecDestroy(b->all);
     b->all = nullptr;
   }
+  b->all_n.clear();
   b->built = false;
 }
 
@@ -95,9 +98,9 @@ hipGraphExec_t capture(const qlles_batch::Member& x, void* uh, double* r, void* 
   return exec;
 }
 
-// one graph: the first stream forks to the others, every member's chunk is
+// one graph: the first stream forks to the others, member i's n[i] steps are
 // captured on its stream, and the streams join back
-hipGraphExec_t capture_all(qlles_batch* b) {
+hipGraphExec_t capture_all(qlles_batch* b, const std::vector<int>& n) {
   for (const auto& x : b->m)
     if (x.p->eng->profiling()) throw std::invalid_argument("disable qlles_profile on batch members");
   hipStream_t o = b->streams[0];
@@ -108,7 +111,8 @@ hipGraphExec_t capture_all(qlles_batch* b) {
     for (int l = 1; l < b->lanes; ++l) QL_HIP(hipStreamWaitEvent(b->streams[l], b->fork, 0));
     for (size_t i = 0; i < b->m.size(); ++i) {
       const auto& x = b->m[i];
-      qlapi::run_steps(x.p, x.it, nullptr, x.uh, b->chunk, x.r, x.np, x.ur, nullptr, b->streams[i % b->lanes]);
+      if (n[i] > 0)
+        qlapi::run_steps(x.p, x.it, nullptr, x.uh, n[i], x.r, x.np, x.ur, nullptr, b->streams[i % b->lanes]);
     }
     for (int l = 1; l < b->lanes; ++l) {
       QL_HIP(hipEventRecord(b->join[l], b->streams[l]));
@@ -131,21 +135,40 @@ void build(qlles_batch* b) {
   if (b->built) return;
   make_streams(b);
   drop_graphs(b);
-  for (size_t i = 0; i < b->m.size(); ++i) {
-    auto& x = b->m[i];
-    x.g = capture(x, x.uh, x.r, x.ur, b->chunk, b->streams[i % b->lanes]);
+  if (b->single != 2) {
+    for (size_t i = 0; i < b->m.size(); ++i) {
+      auto& x = b->m[i];
+      x.g = capture(x, x.uh, x.r, x.ur, b->chunk, b->streams[i % b->lanes]);
+    }
   }
-  if (b->single) b->all = capture_all(b);
+  if (b->single == 1) b->all = capture_all(b, std::vector<int>(b->m.size(), b->chunk));
   b->built = true;
 }
 
-// advance member i by n_i steps: whole chunks by graph replay, the rest eagerly
+// advance member i by n_i steps: whole chunks by graph replay, the rest
+// eagerly; in mode 2, one launch of the graph of the whole run (captured
+// again when the step counts change)
 void run(qlles_batch* b, const std::vector<int>& n, hipStream_t caller) {
   build(b);
+  if (b->single == 2 && (b->all == nullptr || b->all_n != n)) {
+    if (b->all) {
+      QL_HIP(hipStreamSynchronize(caller));  // a previous launch may still run
+      (void)hipGraphExecDestroy(b->all);
+      b->all = nullptr;
+    }
+    b->all = capture_all(b, n);
+    b->all_n = n;
+  }
   QL_HIP(hipEventRecord(b->fork, caller));
   for (hipStream_t s : b->streams) QL_HIP(hipStreamWaitEvent(s, b->fork, 0));
+  if (b->single == 2) {
+    QL_HIP(hipGraphLaunch(b->all, b->streams[0]));
+    QL_HIP(hipEventRecord(b->join[0], b->streams[0]));
+    QL_HIP(hipStreamWaitEvent(caller, b->join[0], 0));
+    return;
+  }
   std::vector<int> left(n);
-  if (b->single && b->all) {
+  if (b->single == 1 && b->all) {
     // whole chunks that every member still needs: one launch of the joint graph
     int common = *std::min_element(left.begin(), left.end()) / b->chunk;
     for (int c = 0; c < common; ++c) QL_HIP(hipGraphLaunch(b->all, b->streams[0]));
@@ -228,11 +251,12 @@ int qlles_batch_add(qlles_batch* b, qlles_plan* p, qlles_interp* it, void* u_hat
 int qlles_batch_configure(qlles_batch* b, int lanes, int chunk, int single_graph) {
   if (!b) return fail(QLLES_ERR_ARG, "null batch");
   if (lanes < 1 || lanes > 16 || chunk < 1 || chunk > 1000) return fail(QLLES_ERR_ARG, "need 1 <= lanes <= 16, 1 <= chunk <= 1000");
+  if (single_graph < 0 || single_graph > 2) return fail(QLLES_ERR_ARG, "single_graph must be 0, 1 or 2");
   return guarded([&] {
     drop_graphs(b);
     b->lanes = lanes;
     b->chunk = chunk;
-    b->single = single_graph ? 1 : 0;
+    b->single = single_graph;
   });
 }
 
@@ -296,6 +320,11 @@ int qlles_batch_tune(qlles_batch* b, int trial_steps, void* stream) {
     QL_HIP(hipEventCreate(&e0));
     QL_HIP(hipEventCreate(&e1));
     try {
+      // the whole-run graph (single = 2) is not a candidate: the trials have
+      // even lengths, where it times well, but members sharing a stream run
+      // their whole trajectories back to back, which uneven lengths turn into
+      // idle streams (24 uneven trajectories: 8357 against 9978
+      // trajectory-steps/s with the chunked joint graph)
       for (int single : {0, 1})
       for (int lanes : {1, 2, 4, 8, 16}) {
         if (lanes > nm && lanes > 1) break;

@@ -42,7 +42,10 @@ class Batch:
         return idx.value
 
     def configure(self, lanes, chunk, single_graph=False):
-        _lib.check(self._lib.qlles_batch_configure(self.handle, int(lanes), int(chunk), 1 if single_graph else 0))
+        """lanes streams, chunk steps per graph; single_graph False / True (one graph per chunk of every
+        member) / "whole" (one graph of every member's whole run: one launch per run)."""
+        mode = 2 if single_graph == "whole" else (1 if single_graph else 0)
+        _lib.check(self._lib.qlles_batch_configure(self.handle, int(lanes), int(chunk), mode))
 
     def tune(self, trial_steps=16):
         """Pick streams x steps-per-graph by timing on scratch copies (trajectories do not move)."""
@@ -53,7 +56,8 @@ class Batch:
         lanes, chunk, single, us = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         _lib.check(self._lib.qlles_batch_get_config(self.handle, ctypes.byref(lanes), ctypes.byref(chunk),
                                                     ctypes.byref(single), ctypes.byref(us)))
-        return dict(lanes=lanes.value, chunk=chunk.value, single_graph=bool(single.value), us_per_traj_step=us.value)
+        mode = {0: False, 1: True, 2: "whole"}[single.value]
+        return dict(lanes=lanes.value, chunk=chunk.value, single_graph=mode, us_per_traj_step=us.value)
 
     def run(self, nsteps):
         """Advance every member: ``nsteps`` an int (all the same) or a list (uneven lengths)."""

@@ -35,6 +35,7 @@ def test_batch_validation_without_device():
         assert lib.qlles_batch_configure(h, 0, 1, 0) == _lib.QLLES_ERR_ARG
         assert lib.qlles_batch_configure(h, 2, 0, 0) == _lib.QLLES_ERR_ARG
         assert lib.qlles_batch_configure(h, 2, 4, 1) == 0
+        assert lib.qlles_batch_configure(h, 2, 4, 3) == _lib.QLLES_ERR_ARG
         lanes, chunk, single, us = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         assert lib.qlles_batch_get_config(h, ctypes.byref(lanes), ctypes.byref(chunk), ctypes.byref(single),
                                           ctypes.byref(us)) == 0
@@ -101,13 +102,21 @@ def test_batch_bit_identical_to_solo(gpu):
     b.run(lengths)
     b.configure(lanes=3, chunk=2, single_graph=True)  # one joint graph for the common chunk
     b.run([4, 4, 6, 4])
+    b.configure(lanes=2, chunk=1, single_graph="whole")  # one graph of every member's whole run
+    b.run([3, 1, 2, 5])
+    b.run([3, 1, 2, 5])  # replayed
+    b.run([1, 2, 0, 1])  # captured again
     cfg = b.tune(trial_steps=4)  # runs on scratch copies: states must not move
     assert cfg["lanes"] in (1, 2, 4, 8, 16) and cfg["chunk"] in (1, 4, 16, 64) and cfg["us_per_traj_step"] > 0
+    assert cfg["single_graph"] in (False, True)
     b.run(5)
     torch.cuda.synchronize()
     for k, ((step, _, it, _), (uh_s, r_s), (uh_b, r_b), ur_b) in enumerate(zip(mem, solo, bat, urs)):
         ur1 = _solo(step, uh_s, it, r_s, lengths[k])
         ur1 = _solo(step, uh_s, it, r_s, [4, 4, 6, 4][k])
+        for n in ([3, 1, 2, 5][k], [3, 1, 2, 5][k], [1, 2, 0, 1][k]):
+            if n:
+                ur1 = _solo(step, uh_s, it, r_s, n)
         ur2 = _solo(step, uh_s, it, r_s, 5)
         torch.cuda.synchronize()
         assert np.array_equal(host(uh_s), host(uh_b)), f"member {k}: spectrum differs from solo run"
