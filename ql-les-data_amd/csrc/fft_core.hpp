@@ -291,7 +291,18 @@ __device__ __forceinline__ void lds_sync() {
   }
 }
 
-template <typename T, bool FWD, int M, int TP, typename RL, int PAD = 0, int SYNC = SYNC_BLOCK>
+// w^r for r = 1 .. R-1 from w (= w^1): squares for the even powers, one more
+// product for the odd ones, so no power is more than three products deep
+template <typename T, int R>
+__device__ __forceinline__ void tw_powers(cx<T> (&w)[R > 1 ? R : 2]) {
+#pragma unroll
+  for (int r = 2; r < R; ++r) w[r] = (r % 2 == 0) ? cmul(w[r / 2], w[r / 2]) : cmul(w[r - 1], w[1]);
+}
+
+// TWR: the twiddles of a butterfly are formed from its first one (one table
+// read instead of R - 1; a few multiplies more), which moves work from the
+// LDS pipe to the vector ALU
+template <typename T, bool FWD, int M, int TP, typename RL, int PAD = 0, int SYNC = SYNC_BLOCK, bool TWR = false>
 struct PFFT {
   static constexpr int NP = RL::n;
   static_assert(RL::total() == M, "radix plan does not multiply to M");
@@ -355,7 +366,13 @@ struct PFFT {
       const int b = j + TP * t;
       if constexpr (Ns > 1) {
         const int bm = b % Ns;
-        if (twstride == 0) {
+        if (twstride == 0 && TWR) {
+          cx<T> w[Rp > 1 ? Rp : 2];
+          w[1] = tw[tw_off(p) + bm * (Rp - 1)];
+          tw_powers<T, Rp>(w);
+#pragma unroll
+          for (int r = 1; r < Rp; ++r) v[t * Rp + r] = FWD ? cmul(v[t * Rp + r], w[r]) : cmulc(v[t * Rp + r], w[r]);
+        } else if (twstride == 0) {
           // per-plan layout (fill_tw)
           const cx<T>* tp = tw + tw_off(p) + bm * (Rp - 1) - 1;
 #pragma unroll

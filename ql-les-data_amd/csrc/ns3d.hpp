@@ -47,6 +47,15 @@
 #ifndef QL_ZDEPTH32
 #define QL_ZDEPTH32 2
 #endif
+// z-pass twiddles formed from one table read (bit 0: FFT passes, bit 1: c2r /
+// r2c pre/post).  FP32 256^3 step: 455 -> 471 steps/s with both (206 -> 168
+// VGPRs); FP64 212.4 -> 211.1-211.6 with either, so off there.
+#ifndef QL_ZTWR32
+#define QL_ZTWR32 3
+#endif
+#ifndef QL_ZTWR64
+#define QL_ZTWR64 0
+#endif
 #ifndef QL_ZDEPTH64
 #define QL_ZDEPTH64 3
 #endif
@@ -200,8 +209,23 @@ struct XCfg {
   static constexpr int THREADS = TIX * TP;
   static constexpr int LS = TIX + LPAD;
 };
-template <typename T, int N> using XFwd = PFFT<T, true, N, XCfg<N>::TP, typename XCfg<N>::R>;
-template <typename T, int N> using XInv = PFFT<T, false, N, XCfg<N>::TP, Reverse<typename XCfg<N>::R>>;
+// x / y-pass twiddles formed from one table read per butterfly: no gain at
+// 256^3 (FP32 470.1-471.0 against 470.8-471.1 steps/s; FP64 211.5-211.9
+// against 212.3-212.5), so off
+#ifndef QL_XTWR32
+#define QL_XTWR32 0
+#endif
+#ifndef QL_XTWR64
+#define QL_XTWR64 0
+#endif
+template <typename T>
+struct XTwr {
+  static constexpr bool value = sizeof(T) == 4 ? QL_XTWR32 : QL_XTWR64;
+};
+template <typename T, int N>
+using XFwd = PFFT<T, true, N, XCfg<N>::TP, typename XCfg<N>::R, 0, SYNC_BLOCK, XTwr<T>::value>;
+template <typename T, int N>
+using XInv = PFFT<T, false, N, XCfg<N>::TP, Reverse<typename XCfg<N>::R>, 0, SYNC_BLOCK, XTwr<T>::value>;
 
 __device__ __forceinline__ bool keep_k(int idx, int N, int KZD) { return idx < KZD || idx > N - KZD; }
 
@@ -701,11 +725,25 @@ struct ZCfg {
   static constexpr int LSR = M + (M >> ZPAD) + 1 + LPAD;
 };
 
+// W_N^{B0 r} a with N = 2M, r < R: a compile-time constant for each r (r is a
+// constant wherever the caller's loop is unrolled)
+template <typename T, int M, int B0, int R, int RR = 0>
+__device__ __forceinline__ cx<T> rot_const(cx<T> a, int r) {
+  if constexpr (RR + 1 >= R) {
+    return twc<T, true, B0 * RR, 2 * M>(a);
+  } else {
+    return r == RR ? twc<T, true, B0 * RR, 2 * M>(a) : rot_const<T, M, B0, R, RR + 1>(a, r);
+  }
+}
+
 // c2r pre-processing of one pencil held in LDS (pb[0..M]) into the pass-0
 // input slots of the M-point inverse FFT: Z'[n] = (X_n + X*_{M-n}) +
 // i W_N^{-n} (X_n - X*_{M-n}); Im of X_0 and X_M is ignored (numpy's irfft).
-template <typename T, int M, class CI, int PAD = 0>
+// With TWR, W_N^n for n = j + TP t + B0 r is W_N^{j + TP t} (one table read
+// per t) times the constant W_N^{B0 r}.
+template <typename T, int M, class CI, int PAD = 0, bool TWR = false>
 __device__ __forceinline__ void c2r_prep(cx<T>* v, const cx<T>* pb, const cx<T>* s_tw, int j) {
+  constexpr int R0 = CI::R(0), B0 = CI::B0;
 #pragma unroll
   for (int s = 0; s < CI::NIN; ++s) {
     cx<T> z = {0, 0};
@@ -717,7 +755,14 @@ __device__ __forceinline__ void c2r_prep(cx<T>* v, const cx<T>* pb, const cx<T>*
         xr.im = 0;
       }
       xr = cconj(xr);
-      z = cadd(cadd(xn, xr), cmuli(cmulc(csub(xn, xr), s_tw[n])));
+      cx<T> w;
+      if constexpr (TWR) {
+        const cx<T> wb = s_tw[n - B0 * (s % R0)];  // n - B0 r = j + TP t
+        w = rot_const<T, M, B0, R0>(wb, s % R0);
+      } else {
+        w = s_tw[n];
+      }
+      z = cadd(cadd(xn, xr), cmuli(cmulc(csub(xn, xr), w)));
     }
     v[s] = z;
   }
@@ -725,16 +770,20 @@ __device__ __forceinline__ void c2r_prep(cx<T>* v, const cx<T>* pb, const cx<T>*
 
 // r2c post-processing from the M-point forward FFT held in LDS (pb[0..M)):
 // X_k = h (Z_k + Z*_{M-k}) + W_N^k h (Z_k - Z*_{M-k})/i, k < KZD, where h is
-// 1/2 times whatever scale the caller folds into the output
-template <typename T, int M, int PAD = 0>
+// 1/2 times whatever scale the caller folds into the output.  With TWR,
+// W_N^k for k = j, j + TP, ... is read once and advanced by the constant W_N^TP.
+template <typename T, int M, int PAD = 0, bool TWR = false, int TPC = 1>
 __device__ __forceinline__ void r2c_post_store(cx<T>* dst, const cx<T>* pb, const cx<T>* s_tw, int j, int TP, int KZD,
                                                T h) {
+  cx<T> w = s_tw[j];
   for (int k = j; k < KZD; k += TP) {
     const cx<T> zk = pb[lds_at<PAD>(k)];
     const cx<T> zr = cconj(pb[lds_at<PAD>(k == 0 ? 0 : M - k)]);
     const cx<T> e = cscale(cadd(zk, zr), h);
     const cx<T> o = crot<T, true>(cscale(csub(zk, zr), h));
-    st_stream<QL_NT_Z && NTBig<T, 2 * M>::value>(&dst[k], cadd(e, cmul(s_tw[k], o)));
+    const cx<T> wk = TWR ? w : s_tw[k];
+    st_stream<QL_NT_Z && NTBig<T, 2 * M>::value>(&dst[k], cadd(e, cmul(wk, o)));
+    if constexpr (TWR) w = twc<T, true, TPC, 2 * M>(w);
   }
 }
 
@@ -758,8 +807,12 @@ struct ZState {
   static constexpr int WL = NT < 64 ? NT : 64;  // lanes of one wavefront of the block
   static constexpr int PWV = WL / TP;           // pencils per wavefront
   static_assert(WL % TP == 0 && P % PWV == 0, "z-pass pencils must not straddle wavefronts");
-  using CI = PFFT<T, false, M, TP, Reverse<typename C::R>, ZP, SYNC_WAVE>;  // last pass: TP butterflies
-  using CF = PFFT<T, true, M, TP, typename C::R, ZP, SYNC_WAVE>;           // pass 0 takes CI's output order
+  // twiddles formed from one table read (QL_ZTWR32 / QL_ZTWR64): bit 0 in the
+  // FFT passes (per butterfly), bit 1 in the c2r / r2c pre- and post-processing
+  static constexpr int TWM = sizeof(T) == 4 ? QL_ZTWR32 : QL_ZTWR64;
+  static constexpr bool TWR = (TWM & 2) != 0;
+  using CI = PFFT<T, false, M, TP, Reverse<typename C::R>, ZP, SYNC_WAVE, (TWM & 1) != 0>;  // last pass: TP butterflies
+  using CF = PFFT<T, true, M, TP, typename C::R, ZP, SYNC_WAVE, (TWM & 1) != 0>;  // pass 0 takes CI's output order
   static constexpr int NO = CI::NOUT;
   // prefetch slots per lane per field: the wavefront's PWV rows of M
   // contiguous entries, plus their Nyquist entries (one per pencil, taken by
@@ -832,7 +885,7 @@ struct ZState {
     if constexpr (F + DEPTH < 6) fetch(F + DEPTH, slot<F>(), row0);
     lds_sync<SYNC_WAVE>();
     cx<T> v[CI::MAXE];
-    c2r_prep<T, M, CI, ZP>(v, pb, s_tw, j);
+    c2r_prep<T, M, CI, ZP, TWR>(v, pb, s_tw, j);
     lds_sync<SYNC_WAVE>();
     CI::run(v, pb, 1, j, s_twi, 0);
     // (re, im) hold the (even, odd) z samples, still without the inverse
@@ -933,7 +986,8 @@ __device__ __forceinline__ void ns3_z_body(const NS3Z<T>& a, cx<T>* s_tw, cx<T>*
     for (int s = 0; s < CF::NOUT; ++s)
       if (CF::out_valid(z.j, s)) pb[lds_at<St::ZP>(CF::out_index(z.j, s))] = v[s];
     lds_sync<SYNC_WAVE>();
-    r2c_post_store<T, M, St::ZP>(a.nz + ((size_t)cc * N * N + z.row) * a.KZP, pb, s_tw, z.j, TP, a.KZD, h);
+    r2c_post_store<T, M, St::ZP, St::TWR, TP>(a.nz + ((size_t)cc * N * N + z.row) * a.KZP, pb, s_tw, z.j, TP, a.KZD,
+                                               h);
   }
 }
 
