@@ -233,9 +233,12 @@ int qlles_interp_hat(qlles_interp* it, int N, const void* u_hat, double* r, int 
  *   qlles_step_advect(plan, it, NULL, u_hat, nsteps[i], r, np, u_r, NULL, s)
  * (or qlles_step without particles) would alone: the same kernels in the
  * same order with fixed-order reductions, so every trajectory is
- * bit-identical to its solo run.  Whole chunks of steps replay a HIP graph
- * captured once per member; members are spread over `lanes` streams.
- * Members need their own plan and interpolator.  Not thread-safe. */
+ * bit-identical to its solo run.  Per-member modes replay HIP graphs of
+ * each member's steps over `lanes` streams; grouped modes step all members
+ * of one kind (dim, N, precision, viscous mode, kernel variants and
+ * interpolator settings; ekin_tot forcing runs alone) with one launch per
+ * kernel of the step for the whole group.  Members need their own plan and
+ * interpolator.  Not thread-safe. */
 typedef struct qlles_batch qlles_batch;
 int qlles_batch_create(qlles_batch** out);
 int qlles_batch_destroy(qlles_batch* b);
@@ -249,10 +252,18 @@ int qlles_batch_add(qlles_batch* b, qlles_plan* plan, qlles_interp* it, void* u_
  * member still needs a whole chunk; single_graph = 2 captures every member's
  * whole run (its own step count, uneven lengths included) into ONE graph, so
  * each qlles_batch_run is a single graph launch (captured again when the
- * step counts change; chunk is unused) */
+ * step counts change; chunk is unused).  single_graph = 3: grouped launches
+ * (each kernel of the step runs once per group of members of one kind; the
+ * groups' streams are interleaved `chunk` steps at a time), eager;
+ * single_graph = 4: the grouped run of every group captured into ONE graph,
+ * one launch per qlles_batch_run (captured again when the step counts
+ * change).  Grouped tables are remade when the order of the step counts
+ * within a group or a member's kernel variants change. */
 int qlles_batch_configure(qlles_batch* b, int lanes, int chunk, int single_graph);
 /* autotuner: for each member, its plan's kernel variants (qlles_plan_tune);
  * then single_graph in {0,1} x lanes in {1,2,4,8,16} x chunk in {1,4,16,64},
+ * single_graph = 3 x lanes x chunk in {1,4,16} and single_graph = 4 x lanes
+ * (lanes up to the number of groups),
  * timed over about trial_steps steps per member on scratch copies of the
  * member states (the trajectories do not move); keeps the fastest */
 int qlles_batch_tune(qlles_batch* b, int trial_steps, void* stream);

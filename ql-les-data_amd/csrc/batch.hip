@@ -1,12 +1,21 @@
 // Batched trajectories (many independent plans advanced together on one GPU).
 //
 // The reference runs one trajectory per process (a SLURM array, one GPU
-// each).  Small grids leave an MI355X mostly idle and launch-bound, so a batch
-// captures each member's chunk of steps into a HIP graph once and replays the
-// graphs on a few streams.  A member's kernels and their arguments are those
-// of qlles_step_advect / qlles_step on the same plan, and every reduction in
-// them runs in a fixed order, so each trajectory is bit-identical to its solo
-// run whatever the stream count, chunk length or interleaving.
+// each).  Small grids leave an MI355X mostly idle and launch-bound.  A batch
+// runs its members in one of two ways:
+//  * per member (modes 0-2): each member's chunk of steps is captured into a
+//    HIP graph once and the graphs are replayed on a few streams (mode 1: one
+//    joint graph per chunk; mode 2: one graph of every member's whole run);
+//  * grouped (modes 3-4): members of one kind (dim, N, precision, viscous
+//    mode, kernel variants, particle interpolator) form a group whose kernels
+//    run once per pass for all of its members (multi-member launches, grid
+//    z-slice = member, arguments from a device table), groups on their own
+//    streams; mode 3 launches eagerly, mode 4 captures the whole run of every
+//    group into one graph, one launch per run.
+// A member's kernels and their arguments are those of qlles_step_advect /
+// qlles_step on the same plan, and every reduction in them runs in a fixed
+// order, so each trajectory is bit-identical to its solo run whatever the
+// mode, stream count, chunk length or interleaving.
 #include <algorithm>
 #include <chrono>
 #include <vector>
@@ -33,7 +42,17 @@ struct qlles_batch {
   int single = 0;  // 1: one graph holds every member's chunk as parallel branches;
                    // 2: one graph holds every member's whole run (its own step count)
   hipGraphExec_t all = nullptr;
-  std::vector<int> all_n;  // mode 2: the step counts `all` was captured for
+  std::vector<int> all_n;  // modes 2, 4: the step counts `all` was captured for
+  // grouped modes: members in table order (step counts non-increasing), the
+  // engine group (nullptr: one member that cannot be grouped, run alone) and
+  // the grouped particle advection (nullptr: per-member interpolation)
+  struct Group {
+    std::vector<int> mem;
+    ql::EngineGroup* eg = nullptr;
+    ql::InterpGroup* ig = nullptr;
+  };
+  std::vector<Group> groups;
+  std::vector<int> group_perm;  // member order the groups were made for
   std::vector<hipStream_t> streams;
   std::vector<hipEvent_t> join;
   hipEvent_t fork = nullptr;
@@ -42,6 +61,18 @@ struct qlles_batch {
 };
 
 namespace {
+
+void drop_groups(qlles_batch* b) {
+  if (b->groups.empty()) return;
+  // group tables may still be read by launches in flight
+  for (hipStream_t s : b->streams) (void)hipStreamSynchronize(s);
+  for (auto& g : b->groups) {
+    delete g.eg;
+    ql::destroy_interp_group(g.ig);
+  }
+  b->groups.clear();
+  b->group_perm.clear();
+}
 
 void drop_graphs(qlles_batch* b) {
   for (auto& x : b->m)
@@ -55,6 +86,7 @@ void drop_graphs(qlles_batch* b) {
   }
   b->all_n.clear();
   b->built = false;
+  drop_groups(b);
 }
 
 void drop_streams(qlles_batch* b) {
@@ -131,11 +163,206 @@ hipGraphExec_t capture_all(qlles_batch* b, const std::vector<int>& n) {
   return exec;
 }
 
+// ---------------- grouped modes ----------------
+
+// how a member's particles are advected: 0 none, 1 a NUFFT that can join a
+// grouped one, 2 per member
+int adv_class(const qlles_interp* it) {
+  if (it == nullptr) return 0;
+  return ql::interp_groupable(it, it) ? 1 : 2;
+}
+
+// Groups for step counts n: members of one kind together (in the order they
+// were added), each group's members by decreasing step count (stable).  The
+// groups are made again when that order changes.
+std::vector<std::vector<int>> partition(const qlles_batch* b) {
+  const int M = (int)b->m.size();
+  std::vector<std::vector<int>> parts;
+  for (int i = 0; i < M; ++i) {
+    const auto& x = b->m[i];
+    bool placed = false;
+    for (auto& pt : parts) {
+      const auto& r = b->m[pt[0]];
+      if (!r.p->eng->groupable_with(x.p->eng)) continue;
+      const int cr = adv_class(r.it), cx = adv_class(x.it);
+      if (cr != cx || (cr == 1 && !ql::interp_groupable(r.it, x.it))) continue;
+      pt.push_back(i);
+      placed = true;
+      break;
+    }
+    if (!placed) parts.push_back({i});
+  }
+  return parts;
+}
+
+void make_groups(qlles_batch* b, const std::vector<int>& n) {
+  for (const auto& x : b->m)
+    if (x.p->eng->profiling()) throw std::invalid_argument("disable qlles_profile on batch members");
+  std::vector<std::vector<int>> parts = partition(b);
+  // the key: the member order, group by group, and every member's kernel
+  // variants (a group launches the variants of its first member)
+  std::vector<int> perm;
+  for (auto& pt : parts) {
+    std::stable_sort(pt.begin(), pt.end(), [&](int a, int c) { return n[a] > n[c]; });
+    perm.insert(perm.end(), pt.begin(), pt.end());
+    perm.push_back(-1);
+  }
+  for (const auto& x : b->m) {
+    int xv = 0, zv = 0;
+    x.p->eng->get_variant(&xv, &zv);
+    perm.push_back(xv);
+    perm.push_back(zv);
+  }
+  if (!b->groups.empty() && perm == b->group_perm) return;
+  if (b->all) {
+    // the whole-run graph reads the old groups' tables
+    for (hipStream_t s : b->streams) QL_HIP(hipStreamSynchronize(s));
+    (void)hipGraphExecDestroy(b->all);
+    b->all = nullptr;
+    b->all_n.clear();
+  }
+  drop_groups(b);
+  try {
+    for (const auto& pt : parts) {
+      qlles_batch::Group g;
+      g.mem = pt;
+      const auto& x0 = b->m[pt[0]];
+      if (x0.p->eng->groupable_with(x0.p->eng)) {
+        std::vector<ql::Engine*> es;
+        std::vector<void*> uh;
+        for (int i : pt) {
+          es.push_back(b->m[i].p->eng);
+          uh.push_back(b->m[i].uh);
+        }
+        g.eg = x0.p->eng->make_group(es, uh);
+        if (adv_class(x0.it) == 1) {
+          std::vector<qlles_interp*> its;
+          std::vector<ql::SpecRef> cont, last;
+          std::vector<double*> r;
+          std::vector<int> np;
+          std::vector<void*> ur;
+          std::vector<double> dt;
+          for (size_t j = 0; j < pt.size(); ++j) {
+            const auto& x = b->m[pt[j]];
+            its.push_back(x.it);
+            cont.push_back(g.eg->spec_cont((int)j));
+            last.push_back(g.eg->spec_last((int)j));
+            r.push_back(x.r);
+            np.push_back(x.np);
+            ur.push_back(x.ur);
+            dt.push_back(x.p->desc.dt);
+          }
+          try {
+            g.ig = ql::make_interp_group(its, x0.p->desc.N, cont, last, r, np, ur, dt);
+          } catch (...) {
+            delete g.eg;
+            throw;
+          }
+        }
+      }
+      b->groups.push_back(g);
+    }
+  } catch (...) {
+    drop_groups(b);
+    throw;
+  }
+  b->group_perm = perm;
+}
+
+// steps [from, to) of group g, whose members take ns[j] steps, on stream s
+void group_steps(qlles_batch* b, qlles_batch::Group& g, const std::vector<int>& ns, int from, int to, hipStream_t s) {
+  if (g.eg == nullptr) {  // a member of its own kind, stepped as qlles_step_advect does
+    const auto& x = b->m[g.mem[0]];
+    const int k = std::min(to, ns[0]) - from;
+    if (k > 0) qlapi::run_steps(x.p, x.it, nullptr, x.uh, k, x.r, x.np, x.ur, nullptr, s);
+    return;
+  }
+  std::function<void(hipStream_t, int, int)> after;
+  if (g.ig) {
+    after = [&g](hipStream_t st, int m, int f) { ql::interp_group_run(g.ig, m, f, st); };
+  } else if (b->m[g.mem[0]].it) {
+    after = [b, &g](hipStream_t st, int m, int f) {
+      for (int j = 0; j < m; ++j) {
+        const auto& x = b->m[g.mem[j]];
+        const ql::SpecRef sp = j < f ? g.eg->spec_cont(j) : g.eg->spec_last(j);
+        ql::interp_hat(x.it, x.p->desc.N, sp.p, sp.ld, sp.q, x.r, x.np, x.ur, x.p->desc.dt, st);
+      }
+    };
+  }
+  g.eg->steps(ns.data(), from, to, s, after);
+}
+
+// the grouped run of step counts n: group i on stream i % lanes, the groups
+// interleaved `chunk` steps at a time so every stream has queued work
+void enqueue_grouped(qlles_batch* b, const std::vector<int>& n) {
+  std::vector<std::vector<int>> ns(b->groups.size());
+  int maxn = 0;
+  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+    for (int i : b->groups[gi].mem) ns[gi].push_back(n[i]);
+    maxn = std::max(maxn, ns[gi][0]);
+  }
+  for (int k = 0; k < maxn; k += b->chunk)
+    for (size_t gi = 0; gi < b->groups.size(); ++gi)
+      if (ns[gi][0] > k)
+        group_steps(b, b->groups[gi], ns[gi], k, std::min(k + b->chunk, ns[gi][0]), b->streams[gi % b->lanes]);
+}
+
+hipGraphExec_t capture_grouped(qlles_batch* b, const std::vector<int>& n) {
+  hipStream_t o = b->streams[0];
+  QL_HIP(hipStreamBeginCapture(o, hipStreamCaptureModeThreadLocal));
+  hipGraph_t graph = nullptr;
+  try {
+    QL_HIP(hipEventRecord(b->fork, o));
+    for (int l = 1; l < b->lanes; ++l) QL_HIP(hipStreamWaitEvent(b->streams[l], b->fork, 0));
+    enqueue_grouped(b, n);
+    for (int l = 1; l < b->lanes; ++l) {
+      QL_HIP(hipEventRecord(b->join[l], b->streams[l]));
+      QL_HIP(hipStreamWaitEvent(o, b->join[l], 0));
+    }
+  } catch (...) {
+    (void)hipStreamEndCapture(o, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  QL_HIP(hipStreamEndCapture(o, &graph));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  QL_HIP(e);
+  return exec;
+}
+
+void run_grouped(qlles_batch* b, const std::vector<int>& n, hipStream_t caller) {
+  make_groups(b, n);  // outside any capture: it allocates and uploads tables
+  if (b->single == 4 && (b->all == nullptr || b->all_n != n)) {
+    if (b->all) {
+      QL_HIP(hipStreamSynchronize(caller));  // a previous launch may still run
+      (void)hipGraphExecDestroy(b->all);
+      b->all = nullptr;
+    }
+    b->all = capture_grouped(b, n);
+    b->all_n = n;
+  }
+  QL_HIP(hipEventRecord(b->fork, caller));
+  for (hipStream_t s : b->streams) QL_HIP(hipStreamWaitEvent(s, b->fork, 0));
+  if (b->single == 4) {
+    QL_HIP(hipGraphLaunch(b->all, b->streams[0]));
+    QL_HIP(hipEventRecord(b->join[0], b->streams[0]));
+    QL_HIP(hipStreamWaitEvent(caller, b->join[0], 0));
+    return;
+  }
+  enqueue_grouped(b, n);
+  for (int l = 0; l < b->lanes; ++l) {
+    QL_HIP(hipEventRecord(b->join[l], b->streams[l]));
+    QL_HIP(hipStreamWaitEvent(caller, b->join[l], 0));
+  }
+}
+
 void build(qlles_batch* b) {
   if (b->built) return;
   make_streams(b);
   drop_graphs(b);
-  if (b->single != 2) {
+  if (b->single < 2) {
     for (size_t i = 0; i < b->m.size(); ++i) {
       auto& x = b->m[i];
       x.g = capture(x, x.uh, x.r, x.ur, b->chunk, b->streams[i % b->lanes]);
@@ -150,6 +377,10 @@ void build(qlles_batch* b) {
 // again when the step counts change)
 void run(qlles_batch* b, const std::vector<int>& n, hipStream_t caller) {
   build(b);
+  if (b->single >= 3) {
+    run_grouped(b, n, caller);
+    return;
+  }
   if (b->single == 2 && (b->all == nullptr || b->all_n != n)) {
     if (b->all) {
       QL_HIP(hipStreamSynchronize(caller));  // a previous launch may still run
@@ -251,7 +482,7 @@ int qlles_batch_add(qlles_batch* b, qlles_plan* p, qlles_interp* it, void* u_hat
 int qlles_batch_configure(qlles_batch* b, int lanes, int chunk, int single_graph) {
   if (!b) return fail(QLLES_ERR_ARG, "null batch");
   if (lanes < 1 || lanes > 16 || chunk < 1 || chunk > 1000) return fail(QLLES_ERR_ARG, "need 1 <= lanes <= 16, 1 <= chunk <= 1000");
-  if (single_graph < 0 || single_graph > 2) return fail(QLLES_ERR_ARG, "single_graph must be 0, 1 or 2");
+  if (single_graph < 0 || single_graph > 4) return fail(QLLES_ERR_ARG, "single_graph must be 0 ... 4");
   return guarded([&] {
     drop_graphs(b);
     b->lanes = lanes;
@@ -325,17 +556,24 @@ int qlles_batch_tune(qlles_batch* b, int trial_steps, void* stream) {
       // their whole trajectories back to back, which uneven lengths turn into
       // idle streams (24 uneven trajectories: 8357 against 9978
       // trajectory-steps/s with the chunked joint graph)
-      for (int single : {0, 1})
+      // grouped modes: streams beyond one per group stay idle; chunk is the
+      // interleaving of the eager mode and has no effect on the graph
+      const int ngroups = (int)partition(b).size();
+      for (int single : {0, 1, 3, 4})
       for (int lanes : {1, 2, 4, 8, 16}) {
-        if (lanes > nm && lanes > 1) break;
+        if (lanes > (single >= 3 ? ngroups : nm) && lanes > 1) break;
         for (int chunk : {1, 4, 16, 64}) {
           if (chunk > trial_steps && chunk > 1) break;
+          if (single == 4 && chunk > 1) break;
+          if (single == 3 && chunk > 16) break;
           t.lanes = lanes;
           t.chunk = chunk;
           t.single = single;
           drop_graphs(&t);
           const int n = std::max(chunk, trial_steps / chunk * chunk);
-          run(&t, std::vector<int>(nm, chunk), caller);  // warm-up (also builds the graphs)
+          // warm-up (also builds the graphs; the whole-run graph of mode 4 for
+          // the timed step counts)
+          run(&t, std::vector<int>(nm, single == 4 ? n : chunk), caller);
           QL_HIP(hipEventRecord(e0, caller));
           run(&t, std::vector<int>(nm, n), caller);
           QL_HIP(hipEventRecord(e1, caller));

@@ -19,6 +19,37 @@ struct Desc {
   int viscous;
 };
 
+// Where a member's spectrum of u_{n+1} sits after a step of a group (see
+// EngineGroup): rows of ld complex entries, the kz = N/2 entries in column
+// N/2 (q == 0) or in a separate plane at element q.
+struct SpecRef {
+  const void* p;
+  int ld;
+  size_t q;
+};
+
+// Engines of one kind -- same dim, N, precision, viscous mode and kernel
+// variants, forcing other than ekin_tot -- stepped together: each kernel of
+// the RK4 step runs once for all members still stepping, z-slice m of its
+// grid running member m with the arguments of entry m of a device table.
+// Member order is fixed when the group is made; the step counts given to
+// steps() must not increase along it, so the members still stepping at any
+// step are a prefix.  Kernels, arguments and reduction orders are those of
+// Engine::step, so every member's trajectory is bit-identical to its own.
+class EngineGroup {
+ public:
+  virtual ~EngineGroup() {}
+  virtual int size() const = 0;
+  // steps [from, to) of a run in which member i takes n[i] steps; from == 0
+  // first loads the states.  after(s, m, f) follows each step: members
+  // [0, f) go on stepping, members [f, m) have just taken their last step.
+  virtual void steps(const int* n, int from, int to, hipStream_t s,
+                     const std::function<void(hipStream_t, int, int)>& after) = 0;
+  // member i's u_{n+1} after a step that is not (cont) / is (last) its last
+  virtual SpecRef spec_cont(int i) const = 0;
+  virtual SpecRef spec_last(int i) const = 0;
+};
+
 class Engine {
  public:
   virtual ~Engine() {}
@@ -41,6 +72,10 @@ class Engine {
   virtual int variants() const = 0;
   virtual void set_variant(int xv, int zv) = 0;
   virtual void get_variant(int* xv, int* zv) const = 0;
+  // whether o can join a group with this engine, and the group of this engine
+  // (first member) and others of its kind stepping the states uhat[i]
+  virtual bool groupable_with(const Engine* o) const = 0;
+  virtual EngineGroup* make_group(const std::vector<Engine*>& members, const std::vector<void*>& uhat) = 0;
   // tangent-linear and adjoint maps of the RHS and of one RK4 step at u_hat
   // (3-D, explicit viscous term, forcing none or fixed power); inputs and
   // outputs are [3][N][N][N/2+1] spectra and may not alias each other.

@@ -1,6 +1,7 @@
 // EngineT<T, DIM, N>: launch orchestration for one solver configuration.
 #pragma once
 #include <algorithm>
+#include <memory>
 #include <vector>
 
 #include "../../include/qlles.h"
@@ -110,20 +111,29 @@ class EngineT final : public Engine {
 
   // ---------------- 3-D launches ----------------
   // stage and viscous mode select a compile-time specialisation of the x-pass
-  template <int SG, bool VIF, int V>
-  static void launch_x3_t(NS3X<T>& a, dim3 g, hipStream_t s) {
-    hipLaunchKernelGGL((k_ns3_x<T, N, SG, VIF, V>), g, dim3(XCfg<N, V>::THREADS), 0, s, a);
+  // every launch passes its arguments as a Pack (ns3d.hpp): entry 0 for a
+  // launch of this plan alone, one entry per member for a group
+  template <class A>
+  static Pack<A> pack1(const A& a) {
+    Pack<A> pk{};
+    pk.m[0] = a;
+    return pk;
   }
+  template <int SG, bool VIF, int V>
+  static void launch_x3_t(const Pack<NS3X<T>>& pk, dim3 g, hipStream_t s) {
+    hipLaunchKernelGGL((k_ns3_x<T, N, SG, VIF, V>), g, dim3(XCfg<N, V>::THREADS), 0, s, pk);
+  }
+  // stage and viscous mode of entry 0 select the specialisation (all entries share them)
   template <int V>
-  static void launch_x3(NS3X<T>& a, dim3 g, hipStream_t s) {
-    const bool vif = a.c.visc_if != 0;
-    switch (a.c.stage) {
-      case -1: launch_x3_t<-1, false, V>(a, g, s); break;
-      case 0: vif ? launch_x3_t<0, true, V>(a, g, s) : launch_x3_t<0, false, V>(a, g, s); break;
-      case 1: vif ? launch_x3_t<1, true, V>(a, g, s) : launch_x3_t<1, false, V>(a, g, s); break;
-      case 2: vif ? launch_x3_t<2, true, V>(a, g, s) : launch_x3_t<2, false, V>(a, g, s); break;
-      case 3: vif ? launch_x3_t<3, true, V>(a, g, s) : launch_x3_t<3, false, V>(a, g, s); break;
-      case 4: vif ? launch_x3_t<4, true, V>(a, g, s) : launch_x3_t<4, false, V>(a, g, s); break;
+  static void launch_x3(const Pack<NS3X<T>>& pk, dim3 g, hipStream_t s) {
+    const bool vif = pk.m[0].c.visc_if != 0;
+    switch (pk.m[0].c.stage) {
+      case -1: launch_x3_t<-1, false, V>(pk, g, s); break;
+      case 0: vif ? launch_x3_t<0, true, V>(pk, g, s) : launch_x3_t<0, false, V>(pk, g, s); break;
+      case 1: vif ? launch_x3_t<1, true, V>(pk, g, s) : launch_x3_t<1, false, V>(pk, g, s); break;
+      case 2: vif ? launch_x3_t<2, true, V>(pk, g, s) : launch_x3_t<2, false, V>(pk, g, s); break;
+      case 3: vif ? launch_x3_t<3, true, V>(pk, g, s) : launch_x3_t<3, false, V>(pk, g, s); break;
+      case 4: vif ? launch_x3_t<4, true, V>(pk, g, s) : launch_x3_t<4, false, V>(pk, g, s); break;
       default: throw std::invalid_argument("bad stage");
     }
   }
@@ -166,7 +176,7 @@ class EngineT final : public Engine {
     dim3 g(xtiles(), N);
     hipEvent_t e0 = nullptr;
     prof_begin(s, &e0);
-    with_xv([&](auto v) { launch_x3<decltype(v)::value>(a, g, s); });
+    with_xv([&](auto v) { launch_x3<decltype(v)::value>(pack1(a), g, s); });
     QL_LAUNCHED();
     prof_end(s, K_XPASS, xpass_bytes(a.c, a.do_inverse, 3, 5) + (a.copy_dst ? 3.0 * C() : 0.0), e0);
   }
@@ -179,7 +189,7 @@ class EngineT final : public Engine {
     prof_begin(s, &e0);
     with_xv([&](auto v) {
       constexpr int V = decltype(v)::value;
-      hipLaunchKernelGGL((k_ns3_yinv<T, N, V>), g, dim3(XCfg<N, V>::THREADS), 0, s, a);
+      hipLaunchKernelGGL((k_ns3_yinv<T, N, V>), g, dim3(XCfg<N, V>::THREADS), 0, s, pack1(a));
     });
     QL_LAUNCHED();
     prof_end(s, K_YINV, (nf == 6 ? 11.0 : 6.0) * C(), e0);
@@ -191,7 +201,7 @@ class EngineT final : public Engine {
     with_zv([&](auto v) {
       constexpr int V = decltype(v)::value;
       hipLaunchKernelGGL((k_ns3_z<T, N, V>), dim3(N * N / Z3Cfg<T, N, V>::P), dim3(Z3Cfg<T, N, V>::THREADS), 0, s,
-                         a);
+                         pack1(a));
     });
     QL_LAUNCHED();
     const double R = (double)N * N * N * sizeof(T);
@@ -205,7 +215,7 @@ class EngineT final : public Engine {
     with_xv([&](auto v) {
       constexpr int V = decltype(v)::value;
       dim3 g((KZD + XCfg<N, V>::TIX - 1) / XCfg<N, V>::TIX, N);
-      hipLaunchKernelGGL((k_ns3_yfwd<T, N, V>), g, dim3(XCfg<N, V>::THREADS), 0, s, a);
+      hipLaunchKernelGGL((k_ns3_yfwd<T, N, V>), g, dim3(XCfg<N, V>::THREADS), 0, s, pack1(a));
     });
     QL_LAUNCHED();
     prof_end(s, K_YFWD, 3.0 * N * (N + KYD) * KZD * 2.0 * sizeof(T), e0);
@@ -235,7 +245,7 @@ class EngineT final : public Engine {
     with_xv([&](auto v) {
       constexpr int V = decltype(v)::value;
       hipLaunchKernelGGL((k_ns2_x<T, N, V>), dim3((Nh + XCfg<N, V>::TIX - 1) / XCfg<N, V>::TIX),
-                         dim3(XCfg<N, V>::THREADS), 0, s, a);
+                         dim3(XCfg<N, V>::THREADS), 0, s, pack1(a));
     });
     QL_LAUNCHED();
     prof_end(s, K_XPASS, xpass_bytes(a.c, a.do_inverse, 2, 3), e0);
@@ -247,7 +257,7 @@ class EngineT final : public Engine {
     with_zv([&](auto v) {
       constexpr int V = decltype(v)::value;
       hipLaunchKernelGGL((k_ns2_y<T, N, V>), dim3((N + ZCfg<N, V>::P - 1) / ZCfg<N, V>::P), dim3(ZCfg<N, V>::THREADS),
-                         0, s, a);
+                         0, s, pack1(a));
     });
     QL_LAUNCHED();
     const double R = (double)N * N * sizeof(T);
@@ -706,6 +716,210 @@ class EngineT final : public Engine {
   void get_variant(int* xv, int* zv) const override {
     if (xv) *xv = xv_;
     if (zv) *zv = zv_;
+  }
+
+  // ---------------- groups (multi-member launches) ----------------
+  // launches for members [0, m) of a group, Pack<A>::K members per launch;
+  // entry(i) gives member i's arguments
+  template <class A, class E, class L>
+  static void packs(int m, E&& entry, L&& launch) {
+    for (int lo = 0; lo < m; lo += Pack<A>::K) {
+      Pack<A> pk{};
+      const int c = std::min(Pack<A>::K, m - lo);
+      for (int i = 0; i < c; ++i) pk.m[i] = entry(lo + i);
+      launch(pk, c);
+    }
+  }
+
+  // The group holds, per member, the arguments step() passes each kernel:
+  // x-pass arguments for the state load (stage -1), the four stages with
+  // u_{n+1} written to the member's P (3-D) and stage 3 of the member's last
+  // step, which writes u_{n+1} to the caller's array and skips the inverse.
+  struct Group final : EngineGroup {
+    std::vector<EngineT*> e;
+    std::vector<cx<T>*> s0;
+    std::vector<NS3X<T>> x3[6];  // load, stages 0..3, stage 3 of the last step
+    std::vector<NS3Y<T>> yi;
+    std::vector<NS3Z<T>> zz;
+    std::vector<NS3YF<T>> yf;
+    std::vector<NS2X<T>> x2[6];
+    std::vector<NS2Y<T>> y2;
+
+    int size() const override { return (int)e.size(); }
+    SpecRef spec_cont(int i) const override {
+      if (DIM == 3) return SpecRef{e[i]->P, e[i]->LDM, e[i]->qs(3)};
+      return SpecRef{s0[i], Nh, 0};
+    }
+    SpecRef spec_last(int i) const override { return SpecRef{s0[i], Nh, 0}; }
+
+    // one pass for members [0, m); those from f on take their last step
+    void pass_x(int t, int m, int f, hipStream_t s) const {
+      const EngineT& E = *e[0];
+      if (DIM == 3) {
+        packs<NS3X<T>>(m, [&](int i) -> const NS3X<T>& { return (t == 4 && i >= f) ? x3[5][i] : x3[t][i]; },
+                       [&](const Pack<NS3X<T>>& pk, int c) {
+                         E.with_xv([&](auto v) { launch_x3<decltype(v)::value>(pk, dim3(E.xtiles(), N, c), s); });
+                         QL_LAUNCHED();
+                       });
+      } else {
+        packs<NS2X<T>>(m, [&](int i) -> const NS2X<T>& { return (t == 4 && i >= f) ? x2[5][i] : x2[t][i]; },
+                       [&](const Pack<NS2X<T>>& pk, int c) {
+                         E.with_xv([&](auto v) {
+                           constexpr int V = decltype(v)::value;
+                           hipLaunchKernelGGL((k_ns2_x<T, N, V>), dim3((Nh + XCfg<N, V>::TIX - 1) / XCfg<N, V>::TIX, 1, c),
+                                              dim3(XCfg<N, V>::THREADS), 0, s, pk);
+                         });
+                         QL_LAUNCHED();
+                       });
+      }
+    }
+    void pass_yz(int m, hipStream_t s) const {
+      const EngineT& E = *e[0];
+      if (DIM == 3) {
+        packs<NS3Y<T>>(m, [&](int i) -> const NS3Y<T>& { return yi[i]; }, [&](const Pack<NS3Y<T>>& pk, int c) {
+          E.with_xv([&](auto v) {
+            constexpr int V = decltype(v)::value;
+            hipLaunchKernelGGL((k_ns3_yinv<T, N, V>), dim3(E.xtiles(), N, c), dim3(XCfg<N, V>::THREADS), 0, s, pk);
+          });
+          QL_LAUNCHED();
+        });
+        packs<NS3Z<T>>(m, [&](int i) -> const NS3Z<T>& { return zz[i]; }, [&](const Pack<NS3Z<T>>& pk, int c) {
+          E.with_zv([&](auto v) {
+            constexpr int V = decltype(v)::value;
+            hipLaunchKernelGGL((k_ns3_z<T, N, V>), dim3(N * N / Z3Cfg<T, N, V>::P, 1, c),
+                               dim3(Z3Cfg<T, N, V>::THREADS), 0, s, pk);
+          });
+          QL_LAUNCHED();
+        });
+        packs<NS3YF<T>>(m, [&](int i) -> const NS3YF<T>& { return yf[i]; }, [&](const Pack<NS3YF<T>>& pk, int c) {
+          E.with_xv([&](auto v) {
+            constexpr int V = decltype(v)::value;
+            hipLaunchKernelGGL((k_ns3_yfwd<T, N, V>), dim3((E.KZD + XCfg<N, V>::TIX - 1) / XCfg<N, V>::TIX, N, c),
+                               dim3(XCfg<N, V>::THREADS), 0, s, pk);
+          });
+          QL_LAUNCHED();
+        });
+      } else {
+        packs<NS2Y<T>>(m, [&](int i) -> const NS2Y<T>& { return y2[i]; }, [&](const Pack<NS2Y<T>>& pk, int c) {
+          E.with_zv([&](auto v) {
+            constexpr int V = decltype(v)::value;
+            hipLaunchKernelGGL((k_ns2_y<T, N, V>), dim3((N + ZCfg<N, V>::P - 1) / ZCfg<N, V>::P, 1, c),
+                               dim3(ZCfg<N, V>::THREADS), 0, s, pk);
+          });
+          QL_LAUNCHED();
+        });
+      }
+    }
+
+    void steps(const int* n, int from, int to, hipStream_t s,
+               const std::function<void(hipStream_t, int, int)>& after) override {
+      const int M = size();
+      for (int i = 1; i < M; ++i)
+        if (n[i] > n[i - 1]) throw std::invalid_argument("group step counts must not increase along the members");
+      auto active = [&](int k) {
+        int c = 0;
+        while (c < M && n[c] > k) ++c;
+        return c;
+      };
+      if (from == 0) {
+        const int m0 = active(0);
+        if (m0 > 0) pass_x(0, m0, m0, s);
+      }
+      for (int k = from; k < to; ++k) {
+        const int m = active(k), f = active(k + 1);
+        if (m == 0) break;
+        for (int st = 0; st < 4; ++st) {
+          pass_yz(m, s);
+          pass_x(1 + st, m, f, s);
+        }
+        if (after) after(s, m, f);
+      }
+    }
+  };
+
+  bool groupable_with(const Engine* o) const override {
+    const auto* x = dynamic_cast<const EngineT*>(o);
+    if (x == nullptr) return false;
+    if (d_.forcing == QLLES_FORCING_EKIN_TOT || x->d_.forcing == QLLES_FORCING_EKIN_TOT) return false;
+    return x->d_.viscous == d_.viscous && x->xv_ == xv_ && x->zv_ == zv_;
+  }
+
+  EngineGroup* make_group(const std::vector<Engine*>& members, const std::vector<void*>& uhat) override {
+    if (members.empty() || members[0] != this || members.size() != uhat.size())
+      throw std::invalid_argument("a group lists this engine first and one state per member");
+    for (const Engine* m : members)
+      if (!groupable_with(m)) throw std::invalid_argument("engines of different kinds cannot share a group");
+    const int M = (int)members.size();
+    for (int i = 0; i < M; ++i)
+      for (int j = 0; j < i; ++j)
+        if (members[i] == members[j]) throw std::invalid_argument("each group member needs its own plan");
+    std::unique_ptr<Group> g(new Group());
+    for (int i = 0; i < M; ++i) {
+      auto* x = static_cast<EngineT*>(members[i]);
+      x->ensure_step();
+      g->e.push_back(x);
+      g->s0.push_back(static_cast<cx<T>*>(uhat[i]));
+    }
+    for (int i = 0; i < M; ++i) {
+      const EngineT& X = *g->e[i];
+      cx<T>* S0 = g->s0[i];
+      if (DIM == 3) {
+        NS3X<T> a = X.xargs3(S0, Nh, 0);
+        a.c = X.ctl(-1);
+        a.us = S0;
+        a.ld_us = Nh;
+        a.copy_dst = X.P;
+        a.ld_copy = X.LDM;
+        a.q_copy = X.qs(3);
+        a.do_inverse = 1;
+        g->x3[0].push_back(a);
+        for (int st = 0; st < 4; ++st) {
+          a = X.xargs3(X.P, X.LDM, X.qs(3));
+          a.c = X.ctl(st);
+          a.us = st == 0 ? X.P : (st == 2 ? X.S1b : X.S1a);
+          a.ld_us = X.LDM;
+          a.q_us = X.qs(3);
+          a.uout = st == 0 ? X.S1a : (st == 1 ? X.S1b : (st == 2 ? X.S1a : X.P));
+          a.ld_uout = X.LDM;
+          a.q_uout = X.qs(3);
+          a.do_inverse = 1;
+          g->x3[1 + st].push_back(a);
+          if (st == 3) {
+            a.uout = S0;
+            a.ld_uout = Nh;
+            a.q_uout = 0;
+            a.do_inverse = 0;
+            g->x3[5].push_back(a);
+          }
+        }
+        g->yi.push_back(NS3Y<T>{X.X5, X.Y6, X.LDM, X.LDM, X.qs(5), X.qs(6), 6, X.twN_});
+        g->zz.push_back(NS3Z<T>{X.Y6, X.NZ, nullptr, nullptr, X.qs(6), X.KZD, X.KZP, X.twN_, nullptr});
+        g->yf.push_back(NS3YF<T>{X.NZ, X.NY, X.KZD, X.KZP, X.KYD, X.twN_});
+      } else {
+        NS2X<T> a = X.xargs2(S0);
+        a.c = X.ctl(-1);
+        a.us = S0;
+        a.ld_us = Nh;
+        a.do_inverse = 1;
+        g->x2[0].push_back(a);
+        for (int st = 0; st < 4; ++st) {
+          a = X.xargs2(S0);
+          a.c = X.ctl(st);
+          a.us = st == 0 ? S0 : (st == 2 ? X.S1b : X.S1a);
+          a.ld_us = st == 0 ? Nh : X.LD;
+          a.uout = st == 0 ? X.S1a : (st == 1 ? X.S1b : (st == 2 ? X.S1a : S0));
+          a.ld_uout = st == 3 ? Nh : X.LD;
+          a.do_inverse = 1;
+          g->x2[1 + st].push_back(a);
+          if (st == 3) {
+            a.do_inverse = 0;
+            g->x2[5].push_back(a);
+          }
+        }
+        g->y2.push_back(NS2Y<T>{X.X5, X.NY, nullptr, nullptr, X.LD, X.KZD, X.KZP, X.twN_});
+      }
+    }
+    return g.release();
   }
 
   void rhs_jvp(const void* uhat, const void* v, void* out, hipStream_t s) override {

@@ -41,6 +41,11 @@ template <typename T> __device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { re
 template <typename T> __device__ __forceinline__ cx<T> cneg(cx<T> a) { return {-a.re, -a.im}; }
 // multiply by +i
 template <typename T> __device__ __forceinline__ cx<T> cmuli(cx<T> a) { return {-a.im, a.re}; }
+// Compiler barrier on a value (no instruction): the code after it sees an
+// opaque register, so the products and sums it forms from v are contracted
+// into FMAs the same way whatever computed v (floating-point contraction
+// otherwise depends on the surrounding code of each kernel specialisation).
+template <typename T> __device__ __forceinline__ void opaque(cx<T>& v) { asm volatile("" : "+v"(v.re), "+v"(v.im)); }
 // store of a value that a later kernel reads back: NT selects a non-temporal
 // store, which MI355X retires at a higher write rate than a cached one but
 // which leaves nothing in the last-level cache for the reader

@@ -20,7 +20,9 @@
 // u_hat and k* = k with every -n/2 component replaced by +n/2 (the .real
 // projection of the asymmetric crop).  After interpolation an optional FP64
 // Euler shift r <- (r + dt u_r) mod 2 pi (shift_fn, integrator.py:21-23).
+#include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "aux_kernels.hpp"
@@ -91,8 +93,9 @@ __device__ __forceinline__ double wrap2pi(double v) {
 
 // coefficients G_k from a source rfft spectrum (grid Ns, row stride ld)
 template <typename T>
-__global__ void k_build_coef(const cx<T>* __restrict__ uh, int Ns, int ld, size_t qplane, int dim, int n, double scale,
-                             const T* __restrict__ corr, cx<T>* __restrict__ out, int mode, int nf) {
+__device__ __forceinline__ void build_coef_run(const cx<T>* __restrict__ uh, int Ns, int ld, size_t qplane, int dim,
+                                               int n, double scale, const T* __restrict__ corr,
+                                               cx<T>* __restrict__ out, int mode, int nf) {
   const int ncomp = dim;
   const size_t per = dim == 3 ? (size_t)n * n * n : (size_t)n * n;
   const size_t cs_in = (dim == 3 ? (size_t)Ns * Ns : (size_t)Ns) * ld;
@@ -137,11 +140,39 @@ __global__ void k_build_coef(const cx<T>* __restrict__ uh, int Ns, int ld, size_
   }
 }
 
+// per-launch arguments of the coefficient builder and the gathers, passed in
+// a Pack (ns3d.hpp): z-slice m of the grid runs entry m (one entry for one
+// interpolator, one per member for a batch group)
+template <typename T>
+struct CoefArgs {
+  const cx<T>* uh;
+  int Ns, ld;
+  size_t qplane;
+  double scale;
+  const T* corr;
+  cx<T>* out;
+};
+template <typename T>
+struct GatherArgs {
+  const T* f;  // fine grid: [dim][nf^dim] complex (2-D) or the packed [nf^3][4] reals (3-D)
+  const double* r;
+  int np;
+  T* ur;
+  double dt;
+  double* r_out;
+};
+
+template <typename T>
+__global__ void k_build_coef(Pack<CoefArgs<T>> pk, int dim, int n, int mode, int nf) {
+  const CoefArgs<T>& a = pk.m[blockIdx.z];
+  build_coef_run<T>(a.uh, a.Ns, a.ld, a.qplane, dim, n, a.scale, a.corr, a.out, mode, nf);
+}
+
 // one wavefront per particle; lanes own (dy, dz) columns of the w^3 stencil
 template <typename T, int DIM>
-__global__ void __launch_bounds__(256)
-k_gather(const cx<T>* __restrict__ f, int nf, int w, T beta, const double* __restrict__ r, int np,
-         T* __restrict__ ur, double dt, double* __restrict__ r_out) {
+__device__ __forceinline__ void gather_run(const cx<T>* __restrict__ f, int nf, int w, T beta,
+                                           const double* __restrict__ r, int np, T* __restrict__ ur, double dt,
+                                           double* __restrict__ r_out) {
   const int lane = threadIdx.x & 63;
   const int pid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (pid >= np) return;  // whole wavefront exits together
@@ -205,9 +236,19 @@ k_gather(const cx<T>* __restrict__ f, int nf, int w, T beta, const double* __res
   }
 }
 
-// real parts of the three fine-grid components, interleaved per point
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+k_gather(Pack<GatherArgs<T>> pk, int nf, int w, T beta) {
+  const GatherArgs<T>& a = pk.m[blockIdx.z];
+  gather_run<T, DIM>(reinterpret_cast<const cx<T>*>(a.f), nf, w, beta, a.r, a.np, a.ur, a.dt, a.r_out);
+}
+
+// real parts of the three fine-grid components, interleaved per point; grid
+// row blockIdx.y packs the blockIdx.y-th of consecutive [3][cs] grids
 template <typename T>
 __global__ void k_pack3(const cx<T>* __restrict__ f, T* __restrict__ out, size_t cs) {
+  f += 3 * cs * blockIdx.y;
+  out += 4 * cs * blockIdx.y;
   for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < cs; o += (size_t)gridDim.x * blockDim.x) {
     out[4 * o] = f[o].re;
     out[4 * o + 1] = f[cs + o].re;
@@ -220,9 +261,8 @@ __global__ void k_pack3(const cx<T>* __restrict__ f, T* __restrict__ out, size_t
 // the kernel width W is a compile-time constant so each lane's W loads along
 // x are issued back to back
 template <typename T, int W>
-__global__ void __launch_bounds__(256)
-k_gather3p(const T* __restrict__ f4, int nf, T beta, const double* __restrict__ r, int np,
-           T* __restrict__ ur, double dt, double* __restrict__ r_out) {
+__device__ __forceinline__ void gather3p_run(const T* __restrict__ f4, int nf, T beta, const double* __restrict__ r,
+                                             int np, T* __restrict__ ur, double dt, double* __restrict__ r_out) {
   constexpr int w = W;
   const int lane = threadIdx.x & 63;
   // workgroups go round-robin over the 8 XCDs; give each XCD a contiguous run
@@ -303,6 +343,13 @@ k_gather3p(const T* __restrict__ f4, int nf, T beta, const double* __restrict__ 
       if (r_out) r_out[(size_t)pid * 3 + c] = wrap2pi(r[(size_t)pid * 3 + c] + dt * (double)acc[c]);
     }
   }
+}
+
+template <typename T, int W>
+__global__ void __launch_bounds__(256)
+k_gather3p(Pack<GatherArgs<T>> pk, int nf, T beta) {
+  const GatherArgs<T>& a = pk.m[blockIdx.z];
+  gather3p_run<T, W>(a.f, nf, beta, a.r, a.np, a.ur, a.dt, a.r_out);
 }
 
 // exact direct sum; one particle per lane, coefficient planes through LDS
@@ -548,32 +595,36 @@ __global__ void k_zero_cx(cx<T>* p, size_t n) {
 }
 
 template <typename T, int W>
-void launch_gather3p_w(dim3 g, hipStream_t s, const T* pk, int nf, T beta, const double* r, int np, T* ur, double dt,
-                       double* r_out) {
-  hipLaunchKernelGGL((k_gather3p<T, W>), g, dim3(256), 0, s, pk, nf, beta, r, np, ur, dt, r_out);
+void launch_gather3p_w(dim3 g, hipStream_t s, const Pack<GatherArgs<T>>& pk, int nf, T beta) {
+  hipLaunchKernelGGL((k_gather3p<T, W>), g, dim3(256), 0, s, pk, nf, beta);
 }
 template <typename T>
-void launch_gather3p(int w, dim3 g, hipStream_t s, const T* pk, int nf, T beta, const double* r, int np, T* ur,
-                     double dt, double* r_out) {
+void launch_gather3p(int w, dim3 g, hipStream_t s, const Pack<GatherArgs<T>>& pk, int nf, T beta) {
   switch (w) {
-    case 2: launch_gather3p_w<T, 2>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
-    case 3: launch_gather3p_w<T, 3>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
-    case 4: launch_gather3p_w<T, 4>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
-    case 5: launch_gather3p_w<T, 5>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
-    case 6: launch_gather3p_w<T, 6>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
-    case 7: launch_gather3p_w<T, 7>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
-    case 8: launch_gather3p_w<T, 8>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
-    case 9: launch_gather3p_w<T, 9>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
-    case 10: launch_gather3p_w<T, 10>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
-    case 11: launch_gather3p_w<T, 11>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
-    case 12: launch_gather3p_w<T, 12>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
-    case 13: launch_gather3p_w<T, 13>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
-    case 14: launch_gather3p_w<T, 14>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
-    case 15: launch_gather3p_w<T, 15>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
-    case 16: launch_gather3p_w<T, 16>(g, s, pk, nf, beta, r, np, ur, dt, r_out); break;
+    case 2: launch_gather3p_w<T, 2>(g, s, pk, nf, beta); break;
+    case 3: launch_gather3p_w<T, 3>(g, s, pk, nf, beta); break;
+    case 4: launch_gather3p_w<T, 4>(g, s, pk, nf, beta); break;
+    case 5: launch_gather3p_w<T, 5>(g, s, pk, nf, beta); break;
+    case 6: launch_gather3p_w<T, 6>(g, s, pk, nf, beta); break;
+    case 7: launch_gather3p_w<T, 7>(g, s, pk, nf, beta); break;
+    case 8: launch_gather3p_w<T, 8>(g, s, pk, nf, beta); break;
+    case 9: launch_gather3p_w<T, 9>(g, s, pk, nf, beta); break;
+    case 10: launch_gather3p_w<T, 10>(g, s, pk, nf, beta); break;
+    case 11: launch_gather3p_w<T, 11>(g, s, pk, nf, beta); break;
+    case 12: launch_gather3p_w<T, 12>(g, s, pk, nf, beta); break;
+    case 13: launch_gather3p_w<T, 13>(g, s, pk, nf, beta); break;
+    case 14: launch_gather3p_w<T, 14>(g, s, pk, nf, beta); break;
+    case 15: launch_gather3p_w<T, 15>(g, s, pk, nf, beta); break;
+    case 16: launch_gather3p_w<T, 16>(g, s, pk, nf, beta); break;
     default: throw std::invalid_argument("NUFFT kernel width out of range");
   }
   QL_LAUNCHED();
+}
+template <class A>
+Pack<A> pack1(const A& a) {
+  Pack<A> pk{};
+  pk.m[0] = a;
+  return pk;
 }
 
 size_t tsize(int dtype) { return dtype == QLLES_F32 ? sizeof(float) : sizeof(double); }
@@ -625,8 +676,10 @@ void run_t(qlles_interp* it, const cx<T>* uh, int Ns, int ld, size_t q, double s
   const size_t per_n = d == 3 ? (size_t)n * n * n : (size_t)n * n;
   const int nb = (int)std::min<size_t>((per_n * d + 255) / 256, 8192);
   const int mode = it->backend == QLLES_INTERP_NUFFT ? 0 : 1;
-  hipLaunchKernelGGL((k_build_coef<T>), dim3(nb), dim3(256), 0, s, uh, Ns, ld, q, d, n, scale,
-                     static_cast<const T*>(it->corr), static_cast<cx<T>*>(it->coef), mode, it->nf);
+  hipLaunchKernelGGL((k_build_coef<T>), dim3(nb), dim3(256), 0, s,
+                     pack1(CoefArgs<T>{uh, Ns, ld, q, scale, static_cast<const T*>(it->corr),
+                                       static_cast<cx<T>*>(it->coef)}),
+                     d, n, mode, it->nf);
   QL_LAUNCHED();
   if (np == 0) return;
   if (mode == 0) {
@@ -643,12 +696,14 @@ void run_t(qlles_interp* it, const cx<T>* uh, int Ns, int ld, size_t q, double s
       hipLaunchKernelGGL((k_pack3<T>), dim3((int)std::min<size_t>((cs + 255) / 256, 4096)), dim3(256), 0, s, fi, pk,
                          cs);
       QL_LAUNCHED();
-      launch_gather3p<T>(it->w, dim3((np + 3) / 4), s, pk, nf, (T)it->beta, r, np, ur, dt, r_out);
+      launch_gather3p<T>(it->w, dim3((np + 3) / 4), s, pack1(GatherArgs<T>{pk, r, np, ur, dt, r_out}), nf,
+                         (T)it->beta);
     } else {
       c2c_strided<T>(nf, co, fi, 2, nf, false, (T)1, twf, s);
       c2c_contig<T>(nf, fi, fi, 2 * nf, false, (T)1, twf, s);
-      hipLaunchKernelGGL((k_gather<T, 2>), dim3((np + 3) / 4), dim3(256), 0, s, fi, nf, it->w, (T)it->beta, r, np,
-                         ur, dt, r_out);
+      hipLaunchKernelGGL((k_gather<T, 2>), dim3((np + 3) / 4), dim3(256), 0, s,
+                         pack1(GatherArgs<T>{reinterpret_cast<const T*>(fi), r, np, ur, dt, r_out}), nf, it->w,
+                         (T)it->beta);
     }
     QL_LAUNCHED();
   } else {
@@ -782,8 +837,10 @@ void hat_vjp_t(qlles_interp* it, int N, const cx<T>* uh, const double* r, int np
   // position cotangent: derivative coefficients of the forward interpolant
   const size_t per_n = (size_t)n * n * n;
   hipLaunchKernelGGL((k_build_coef<T>), dim3((int)std::min<size_t>((per_n * 3 + 255) / 256, 8192)), dim3(256), 0,
-                     s, uh, N, N / 2 + 1, (size_t)0, 3, n, 1.0 / ((double)N * N * N),
-                     static_cast<const T*>(it->corr), static_cast<cx<T>*>(it->coef), 0, nf);
+                     s,
+                     pack1(CoefArgs<T>{uh, N, N / 2 + 1, (size_t)0, 1.0 / ((double)N * N * N),
+                                       static_cast<const T*>(it->corr), static_cast<cx<T>*>(it->coef)}),
+                     3, n, 0, nf);
   QL_LAUNCHED();
   for (int j = 0; j < 3; ++j) {
     hipLaunchKernelGGL((k_coef_deriv<T>), dim3(nbz), dim3(256), 0, s, static_cast<const cx<T>*>(it->coef), fb, nf, j);
@@ -823,6 +880,168 @@ void interp_hat(qlles_interp* it, int N, const void* u_hat, int ld, size_t q, do
   else
     run_t<double>(it, static_cast<const cx<double>*>(u_hat), N, ld, q, scale, r, np, static_cast<double*>(u_r), dt,
                   r_out, s);
+}
+
+
+// ---------------- groups of batch members ----------------
+
+struct InterpGroup {
+  int dim = 0, n = 0, nf = 0, w = 0, dtype = 0, M = 0, max_np = 0;
+  double beta = 0;
+  void* coef = nullptr;    // [M][dim][nf^dim] zero-padded, deconvolved coefficients
+  void* fine = nullptr;    // [M][dim][nf^dim] fine-grid values
+  void* packed = nullptr;  // 3-D: [M][nf^3][4]
+  // per member: coefficient-builder arguments reading the spectrum after a
+  // step that is not / is the member's last, and gather arguments
+  std::vector<unsigned char> cont, last, gath;  // CoefArgs<T>[M], CoefArgs<T>[M], GatherArgs<T>[M]
+};
+
+bool interp_groupable(const qlles_interp* a, const qlles_interp* b) {
+  return a->backend == QLLES_INTERP_NUFFT && b->backend == QLLES_INTERP_NUFFT && a->dim == b->dim && a->n == b->n &&
+         a->dtype == b->dtype && a->w == b->w && a->beta == b->beta && a->nf == b->nf;
+}
+
+void destroy_interp_group(InterpGroup* g) {
+  if (!g) return;
+  if (g->coef) (void)hipFree(g->coef);
+  if (g->fine) (void)hipFree(g->fine);
+  if (g->packed) (void)hipFree(g->packed);
+  delete g;
+}
+
+namespace {
+
+template <class A>
+const A* entries(const std::vector<unsigned char>& v) {
+  return reinterpret_cast<const A*>(v.data());
+}
+template <class A>
+void put_entries(std::vector<unsigned char>& v, const std::vector<A>& a) {
+  v.resize(sizeof(A) * a.size());
+  std::memcpy(v.data(), a.data(), v.size());
+}
+
+// launches for members [0, m), Pack<A>::K per launch; entry(i) gives member i's arguments
+template <class A, class E, class L>
+void packs(int m, E&& entry, L&& launch) {
+  for (int lo = 0; lo < m; lo += Pack<A>::K) {
+    Pack<A> pk{};
+    const int c = std::min(Pack<A>::K, m - lo);
+    for (int i = 0; i < c; ++i) pk.m[i] = entry(lo + i);
+    launch(pk, c, lo);
+  }
+}
+
+template <typename T>
+void group_setup_t(InterpGroup* g, const std::vector<qlles_interp*>& its, int N, const std::vector<SpecRef>& cont,
+                   const std::vector<SpecRef>& last, const std::vector<double*>& r, const std::vector<int>& np,
+                   const std::vector<void*>& ur, const std::vector<double>& dt) {
+  const int M = g->M, d = g->dim, nf = g->nf;
+  const size_t per_f = d == 3 ? (size_t)nf * nf * nf : (size_t)nf * nf;
+  QL_HIP(hipMalloc(&g->coef, sizeof(cx<T>) * per_f * d * M));
+  QL_HIP(hipMemset(g->coef, 0, sizeof(cx<T>) * per_f * d * M));  // the padding stays zero
+  QL_HIP(hipMalloc(&g->fine, sizeof(cx<T>) * per_f * d * M));
+  if (d == 3) QL_HIP(hipMalloc(&g->packed, sizeof(T) * 4 * per_f * M));
+  const double scale = 1.0 / (d == 3 ? (double)N * N * N : (double)N * N);
+  std::vector<CoefArgs<T>> tc(M), tl(M);
+  std::vector<GatherArgs<T>> tg(M);
+  for (int i = 0; i < M; ++i) {
+    cx<T>* out = static_cast<cx<T>*>(g->coef) + (size_t)i * d * per_f;
+    const T* corr = static_cast<const T*>(its[i]->corr);
+    tc[i] = CoefArgs<T>{static_cast<const cx<T>*>(cont[i].p), N, cont[i].ld, cont[i].q, scale, corr, out};
+    tl[i] = CoefArgs<T>{static_cast<const cx<T>*>(last[i].p), N, last[i].ld, last[i].q, scale, corr, out};
+    const T* f = d == 3 ? static_cast<const T*>(g->packed) + (size_t)i * 4 * per_f
+                        : reinterpret_cast<const T*>(static_cast<const cx<T>*>(g->fine) + (size_t)i * d * per_f);
+    tg[i] = GatherArgs<T>{f, r[i], np[i], static_cast<T*>(ur[i]), dt[i], dt[i] != 0.0 ? r[i] : nullptr};
+  }
+  put_entries(g->cont, tc);
+  put_entries(g->last, tl);
+  put_entries(g->gath, tg);
+}
+
+template <typename T>
+void group_run_t(InterpGroup* g, int m, int f, hipStream_t s) {
+  const int n = g->n, d = g->dim, nf = g->nf;
+  const size_t per_n = d == 3 ? (size_t)n * n * n : (size_t)n * n;
+  const int nb = (int)std::min<size_t>((per_n * d + 255) / 256, 8192);
+  const auto* tc = entries<CoefArgs<T>>(g->cont);
+  const auto* tl = entries<CoefArgs<T>>(g->last);
+  const auto* tg = entries<GatherArgs<T>>(g->gath);
+  packs<CoefArgs<T>>(m, [&](int i) -> const CoefArgs<T>& { return i < f ? tc[i] : tl[i]; },
+                     [&](const Pack<CoefArgs<T>>& pk, int c, int) {
+                       hipLaunchKernelGGL((k_build_coef<T>), dim3(nb, 1, c), dim3(256), 0, s, pk, d, n, 0, nf);
+                       QL_LAUNCHED();
+                     });
+  if (g->max_np == 0) return;
+  const cx<T>* twf = twiddles<T>(nf);
+  cx<T>* co = static_cast<cx<T>*>(g->coef);
+  cx<T>* fi = static_cast<cx<T>*>(g->fine);
+  if (d == 3) {
+    c2c_strided<T>(nf, co, fi, 3 * m, nf * nf, false, (T)1, twf, s);
+    c2c_strided<T>(nf, fi, fi, 3 * m * nf, nf, false, (T)1, twf, s);
+    c2c_contig<T>(nf, fi, fi, 3 * m * nf * nf, false, (T)1, twf, s);
+    const size_t cs = (size_t)nf * nf * nf;
+    hipLaunchKernelGGL((k_pack3<T>), dim3((int)std::min<size_t>((cs + 255) / 256, 4096), m), dim3(256), 0, s, fi,
+                       static_cast<T*>(g->packed), cs);
+    QL_LAUNCHED();
+    packs<GatherArgs<T>>(m, [&](int i) -> const GatherArgs<T>& { return tg[i]; },
+                         [&](const Pack<GatherArgs<T>>& pk, int c, int) {
+                           launch_gather3p<T>(g->w, dim3((g->max_np + 3) / 4, 1, c), s, pk, nf, (T)g->beta);
+                         });
+  } else {
+    c2c_strided<T>(nf, co, fi, 2 * m, nf, false, (T)1, twf, s);
+    c2c_contig<T>(nf, fi, fi, 2 * m * nf, false, (T)1, twf, s);
+    packs<GatherArgs<T>>(m, [&](int i) -> const GatherArgs<T>& { return tg[i]; },
+                         [&](const Pack<GatherArgs<T>>& pk, int c, int) {
+                           hipLaunchKernelGGL((k_gather<T, 2>), dim3((g->max_np + 3) / 4, 1, c), dim3(256), 0, s, pk,
+                                              nf, g->w, (T)g->beta);
+                           QL_LAUNCHED();
+                         });
+  }
+}
+
+}  // namespace
+
+InterpGroup* make_interp_group(const std::vector<qlles_interp*>& its, int N, const std::vector<SpecRef>& cont,
+                               const std::vector<SpecRef>& last, const std::vector<double*>& r,
+                               const std::vector<int>& np, const std::vector<void*>& ur,
+                               const std::vector<double>& dt) {
+  const size_t M = its.size();
+  if (M == 0 || cont.size() != M || last.size() != M || r.size() != M || np.size() != M || ur.size() != M ||
+      dt.size() != M)
+    throw std::invalid_argument("one spectrum, position and velocity array per group member");
+  for (size_t i = 0; i < M; ++i) {
+    if (!interp_groupable(its[0], its[i])) throw std::invalid_argument("interpolators of a group must match");
+    if (np[i] < 0) throw std::invalid_argument("particle counts must be >= 0");
+    for (const SpecRef* sp : {&cont[i], &last[i]})
+      if (sp->q == 0 ? sp->ld < N / 2 + 1 : (sp->ld < N / 2 || its[0]->dim != 3))
+        throw std::invalid_argument("spectrum row stride too small for its layout");
+  }
+  if (N < its[0]->n || (N % 2) != 0) throw std::invalid_argument("solver grid N must be even and >= n");
+  auto* g = new InterpGroup();
+  g->dim = its[0]->dim;
+  g->n = its[0]->n;
+  g->nf = its[0]->nf;
+  g->w = its[0]->w;
+  g->beta = its[0]->beta;
+  g->dtype = its[0]->dtype;
+  g->M = (int)M;
+  for (int v : np) g->max_np = std::max(g->max_np, v);
+  try {
+    if (g->dtype == QLLES_F32) group_setup_t<float>(g, its, N, cont, last, r, np, ur, dt);
+    else group_setup_t<double>(g, its, N, cont, last, r, np, ur, dt);
+  } catch (...) {
+    destroy_interp_group(g);
+    throw;
+  }
+  return g;
+}
+
+void interp_group_run(InterpGroup* g, int m, int f, hipStream_t s) {
+  if (m < 0 || m > g->M || f < 0 || f > m) throw std::invalid_argument("group member range out of bounds");
+  if (m == 0) return;
+  if (g->dtype == QLLES_F32) group_run_t<float>(g, m, f, s);
+  else group_run_t<double>(g, m, f, s);
 }
 
 }  // namespace ql

@@ -2,7 +2,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "../../include/qlles.h"
+#include "engine.hpp"
 
 namespace ql {
 qlles_interp* make_interp(int dim, int n, int dtype, int backend, double tol);
@@ -20,4 +23,21 @@ void interp_hat(qlles_interp* it, int N, const void* u_hat, int ld, size_t q, do
 // r_bar (optional) += (du_r/dr)^T u_bar
 void interp_hat_vjp(qlles_interp* it, int N, const void* u_hat, const double* r, int np, const void* u_bar,
                     void* lam_hat, double* r_bar, hipStream_t s);
+
+// Particle advection of a group of batch members (see EngineGroup): one
+// filter + type-2 NUFFT + shift for all members stepping, with the members'
+// coefficient and fine grids side by side so the fine-grid transforms are one
+// batched call.  Members share dim, n, precision and kernel (NUFFT backend);
+// member i's particles r[i] (np[i], FP64, updated in place when dt[i] != 0)
+// and velocities ur[i] are its own, as is its interpolator's correction
+// table.  Results equal interp_hat's for each member bit for bit.
+struct InterpGroup;
+bool interp_groupable(const qlles_interp* a, const qlles_interp* b);
+InterpGroup* make_interp_group(const std::vector<qlles_interp*>& its, int N, const std::vector<SpecRef>& cont,
+                               const std::vector<SpecRef>& last, const std::vector<double*>& r,
+                               const std::vector<int>& np, const std::vector<void*>& ur,
+                               const std::vector<double>& dt);
+// members [0, f) read their spec_cont spectrum, [f, m) their spec_last one
+void interp_group_run(InterpGroup* g, int m, int f, hipStream_t s);
+void destroy_interp_group(InterpGroup* g);
 }  // namespace ql

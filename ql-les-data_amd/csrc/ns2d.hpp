@@ -58,8 +58,7 @@ __device__ double forced_shell_energy2(const cx<T>* us, int ld, int kf2, double*
 }
 
 template <typename T, int N, int V = 0>
-__global__ void __launch_bounds__((XCfg<N, V>::THREADS))
-k_ns2_x(NS2X<T> a) {
+__device__ __forceinline__ void ns2_x_run(const NS2X<T>& a) {
   using C = XCfg<N, V>;
   constexpr int Nh = N / 2 + 1;
   using FF = XFwd<T, N>;
@@ -196,6 +195,12 @@ k_ns2_x(NS2X<T> a) {
   }
 }
 
+template <typename T, int N, int V = 0>
+__global__ void __launch_bounds__((XCfg<N, V>::THREADS))
+k_ns2_x(Pack<NS2X<T>> pk) {
+  ns2_x_run<T, N, V>(pk.m[blockIdx.z]);
+}
+
 template <typename T>
 struct NS2Y {
   const cx<T>* in;   // [3][N][ld]
@@ -207,8 +212,7 @@ struct NS2Y {
 };
 
 template <typename T, int N, int V = 0>
-__global__ void __launch_bounds__((ZCfg<N, V>::THREADS))
-k_ns2_y(NS2Y<T> a) {
+__device__ __forceinline__ void ns2_y_run(const NS2Y<T>& a) {
   using C = ZCfg<N, V>;
   constexpr int M = C::M, TP = C::TP, P = C::P;
   constexpr int Nh = M + 1;
@@ -301,6 +305,12 @@ k_ns2_y(NS2Y<T> a) {
       }
     }
   }
+}
+
+template <typename T, int N, int V = 0>
+__global__ void __launch_bounds__((ZCfg<N, V>::THREADS))
+k_ns2_y(Pack<NS2Y<T>> pk) {
+  ns2_y_run<T, N, V>(pk.m[blockIdx.z]);
 }
 
 }  // namespace qk

@@ -107,6 +107,21 @@ template <typename T, int N> struct ZWpe {
 
 namespace qk {
 
+// Arguments of up to K launches of one kernel, passed by value: z-slice m of
+// the grid runs with entry m.  A launch for one plan fills entry 0; a batch
+// group fills one entry per member (several launches when it has more than K
+// members).  Plans stepped alone and groups therefore run the very same
+// kernels, so a member's results are bit-identical to its solo run (two
+// kernels compiled from one body need not round alike: floating-point
+// contraction depends on the surrounding code).  K keeps the kernel
+// arguments near 1.3 KB.
+template <class A>
+struct Pack {
+  static constexpr int K0 = 1280 / (int)sizeof(A);
+  static constexpr int K = K0 < 1 ? 1 : (K0 > 16 ? 16 : K0);
+  A m[K];
+};
+
 // RK / forcing controls shared by the 3-D and 2-D x-pass kernels
 template <typename T>
 struct StageCtl {
@@ -313,262 +328,12 @@ template <int SG, bool VIF> struct AccOut {
 // the stores are predicated, which lets loads of later slots overlap.
 template <typename T, int N, int SG, bool VIF, int V = 0>
 __global__ void __launch_bounds__((XCfg<N, V>::THREADS)) QL_ATTR_X
-k_ns3_x(NS3X<T> a) {
-  using C = XCfg<N, V>;
-  constexpr bool NTL = QL_NTL && NTBig<T, N>::value;
-  constexpr int Nh = N / 2 + 1;
-  using FF = XFwd<T, N>;
-  using IF = XInv<T, N>;
-  static_assert(FF::NOUT == IF::NIN, "fft hand-off mismatch");
-  constexpr int NS = FF::NOUT;
-  __shared__ cx<T> s_twf[FF::TWL > 0 ? FF::TWL : 1];
-  __shared__ cx<T> s_twi[IF::TWL > 0 ? IF::TWL : 1];
-  __shared__ cx<T> s_buf[N * C::LS];
-  __shared__ double s_red[2 * C::THREADS];
-
-  const int tid = threadIdx.x, p = tid % C::TIX, j = tid / C::TIX;
-  // Tiles 0..NTR-1 hold TIX consecutive kz < N/2 of one ky row; tile NTR holds
-  // the kz = N/2 column for TIX consecutive ky rows (so no workgroup runs its
-  // transforms for a single valid column).
-  constexpr int NTR = (N / 2) / C::TIX;
-  const bool nyq = NTR > 0 && (int)blockIdx.x == NTR;
-  const StageCtl<T>& c = a.c;
-  if (nyq && (int)blockIdx.y >= N / C::TIX) {
-    if (c.partials != nullptr && tid == 0) {
-      const int bid = blockIdx.y * gridDim.x + blockIdx.x;
-      c.partials[2 * bid] = 0.0;
-      c.partials[2 * bid + 1] = 0.0;
-    }
-    return;  // whole workgroup leaves before any barrier
-  }
-  const int kz = nyq ? N / 2 : (int)blockIdx.x * C::TIX + p;
-  const bool ok = kz < Nh;
-  const int kzs = ok ? kz : Nh - 1;  // in-bounds column for the unconditional loads
-  const int kyi = nyq ? (int)blockIdx.y * C::TIX + p : (int)blockIdx.y;
-  const int ky = kfreq(kyi, N);
-  const int kzv = kz;  // rfft axis: index N/2 carries +N/2 (spectral_solver.py:34-36)
-  FF::fill_tw(s_twf, a.tw, 1);
-  IF::fill_tw(s_twi, a.tw, 1);
-  __syncthreads();
-  const cx<T>* __restrict__ p_us = a.us;
-  const cx<T>* __restrict__ p_un = a.un;
-  cx<T>* __restrict__ p_acc = a.acc;
-  cx<T>* __restrict__ p_out = a.uout;
-  const cx<T> zero = {0, 0};
-
-  cx<T> w[3][NS] = {};  // N^ after the forward FFT, then the new stage field
-  if constexpr (SG >= 0) {
-    // ---- forward x-FFT of the dealiased nonlinear term ----
-    const int KZD = a.KZD, KYD = a.KYD;
-    // N^ is zero outside |k| < kmax: tiles without a kept mode skip the
-    // forward transform altogether (block-uniform)
-    const bool blk_fwd = !nyq && (int)blockIdx.x * C::TIX < KZD && keep_k(kyi, N, KZD);
-    if (blk_fwd) {
-      const int kyd = kyi < KZD ? kyi : kyi - (N - KYD);
-      const bool fwd_ok = ok && kz < KZD;
-      // all three components are requested before the first transform
-      cx<T> pre[3][FF::NIN];
-#pragma unroll
-      for (int cc = 0; cc < 3; ++cc) {
-#pragma unroll
-        for (int s = 0; s < FF::NIN; ++s) {
-          pre[cc][s] = (fwd_ok && FF::in_valid(j, s))
-                           ? ld_stream<NTL>(&a.ny[((size_t)(cc * N + FF::in_index(j, s)) * KYD + kyd) * a.KZP + kz])
-                           : zero;
-        }
-      }
-#pragma unroll
-      for (int cc = 0; cc < 3; ++cc) {
-        cx<T> v[FF::MAXE];
-#pragma unroll
-        for (int s = 0; s < FF::NIN; ++s) v[s] = pre[cc][s];
-        FF::run(v, s_buf + p, C::LS, j, s_twf, 0);
-#pragma unroll
-        for (int s = 0; s < NS; ++s) w[cc][s] = v[s];
-      }
-    }
-
-    // ---- forcing normalisation (FP64) ----
-    T fscale = 0;
-    const int kz0 = blockIdx.x * C::TIX;
-    if (c.force_power && !nyq && kz0 * kz0 <= c.kf2) {
-      // only blocks that hold forced modes need E_f (uniform branch per block)
-      if (ky * ky <= c.kf2) {
-        const double ef = forced_shell_energy3<T, N>(a.us, a.ld_us, c.kf2, s_red) /
-                          ((double)N * N * N * (double)N * N * N);
-        fscale = (T)(c.power / (2.0 * ef + 2.220446049250313e-16));
-      }
-    }
-
-    // ---- pointwise: dealias kx, projection, viscous, forcing, RK update ----
-    const ColAt l_us = ColAt::make<N>(a.ld_us, a.q_us, kzs), l_un = ColAt::make<N>(a.ld_un, a.q_un, kzs);
-    const ColAt l_acc = ColAt::make<N>(a.ld, a.q_acc, kzs), l_out = ColAt::make<N>(a.ld_uout, a.q_uout, kzs);
-    const double wgt = (kz == 0 || kz == N / 2) ? 1.0 : 2.0;
-    double epart[2] = {0.0, 0.0};
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const int kxi = FF::out_index(j, s);
-      const int kx = kfreq(kxi, N);
-      const bool valid = ok && FF::out_valid(j, s);
-      const bool keep_x = keep_k(kxi, N, a.KZD);
-      const size_t row = (size_t)kxi * N + kyi;
-      // loads first, unconditionally (clamped column), for all components
-      cx<T> u3[3], n3[3], c3[3];
-#pragma unroll
-      for (int cc = 0; cc < 3; ++cc) {
-        u3[cc] = ld_stream<NTL>(&p_us[l_us.at(cc, row)]);
-        if constexpr (SG == 1 || SG == 2 || (!VIF && SG == 3)) n3[cc] = ld_stream<NTL>(&p_un[l_un.at(cc, row)]);
-        if constexpr (AccIn<SG, VIF>::value) c3[cc] = ld_stream<NTL>(&p_acc[l_acc.at(cc, row)]);
-      }
-      const cx<T> n0 = keep_x ? w[0][s] : zero;
-      const cx<T> n1 = keep_x ? w[1][s] : zero;
-      const cx<T> n2 = keep_x ? w[2][s] : zero;
-      const int k2 = kx * kx + ky * ky + kzv * kzv;
-      const T inv = k2 == 0 ? (T)0 : (T)1 / (T)k2;
-      // p^ = sum_c N^_c k_c / k^2 (spectral_solver.py:57)
-      const cx<T> pr = {(n0.re * (T)kx + n1.re * (T)ky + n2.re * (T)kzv) * inv,
-                        (n0.im * (T)kx + n1.im * (T)ky + n2.im * (T)kzv) * inv};
-      const T nuk2 = c.nu * (T)k2;
-      const T fsc = (k2 > 0 && k2 <= c.kf2) ? fscale : (T)0;
-      cx<T> du[3];
-      du[0] = csub(n0, cscale(pr, (T)kx));
-      du[1] = csub(n1, cscale(pr, (T)ky));
-      du[2] = csub(n2, cscale(pr, (T)kzv));
-      T eh = 1, e1 = 1;
-      if constexpr (VIF) {  // integrating factor exp(-nu k^2 h), exact for the linear part
-        eh = tpexp(-nuk2 * (T)0.5 * c.dt);
-        e1 = tpexp(-nuk2 * c.dt);
-      }
-#pragma unroll
-      for (int cc = 0; cc < 3; ++cc) {
-        if constexpr (!VIF) du[cc] = csub(du[cc], cscale(u3[cc], nuk2));  // spectral_solver.py:59
-        du[cc] = cadd(du[cc], cscale(u3[cc], fsc));
-        cx<T> out, acc_new = zero;
-        if constexpr (SG == 4) {
-          out = du[cc];
-        } else if constexpr (!VIF) {
-          // spectral_solver.py:74-80 with the accumulator stored once, as an
-          // increment.  The reference's accumulator is u_n plus
-          // dt/6 k1 + dt/3 k2 + dt/3 k3, and dt/6 k1 = (u_1 - u_n)/3,
-          // dt/3 k3 = (u_3 - u_n)/3.  So stage 1 stores only
-          // G = (u_1 - u_n)/3 + dt/3 k2, stage 2 neither reads nor writes an
-          // accumulator, and stage 3 forms
-          // u_{n+1} = u_n + (G + (u_3 - u_n)/3 + dt/6 k4).  The increments
-          // are summed at their own scale and reach u_n in one rounding.
-          constexpr T third = (T)(1.0 / 3.0);
-          if constexpr (SG == 0) {
-            out = cadd(u3[cc], cscale(du[cc], c.cb));
-          } else if constexpr (SG == 1) {
-            acc_new = cadd(cscale(csub(u3[cc], n3[cc]), third), cscale(du[cc], c.ca));
-            out = cadd(n3[cc], cscale(du[cc], c.cb));
-          } else if constexpr (SG == 2) {
-            out = cadd(n3[cc], cscale(du[cc], c.cb));
-          } else {
-            const cx<T> inc = cadd(cadd(c3[cc], cscale(csub(u3[cc], n3[cc]), third)), cscale(du[cc], c.ca));
-            out = cadd(n3[cc], inc);
-          }
-        } else {
-          const T dt = c.dt;
-          if constexpr (SG == 0) {
-            acc_new = cadd(cscale(u3[cc], e1), cscale(du[cc], dt / (T)6 * e1));
-            out = cscale(cadd(u3[cc], cscale(du[cc], dt / (T)2)), eh);
-          } else if constexpr (SG == 1) {
-            acc_new = cadd(c3[cc], cscale(du[cc], dt / (T)3 * eh));
-            out = cadd(cscale(n3[cc], eh), cscale(du[cc], dt / (T)2));
-          } else if constexpr (SG == 2) {
-            acc_new = cadd(c3[cc], cscale(du[cc], dt / (T)3 * eh));
-            out = cadd(cscale(n3[cc], e1), cscale(du[cc], dt * eh));
-          } else {
-            out = cadd(c3[cc], cscale(du[cc], dt / (T)6));
-          }
-        }
-        if (valid) {
-          constexpr bool nt = QL_NT_XS && NTBig<T, N>::value;
-          if constexpr (AccOut<SG, VIF>::value) st_stream<nt>(&p_acc[l_acc.at(cc, row)], acc_new);
-          st_stream<nt>(&p_out[l_out.at(cc, row)], out);
-        }
-        w[cc][s] = valid ? out : zero;
-        if (c.partials != nullptr && valid) {
-          const double m2 = wgt * ((double)out.re * out.re + (double)out.im * out.im);
-          epart[0] += m2;
-          if (k2 > 0 && k2 <= c.kf2) epart[1] += m2;
-        }
-      }
-    }
-    if (c.partials != nullptr) {
-      block_sum<2>(epart, s_red);
-      if (tid == 0) {
-        const int bid = blockIdx.y * gridDim.x + blockIdx.x;
-        c.partials[2 * bid] = epart[0];
-        c.partials[2 * bid + 1] = epart[1];
-      }
-    }
-  } else {
-    // ---- load the state (optionally rescaling the forced shell) ----
-    const double alpha = c.alpha ? *c.alpha : 1.0;
-    const ColAt l_un = ColAt::make<N>(a.ld_un, a.q_un, kzs);
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const size_t row = (size_t)IF::in_index(j, s) * N + kyi;
-#pragma unroll
-      for (int cc = 0; cc < 3; ++cc) w[cc][s] = p_un[l_un.at(cc, row)];
-    }
-    if (c.alpha) {
-#pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        const int kxi = IF::in_index(j, s);
-        const int kx = kfreq(kxi, N);
-        const int k2 = kx * kx + ky * ky + kzv * kzv;
-        const bool masked = ok && IF::in_valid(j, s) && k2 > 0 && k2 <= c.kf2;
-#pragma unroll
-        for (int cc = 0; cc < 3; ++cc) {
-          if (masked) {
-            w[cc][s] = cscale(w[cc][s], (T)alpha);  // spectral_solver.py:112
-            a.rescale_dst[l_un.at(cc, (size_t)kxi * N + kyi)] = w[cc][s];
-          }
-        }
-      }
-    }
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const bool valid = ok && IF::in_valid(j, s);
-      if (!valid) {
-#pragma unroll
-        for (int cc = 0; cc < 3; ++cc) w[cc][s] = zero;
-      } else if (a.copy_dst != nullptr) {
-        const ColAt l_cp = ColAt::make<N>(a.ld_copy, a.q_copy, kz);
-        const size_t row = (size_t)IF::in_index(j, s) * N + kyi;
-#pragma unroll
-        for (int cc = 0; cc < 3; ++cc) a.copy_dst[l_cp.at(cc, row)] = w[cc][s];
-      }
-    }
-  }
-
-  if (a.do_inverse) {
-#pragma unroll
-    for (int f = 0; f < 5; ++f) {
-      cx<T> v[IF::MAXE];
-#pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        const int kx = kfreq(IF::in_index(j, s), N);
-        if (f < 3) {
-          v[s] = w[f][s];
-        } else if (f == 3) {  // omega_y^ = i (kz u^ - kx w^)
-          v[s] = cmuli(csub(cscale(w[0][s], (T)kzv), cscale(w[2][s], (T)kx)));
-        } else {  // omega_z^ = i (kx v^ - ky u^)
-          v[s] = cmuli(csub(cscale(w[1][s], (T)kx), cscale(w[0][s], (T)ky)));
-        }
-      }
-      IF::run(v, s_buf + p, C::LS, j, s_twi, 0);
-      if (ok) {
-        const ColAt l_x5 = ColAt::make<N>(a.ld, a.q_x5, kz);
-#pragma unroll
-        for (int s = 0; s < IF::NOUT; ++s)
-          if (IF::out_valid(j, s))
-            st_stream<QL_NT_X5 && NTBig<T, N>::value>(&a.x5[l_x5.at(f, (size_t)IF::out_index(j, s) * N + kyi)], v[s]);
-      }
-    }
-  }
+k_ns3_x(Pack<NS3X<T>> pk) {
+  const NS3X<T>& a = pk.m[blockIdx.z];
+  // the body is included (not called) so that it compiles as the kernel's own
+  // code, reading its arguments where they lie: as a called function taking
+  // `a`, it compiled to a schedule that spilled 60-80 more registers per lane
+#include "ns3x_body.inc"
 }
 
 // ---------------------------------------------------------------------------
@@ -584,15 +349,18 @@ struct NS3Y {
   const cx<T>* tw;
 };
 
+// LDS of the y passes, declared by each kernel that runs the shared body (an
+// LDS variable declared in the body would be one object used by two kernels)
+#define QL_YI_LDS                                                              \
+  __shared__ cx<T> s_tw[XInv<T, N>::TWL > 0 ? XInv<T, N>::TWL : 1];          \
+  __shared__ cx<T> s_buf[N * XCfg<N, V>::LS]
+
 template <typename T, int N, int V = 0>
-__global__ void __launch_bounds__((XCfg<N, V>::THREADS)) QL_ATTR_YI
-k_ns3_yinv(NS3Y<T> a) {
+__device__ __forceinline__ void ns3_yinv_run(const NS3Y<T>& a, cx<T>* s_tw, cx<T>* s_buf) {
   using C = XCfg<N, V>;
   constexpr bool NTL = QL_NTL && NTBig<T, N>::value;
   constexpr int Nh = N / 2 + 1;
   using F = XInv<T, N>;
-  __shared__ cx<T> s_tw[F::TWL > 0 ? F::TWL : 1];
-  __shared__ cx<T> s_buf[N * C::LS];
   const int tid = threadIdx.x, p = tid % C::TIX, j = tid / C::TIX;
   // tiles 0..NTR-1: TIX consecutive kz < N/2 of one x row; tile NTR: the
   // kz = N/2 column of TIX consecutive x rows
@@ -671,6 +439,13 @@ k_ns3_yinv(NS3Y<T> a) {
     F::run(v, s_buf + p, C::LS, j, s_tw, 0);
     store(2, v);
   }
+}
+
+template <typename T, int N, int V = 0>
+__global__ void __launch_bounds__((XCfg<N, V>::THREADS)) QL_ATTR_YI
+k_ns3_yinv(Pack<NS3Y<T>> pk) {
+  QL_YI_LDS;
+  ns3_yinv_run<T, N, V>(pk.m[blockIdx.z], s_tw, s_buf);
 }
 
 // ---------------------------------------------------------------------------
@@ -991,17 +766,26 @@ __device__ __forceinline__ void ns3_z_body(const NS3Z<T>& a, cx<T>* s_tw, cx<T>*
   }
 }
 
+#define QL_Z_LDS                                                                            \
+  using St = ZState<T, N, V>;                                                               \
+  __shared__ cx<T> s_tw[N];                                                                 \
+  __shared__ cx<T> s_twi[St::CI::TWL > 0 ? St::CI::TWL : 1];                                \
+  __shared__ cx<T> s_twf[St::CF::TWL > 0 ? St::CF::TWL : 1];                                \
+  __shared__ cx<T> s_buf[Z3Cfg<T, N, V>::P * Z3Cfg<T, N, V>::LSR]
+
 template <typename T, int N, int V = 0>
-__global__ void __launch_bounds__((Z3Cfg<T, N, V>::THREADS)) QL_ATTR_Z
-k_ns3_z(NS3Z<T> a) {
-  using St = ZState<T, N, V>;
-  __shared__ cx<T> s_tw[N];
-  __shared__ cx<T> s_twi[St::CI::TWL > 0 ? St::CI::TWL : 1];
-  __shared__ cx<T> s_twf[St::CF::TWL > 0 ? St::CF::TWL : 1];
-  __shared__ cx<T> s_buf[Z3Cfg<T, N, V>::P * Z3Cfg<T, N, V>::LSR];
+__device__ __forceinline__ void ns3_z_run(const NS3Z<T>& a, cx<T>* s_tw, cx<T>* s_twi, cx<T>* s_twf,
+                                          cx<T>* s_buf) {
   // the register prefetch ring is indexed by compile-time field numbers only
   if (a.u_in != nullptr) ns3_z_body<T, N, true, V>(a, s_tw, s_twi, s_twf, s_buf);
   else ns3_z_body<T, N, false, V>(a, s_tw, s_twi, s_twf, s_buf);
+}
+
+template <typename T, int N, int V = 0>
+__global__ void __launch_bounds__((Z3Cfg<T, N, V>::THREADS)) QL_ATTR_Z
+k_ns3_z(Pack<NS3Z<T>> pk) {
+  QL_Z_LDS;
+  ns3_z_run<T, N, V>(pk.m[blockIdx.z], s_tw, s_twi, s_twf, s_buf);
 }
 
 // ---------------------------------------------------------------------------
@@ -1015,14 +799,15 @@ struct NS3YF {
   const cx<T>* tw;
 };
 
+#define QL_YF_LDS                                                              \
+  __shared__ cx<T> s_tw[XFwd<T, N>::TWL > 0 ? XFwd<T, N>::TWL : 1];          \
+  __shared__ cx<T> s_buf[N * XCfg<N, V>::LS]
+
 template <typename T, int N, int V = 0>
-__global__ void __launch_bounds__((XCfg<N, V>::THREADS)) QL_ATTR_YF
-k_ns3_yfwd(NS3YF<T> a) {
+__device__ __forceinline__ void ns3_yfwd_run(const NS3YF<T>& a, cx<T>* s_tw, cx<T>* s_buf) {
   using C = XCfg<N, V>;
   constexpr bool NTL = QL_NTL && NTBig<T, N>::value;
   using F = XFwd<T, N>;
-  __shared__ cx<T> s_tw[F::TWL > 0 ? F::TWL : 1];
-  __shared__ cx<T> s_buf[N * C::LS];
   const int tid = threadIdx.x, p = tid % C::TIX, j = tid / C::TIX;
   const int kz = blockIdx.x * C::TIX + p;
   const bool ok = kz < a.KZD;
@@ -1056,6 +841,13 @@ k_ns3_yfwd(NS3YF<T> a) {
       }
     }
   }
+}
+
+template <typename T, int N, int V = 0>
+__global__ void __launch_bounds__((XCfg<N, V>::THREADS)) QL_ATTR_YF
+k_ns3_yfwd(Pack<NS3YF<T>> pk) {
+  QL_YF_LDS;
+  ns3_yfwd_run<T, N, V>(pk.m[blockIdx.z], s_tw, s_buf);
 }
 
 }  // namespace qk

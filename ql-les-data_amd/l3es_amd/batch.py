@@ -2,10 +2,13 @@
 
 The reference advances one trajectory per process and GPU (SLURM arrays,
 ``slurm_*.sh:5``).  ``Batch`` advances many independent trajectories -- each
-with its own plan, state and optional particles -- together: whole chunks of
-steps replay a HIP graph captured once per member, spread over a few streams.
-Each trajectory is bit-identical to the same steps taken alone with
-``_Step.advance`` / ``qlles_step_advect``.
+with its own plan, state and optional particles -- together, either member by
+member (HIP graphs of each member's steps replayed over a few streams) or
+grouped: members of one kind (dim, grid, precision, viscous mode, kernel
+variants, interpolator settings) are stepped by multi-member launches, one
+launch per kernel of the step for the whole group.  Each trajectory is
+bit-identical to the same steps taken alone with ``_Step.advance`` /
+``qlles_step_advect``.
 """
 
 import ctypes
@@ -41,11 +44,16 @@ class Batch:
         self.members.append((step, u_hat, interp, r, u_r))
         return idx.value
 
+    MODES = {False: 0, True: 1, "whole": 2, "grouped": 3, "grouped-whole": 4}
+
     def configure(self, lanes, chunk, single_graph=False):
-        """lanes streams, chunk steps per graph; single_graph False / True (one graph per chunk of every
-        member) / "whole" (one graph of every member's whole run: one launch per run)."""
-        mode = 2 if single_graph == "whole" else (1 if single_graph else 0)
-        _lib.check(self._lib.qlles_batch_configure(self.handle, int(lanes), int(chunk), mode))
+        """lanes streams, chunk steps per graph; single_graph False (per-member graphs) / True (one graph per
+        chunk of every member) / "whole" (one graph of every member's whole run: one launch per run) /
+        "grouped" (multi-member launches per group of one kind, eager, groups interleaved chunk steps at a
+        time) / "grouped-whole" (the grouped run captured into one graph: one launch per run)."""
+        if single_graph not in self.MODES:
+            raise ValueError(f"single_graph must be one of {list(self.MODES)}")
+        _lib.check(self._lib.qlles_batch_configure(self.handle, int(lanes), int(chunk), self.MODES[single_graph]))
 
     def tune(self, trial_steps=16):
         """Pick streams x steps-per-graph by timing on scratch copies (trajectories do not move)."""
@@ -56,7 +64,7 @@ class Batch:
         lanes, chunk, single, us = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         _lib.check(self._lib.qlles_batch_get_config(self.handle, ctypes.byref(lanes), ctypes.byref(chunk),
                                                     ctypes.byref(single), ctypes.byref(us)))
-        mode = {0: False, 1: True, 2: "whole"}[single.value]
+        mode = {v: k for k, v in self.MODES.items()}[single.value]
         return dict(lanes=lanes.value, chunk=chunk.value, single_graph=mode, us_per_traj_step=us.value)
 
     def run(self, nsteps):

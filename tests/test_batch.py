@@ -34,8 +34,11 @@ def test_batch_validation_without_device():
         assert lib.qlles_batch_add(h, None, None, None, None, 0, None, None) == _lib.QLLES_ERR_ARG
         assert lib.qlles_batch_configure(h, 0, 1, 0) == _lib.QLLES_ERR_ARG
         assert lib.qlles_batch_configure(h, 2, 0, 0) == _lib.QLLES_ERR_ARG
+        assert lib.qlles_batch_configure(h, 2, 4, 3) == 0
+        assert lib.qlles_batch_configure(h, 2, 4, 4) == 0
+        assert lib.qlles_batch_configure(h, 2, 4, 5) == _lib.QLLES_ERR_ARG
+        assert lib.qlles_batch_configure(h, 2, 4, -1) == _lib.QLLES_ERR_ARG
         assert lib.qlles_batch_configure(h, 2, 4, 1) == 0
-        assert lib.qlles_batch_configure(h, 2, 4, 3) == _lib.QLLES_ERR_ARG
         lanes, chunk, single, us = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         assert lib.qlles_batch_get_config(h, ctypes.byref(lanes), ctypes.byref(chunk), ctypes.byref(single),
                                           ctypes.byref(us)) == 0
@@ -108,7 +111,7 @@ def test_batch_bit_identical_to_solo(gpu):
     b.run([1, 2, 0, 1])  # captured again
     cfg = b.tune(trial_steps=4)  # runs on scratch copies: states must not move
     assert cfg["lanes"] in (1, 2, 4, 8, 16) and cfg["chunk"] in (1, 4, 16, 64) and cfg["us_per_traj_step"] > 0
-    assert cfg["single_graph"] in (False, True)
+    assert cfg["single_graph"] in (False, True, "grouped", "grouped-whole")
     b.run(5)
     torch.cuda.synchronize()
     for k, ((step, _, it, _), (uh_s, r_s), (uh_b, r_b), ur_b) in enumerate(zip(mem, solo, bat, urs)):
@@ -177,3 +180,92 @@ def test_variant_cache_roundtrip(gpu, tmp_path):
     json.dump(dict(cfg, xv=1 - cfg["xv"]), open(pref + ".json", "w"))
     custom_jit_integrate(pref, False, step2, None, uh2)
     assert step2.plan.variant()[0] == 1 - cfg["xv"]
+
+
+def _grouped_members():
+    """Members of a few kinds, several of each: 3-D FP32 power-forced HIT at 32^3 with NUFFT particles
+    (different seeds, viscosities, time steps and powers), 2-D FP32 Kolmogorov at 64^2 with NUFFT
+    particles, 3-D FP64 Taylor-Green at 16^3 without particles, 2-D FP64 Taylor-Green at 32^2 with DFT
+    particles (advected member by member), a 3-D member held at another kernel variant (a group of its
+    own) and an ekin_tot-forced member (runs alone)."""
+    from l3es_amd import set_up_solver
+    from l3es_amd.integrator import pos_init_jittered, spectral_interpolator_wrapper
+
+    specs = [
+        # (N, dim, case, precision, nu, dt, kwargs, particle backend, n, variant)
+        (32, 3, "HIT", "float32", 2e-3, 2e-3, dict(forcing_type="power", power=0.1, kf=2), "nufft", 16, (0, 0)),
+        (64, 2, "Kolm", "float32", 1e-3, 1e-3,
+         dict(forcing_type="kolmogorov", drag=0.1, kolm_amp=1.0, kolm_k=4, kf=None), "nufft", 32, (0, 0)),
+        (32, 3, "HIT", "float32", 1e-3, 1e-3, dict(forcing_type="power", power=0.3, kf=2), "nufft", 16, (0, 0)),
+        (16, 3, "TGV3D", "float64", 2e-3, 2e-3, dict(forcing_type="none", kf=None), None, 0, (0, 0)),
+        (32, 3, "HIT", "float32", 3e-3, 2e-3, dict(forcing_type="power", power=0.2, kf=1), "nufft", 16, (0, 0)),
+        (64, 2, "Kolm", "float32", 2e-3, 1e-3,
+         dict(forcing_type="kolmogorov", drag=0.05, kolm_amp=0.5, kolm_k=2, kf=None), "nufft", 32, (0, 0)),
+        (16, 3, "TGV3D", "float64", 1e-3, 1e-3, dict(forcing_type="none", kf=None), None, 0, (0, 0)),
+        (32, 2, "TGV2D", "float64", 2e-3, 2e-3, dict(forcing_type="none", kf=None), "dft", 16, (0, 0)),
+        (32, 2, "TGV2D", "float64", 1e-3, 2e-3, dict(forcing_type="none", kf=None), "dft", 16, (0, 0)),
+        (32, 3, "HIT", "float32", 2e-3, 2e-3, dict(forcing_type="power", power=0.1, kf=2), "nufft", 16, (1, 1)),
+        (16, 3, "HIT", "float64", 2e-3, 2e-3, dict(forcing_type="ekin_tot", e_kin_target=0.5, kf=2), "nufft", 8,
+         (0, 0)),
+    ]
+    out = []
+    for i, (N, dim, case, prec, nu, dt, kw, be, n, var) in enumerate(specs):
+        kw = dict(kw)
+        kf = kw.pop("kf")
+        _, uh, _, _, step, _, _, _, _ = set_up_solver(N, nu, dim, case, dt, 30 + i, max(n, 8), kf, precision=prec,
+                                                      warmup=False, **kw)
+        step.plan.set_variant(*var)
+        it = r = None
+        if be is not None:
+            it = spectral_interpolator_wrapper(n, tuple(range(1, dim + 1)), backend=be, precision=prec)
+            r = torch.from_numpy(pos_init_jittered(n, dim, seed=i)).cuda()
+        out.append((step, uh, it, r))
+    return out
+
+
+@pytest.mark.gpu
+def test_batch_grouped_bit_identical_to_solo(gpu):
+    """Grouped modes: multi-member launches per group of one kind (eager, and the whole run as one graph),
+    with uneven lengths (members finishing at different steps, zero-step members), group tables remade when
+    the order of the step counts changes, and a tuner run in between; every member equals its solo run."""
+    from l3es_amd import Batch
+
+    mem = _grouped_members()
+    M = len(mem)
+    solo = [(uh.clone(), None if r is None else r.clone()) for (_, uh, _, r) in mem]
+    bat = [(uh.clone(), None if r is None else r.clone()) for (_, uh, _, r) in mem]
+    b = Batch()
+    urs = []
+    for (step, _, it, _), (uh, r) in zip(mem, bat):
+        ur = None if it is None else torch.empty((r.shape[0], it.dim), dtype=it.dtype, device=r.device)
+        urs.append(ur)
+        b.add(step, uh, it, r, ur)
+    runs = []
+
+    def run(cfg, lengths, repeat=1):
+        if cfg is not None:
+            b.configure(**cfg)
+        for _ in range(repeat):
+            b.run(lengths)
+            runs.append(lengths)
+
+    run(dict(lanes=2, chunk=2, single_graph="grouped"), [5, 3, 7, 4, 2, 6, 0, 3, 5, 4, 3])
+    run(None, [2, 4, 1, 3, 5, 2, 2, 1, 0, 2, 2])  # another order within the groups: tables remade
+    run(dict(lanes=3, chunk=1, single_graph="grouped-whole"), [4, 2, 3, 1, 4, 5, 2, 3, 3, 1, 2], repeat=2)
+    run(None, [1, 3, 0, 2, 2, 1, 3, 0, 1, 2, 1])  # captured again
+    run(dict(lanes=16, chunk=4, single_graph="grouped"), [6] * M)
+    cfg = b.tune(trial_steps=4)  # scratch copies: the states do not move
+    assert cfg["us_per_traj_step"] > 0
+    b.configure(lanes=4, chunk=1, single_graph="grouped-whole")
+    run(None, [3, 1, 2, 2, 1, 3, 1, 2, 2, 1, 1])
+    torch.cuda.synchronize()
+    for k, ((step, _, it, _), (uh_s, r_s), (uh_b, r_b), ur_b) in enumerate(zip(mem, solo, bat, urs)):
+        ur = None
+        for lengths in runs:
+            if lengths[k]:
+                ur = _solo(step, uh_s, it, r_s, lengths[k])
+        torch.cuda.synchronize()
+        assert np.array_equal(host(uh_s), host(uh_b)), f"member {k}: spectrum differs from solo run"
+        if it is not None:
+            assert np.array_equal(host(r_s), host(r_b)), f"member {k}: positions differ"
+            assert np.array_equal(host(ur), host(ur_b)), f"member {k}: particle velocities differ"

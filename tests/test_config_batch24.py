@@ -88,42 +88,57 @@ def test_batch24_bit_identical_to_solo_and_energy_vs_oracle(gpu):
     cfg_t = b.tune(trial_steps=8)  # on scratch copies: the states do not move
     print(f"[batch24] tuned {cfg_t}", flush=True)
     b.run(lengths)
+    # then both grouped modes explicitly (multi-member launches, 7 groups of 3-4 members), other lengths
+    more = [[3 + (i % 3) for i in range(M)], [2 + ((i + 1) % 4) for i in range(M)]]
+    b.configure(lanes=4, chunk=2, single_graph="grouped")
+    b.run(more[0])
+    b.configure(lanes=8, chunk=1, single_graph="grouped-whole")
+    b.run(more[1])
     torch.cuda.synchronize()
     for i, ((step, uh_b, it, r_b), (uh_s, r_s)) in enumerate(zip(mem, solo)):
-        ur_s = _advect_solo(step, it, uh_s, r_s, lengths[i])
+        for k in (lengths[i], more[0][i], more[1][i]):
+            ur_s = _advect_solo(step, it, uh_s, r_s, k)
         torch.cuda.synchronize()
         assert np.array_equal(host(uh_s), host(uh_b)), f"member {i}: spectrum differs from its solo run"
         assert np.array_equal(host(r_s), host(r_b)), f"member {i}: positions differ"
         assert np.array_equal(host(ur_s), host(urs[i])), f"member {i}: particle velocities differ"
     print("[batch24] every member bit-identical to its solo run", flush=True)
     worst = 0.0
+    total = [a + b_ + c for a, b_, c in zip(lengths, *more)]
     for k in range(len(SIZES)):
-        i = min(range(k, M, len(SIZES)), key=lambda j: lengths[j])  # shortest member of this size
-        _, uho, _ = oracle_run(i, lengths[i])
+        i = min(range(k, M, len(SIZES)), key=lambda j: total[j])  # shortest member of this size
+        _, uho, _ = oracle_run(i, total[i])
         dim, N = SIZES[k]
         e_o = O.kinetic_energy_hat(uho, N, dim)
         e_g = kinetic_energy_hat(mem[i][1])[0]
         err = abs(e_g - e_o) / abs(e_o)
         worst = max(worst, err)
-        print(f"[batch24] member {i} ({dim}-D {N}, {lengths[i]} steps): FP32 energy vs FP64 oracle {err:.2e}",
+        print(f"[batch24] member {i} ({dim}-D {N}, {total[i]} steps): FP32 energy vs FP64 oracle {err:.2e}",
               flush=True)
     assert worst < 1e-5, worst
 
 
-def test_batch_fp64_one_member_per_size_matches_oracle(gpu):
+@pytest.mark.parametrize("mode", ["per-member", "grouped"])
+def test_batch_fp64_one_member_per_size_matches_oracle(gpu, mode):
+    """per-member: one member per grid size, whole chunks by graph replay and the remainders eagerly;
+    grouped: two members per grid size (other seeds) stepped by multi-member launches in one graph."""
     from l3es_amd import Batch
 
-    lengths = [1, 2, 3, 1, 2, 1, 2]
-    mem = [make(i, "float64") for i in range(len(SIZES))]
+    idx = list(range(len(SIZES))) if mode == "per-member" else list(range(2 * len(SIZES)))
+    lengths = [[1, 2, 3, 1, 2, 1, 2, 2, 1, 2, 3, 1, 2, 1][j] for j in range(len(idx))]
+    mem = [make(i, "float64") for i in idx]
     r0 = [host(r).copy() for (_, _, _, r) in mem]
     b = Batch()
     for step, uh, it, r in mem:
         b.add(step, uh, it, r)
-    b.configure(lanes=4, chunk=2)  # whole chunks by graph replay, the remainders eagerly
+    if mode == "per-member":
+        b.configure(lanes=4, chunk=2)
+    else:
+        b.configure(lanes=4, chunk=1, single_graph="grouped-whole")
     b.run(lengths)
     torch.cuda.synchronize()
-    for i, (step, uh, it, r) in enumerate(mem):
-        dim, N = SIZES[i]
+    for i, (step, uh, it, r) in zip(idx, mem):
+        dim, N = SIZES[i % len(SIZES)]
         _, uho, ro = oracle_run(i, lengths[i], r0[i])
         err = np.abs(host(uh) - uho).max() / np.abs(uho).max()
         assert err < 1e-12, (dim, N, err)
