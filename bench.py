@@ -260,6 +260,7 @@ def run_batch(args):
     for (dim, N, step, uh, it, r) in mem:
         b.add(step, uh, it, r)
     cfg = b.tune(trial_steps=64)
+    kind_of, kind_us = b.kinds()
     # warm-up with the timed step counts (a whole-run graph is captured for them here)
     b.run(lengths)
     torch.cuda.synchronize()
@@ -290,6 +291,7 @@ def run_batch(args):
                                "HIT 64^3-128^3 (16^3 particles), uneven lengths K..1.75K, NUFFT tol 1e-6",
                    "sizes": [f"{d}D {N}" for (d, N, *_rest) in mem], "lengths": lengths},
         "batched_tuned": cfg, "solo_sequential_traj_steps_per_s": total / t_solo, "speedup_vs_solo": t_solo / t_b,
+        "kinds": {"kind_of_member": kind_of, "us_per_step_alone": kind_us},
         "kernel_variants": variants,
         "ke_rel_err_fp32_vs_fp64_worst": max(errs), "ke_rel_err_fp32_vs_fp64": errs,
     }

@@ -261,14 +261,22 @@ int qlles_batch_add(qlles_batch* b, qlles_plan* plan, qlles_interp* it, void* u_
  * within a group or a member's kernel variants change. */
 int qlles_batch_configure(qlles_batch* b, int lanes, int chunk, int single_graph);
 /* autotuner: for each member, its plan's kernel variants (qlles_plan_tune);
- * then single_graph in {0,1} x lanes in {1,2,4,8,16} x chunk in {1,4,16,64},
- * single_graph = 3 x lanes x chunk in {1,4,16} and single_graph = 4 x lanes
- * (lanes up to the number of groups),
+ * then each kind's group alone, timed per step (grouped modes place groups on
+ * streams longest first by these times); then single_graph in {0,1} x lanes
+ * in {1,2,4,8,16} x chunk in {1,4,16,64}, single_graph = 3 x lanes in
+ * {1,2,3,4,6,8,16} x chunk in {1,4,16} and single_graph = 4 x lanes (lanes up
+ * to the number of groups),
  * timed over about trial_steps steps per member on scratch copies of the
  * member states (the trajectories do not move); keeps the fastest */
 int qlles_batch_tune(qlles_batch* b, int trial_steps, void* stream);
 /* current configuration and the tuned time per trajectory-step (0 if untuned) */
 int qlles_batch_get_config(const qlles_batch* b, int* lanes, int* chunk, int* single_graph, double* us_per_traj_step);
+/* the kinds the grouped modes step together: *nkinds, kind_of[i] (optional,
+ * one per member) the kind of member i, and us_per_step[k] (optional, up to
+ * cap entries) the time per step of kind k's group alone as the tuner
+ * measured it (0 before tuning).  No reference counterpart (a scheduling
+ * report). */
+int qlles_batch_kinds(const qlles_batch* b, int* nkinds, int* kind_of, double* us_per_step, int cap);
 /* nsteps: per-member step counts (NULL: every member takes nsteps_all).
  * Stream-ordered with respect to `stream` (work forks onto the batch's
  * streams and joins back). */

@@ -53,6 +53,10 @@ struct qlles_batch {
   };
   std::vector<Group> groups;
   std::vector<int> group_perm;  // member order the groups were made for
+  // microseconds per step of each kind's group run alone (in the order the
+  // kinds first appear among the members), measured by the tuner; empty:
+  // estimated from the grid sizes
+  std::vector<double> kind_us;
   std::vector<hipStream_t> streams;
   std::vector<hipEvent_t> join;
   hipEvent_t fork = nullptr;
@@ -292,19 +296,52 @@ void group_steps(qlles_batch* b, qlles_batch::Group& g, const std::vector<int>& 
   g.eg->steps(ns.data(), from, to, s, after);
 }
 
-// the grouped run of step counts n: group i on stream i % lanes, the groups
-// interleaved `chunk` steps at a time so every stream has queued work
+// time per step of group g run alone: measured by the tuner, else estimated
+// as the longer of its launch chain (about 6 us per kernel) and its bytes at
+// 4 TB/s (3-D: 155 B per grid point and byte of precision, the fused step's
+// traffic; 2-D: 50)
+double group_us(const qlles_batch* b, size_t gi) {
+  if (gi < b->kind_us.size() && b->kind_us[gi] > 0) return b->kind_us[gi];
+  const auto& g = b->groups[gi];
+  const ql::Desc& d = b->m[g.mem[0]].p->desc;
+  const double pts = d.dim == 3 ? (double)d.N * d.N * d.N : (double)d.N * d.N;
+  const double eb = d.dtype == QLLES_F32 ? 4.0 : 8.0;
+  const double bytes = (d.dim == 3 ? 155.0 : 50.0) * eb * pts * (double)g.mem.size();
+  return std::max((d.dim == 3 ? 22.0 : 12.0) * 6.0, bytes / 4e6);
+}
+
+// the grouped run of step counts n.  Groups go to the lanes longest first,
+// each to the lane with the least work so far (a group's work: its time per
+// step times its longest member's steps), so the heaviest kinds do not share
+// a stream -- and with it a hardware queue, which runs its kernels in order
+// -- with others.  The groups are interleaved `chunk` steps at a time so
+// every stream has queued work.
 void enqueue_grouped(qlles_batch* b, const std::vector<int>& n) {
-  std::vector<std::vector<int>> ns(b->groups.size());
+  const size_t G = b->groups.size();
+  std::vector<std::vector<int>> ns(G);
   int maxn = 0;
-  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+  for (size_t gi = 0; gi < G; ++gi) {
     for (int i : b->groups[gi].mem) ns[gi].push_back(n[i]);
     maxn = std::max(maxn, ns[gi][0]);
   }
+  std::vector<size_t> order(G);
+  std::vector<double> work(G);
+  for (size_t gi = 0; gi < G; ++gi) {
+    order[gi] = gi;
+    work[gi] = group_us(b, gi) * ns[gi][0];
+  }
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return work[x] > work[y]; });
+  std::vector<int> lane(G, 0);
+  std::vector<double> load(b->lanes, 0.0);
+  for (size_t gi : order) {
+    const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    lane[gi] = l;
+    load[l] += work[gi];
+  }
   for (int k = 0; k < maxn; k += b->chunk)
-    for (size_t gi = 0; gi < b->groups.size(); ++gi)
+    for (size_t gi : order)
       if (ns[gi][0] > k)
-        group_steps(b, b->groups[gi], ns[gi], k, std::min(k + b->chunk, ns[gi][0]), b->streams[gi % b->lanes]);
+        group_steps(b, b->groups[gi], ns[gi], k, std::min(k + b->chunk, ns[gi][0]), b->streams[lane[gi]]);
 }
 
 hipGraphExec_t capture_grouped(qlles_batch* b, const std::vector<int>& n) {
@@ -475,6 +512,7 @@ int qlles_batch_add(qlles_batch* b, qlles_plan* p, qlles_interp* it, void* u_hat
                           it ? (size_t)np * dim * sizeof(double) : 0, it ? (size_t)np * dim * elem_bytes(p) : 0};
     b->m.push_back(x);
     drop_graphs(b);
+    b->kind_us.clear();
     if (index) *index = (int)b->m.size() - 1;
   });
 }
@@ -498,6 +536,19 @@ int qlles_batch_get_config(const qlles_batch* b, int* lanes, int* chunk, int* si
   if (single_graph) *single_graph = b->single;
   if (us_per_traj_step) *us_per_traj_step = b->tuned_us_per_traj_step;
   return QLLES_OK;
+}
+
+int qlles_batch_kinds(const qlles_batch* b, int* nkinds, int* kind_of, double* us_per_step, int cap) {
+  if (!b || !nkinds) return fail(QLLES_ERR_ARG, "null argument");
+  return guarded([&] {
+    const auto parts = partition(b);
+    *nkinds = (int)parts.size();
+    for (size_t k = 0; k < parts.size(); ++k) {
+      if (kind_of)
+        for (int i : parts[k]) kind_of[i] = (int)k;
+      if (us_per_step && (int)k < cap) us_per_step[k] = k < b->kind_us.size() ? b->kind_us[k] : 0.0;
+    }
+  });
 }
 
 int qlles_batch_run(qlles_batch* b, const int* nsteps, int nsteps_all, void* stream) {
@@ -551,6 +602,31 @@ int qlles_batch_tune(qlles_batch* b, int trial_steps, void* stream) {
     QL_HIP(hipEventCreate(&e0));
     QL_HIP(hipEventCreate(&e1));
     try {
+      // each kind's group alone, timed per step: the costs that spread the
+      // groups over the streams in the grouped modes
+      {
+        t.single = 3;
+        t.lanes = 1;
+        t.chunk = 1;
+        make_streams(&t);
+        const std::vector<int> even(nm, trial_steps);
+        make_groups(&t, even);
+        std::vector<double> us(t.groups.size(), 0.0);
+        for (size_t gi = 0; gi < t.groups.size(); ++gi) {
+          std::vector<int> ns(t.groups[gi].mem.size(), trial_steps);
+          group_steps(&t, t.groups[gi], ns, 0, 1, caller);  // warm-up
+          QL_HIP(hipEventRecord(e0, caller));
+          group_steps(&t, t.groups[gi], ns, 0, trial_steps, caller);
+          QL_HIP(hipEventRecord(e1, caller));
+          QL_HIP(hipEventSynchronize(e1));
+          float ms = 0.f;
+          QL_HIP(hipEventElapsedTime(&ms, e0, e1));
+          us[gi] = 1e3 * ms / trial_steps;
+        }
+        t.kind_us = us;
+        b->kind_us = us;
+        drop_graphs(&t);
+      }
       // the whole-run graph (single = 2) is not a candidate: the trials have
       // even lengths, where it times well, but members sharing a stream run
       // their whole trajectories back to back, which uneven lengths turn into
@@ -560,7 +636,8 @@ int qlles_batch_tune(qlles_batch* b, int trial_steps, void* stream) {
       // interleaving of the eager mode and has no effect on the graph
       const int ngroups = (int)partition(b).size();
       for (int single : {0, 1, 3, 4})
-      for (int lanes : {1, 2, 4, 8, 16}) {
+      for (int lanes : {1, 2, 3, 4, 6, 8, 16}) {
+        if (single < 3 && (lanes == 3 || lanes == 6)) continue;
         if (lanes > (single >= 3 ? ngroups : nm) && lanes > 1) break;
         for (int chunk : {1, 4, 16, 64}) {
           if (chunk > trial_steps && chunk > 1) break;

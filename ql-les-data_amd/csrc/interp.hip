@@ -657,6 +657,9 @@ void setup_t(qlles_interp* it) {
   if (it->backend == QLLES_INTERP_NUFFT) {
     QL_HIP(hipMalloc(&it->coef, sizeof(cx<T>) * per_f * d));
     QL_HIP(hipMemset(it->coef, 0, sizeof(cx<T>) * per_f * d));
+    // the fill runs on the null stream, which does not order itself with the
+    // non-blocking streams a batch launches on: finish it here
+    QL_HIP(hipDeviceSynchronize());
     QL_HIP(hipMalloc(&it->fine, sizeof(cx<T>) * per_f * d));
     if (d == 3) QL_HIP(hipMalloc(&it->packed, sizeof(T) * 4 * per_f));
   } else {
@@ -940,6 +943,9 @@ void group_setup_t(InterpGroup* g, const std::vector<qlles_interp*>& its, int N,
   const size_t per_f = d == 3 ? (size_t)nf * nf * nf : (size_t)nf * nf;
   QL_HIP(hipMalloc(&g->coef, sizeof(cx<T>) * per_f * d * M));
   QL_HIP(hipMemset(g->coef, 0, sizeof(cx<T>) * per_f * d * M));  // the padding stays zero
+  // the fill runs on the null stream, unordered with the batch's non-blocking
+  // streams that first use the grids: finish it before returning
+  QL_HIP(hipDeviceSynchronize());
   QL_HIP(hipMalloc(&g->fine, sizeof(cx<T>) * per_f * d * M));
   if (d == 3) QL_HIP(hipMalloc(&g->packed, sizeof(T) * 4 * per_f * M));
   const double scale = 1.0 / (d == 3 ? (double)N * N * N : (double)N * N);

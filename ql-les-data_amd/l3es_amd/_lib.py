@@ -32,7 +32,7 @@ EXPORTS = (
     "qlles_energy_hat", "qlles_filter", "qlles_interp_create", "qlles_interp_destroy",
     "qlles_interp_field", "qlles_interp_hat", "qlles_last_error", "qlles_profile", "qlles_profile_read",
     "qlles_step_advect", "qlles_batch_create", "qlles_batch_destroy", "qlles_batch_add",
-    "qlles_batch_configure", "qlles_batch_tune", "qlles_batch_get_config", "qlles_batch_run",
+    "qlles_batch_configure", "qlles_batch_tune", "qlles_batch_get_config", "qlles_batch_kinds", "qlles_batch_run",
     "qlles_rhs_jvp", "qlles_rhs_vjp", "qlles_step_jvp", "qlles_step_vjp", "qlles_irfftn_adjoint",
     "qlles_rfftn_adjoint", "qlles_interp_hat_vjp", "qlles_plan_set_variant", "qlles_plan_get_variant",
     "qlles_plan_tune", "qlles_energy_spectrum", "qlles_sph_create", "qlles_sph_destroy", "qlles_sph_density",
@@ -100,6 +100,7 @@ def load():
         "qlles_batch_configure": (ci, [vp, ci, ci, ci]),
         "qlles_batch_tune": (ci, [vp, ci, vp]),
         "qlles_batch_get_config": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci), pd]),
+        "qlles_batch_kinds": (ci, [vp, ctypes.POINTER(ci), vp, vp, ci]),
         "qlles_batch_run": (ci, [vp, vp, ci, vp]),
         "qlles_rhs_jvp": (ci, [vp, vp, vp, vp, vp]),
         "qlles_rhs_vjp": (ci, [vp, vp, vp, vp, vp, vp]),

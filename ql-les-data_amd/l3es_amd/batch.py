@@ -67,6 +67,16 @@ class Batch:
         mode = {v: k for k, v in self.MODES.items()}[single.value]
         return dict(lanes=lanes.value, chunk=chunk.value, single_graph=mode, us_per_traj_step=us.value)
 
+    def kinds(self):
+        """The kinds the grouped modes step together: (kind of each member, time per step of each kind's
+        group alone in microseconds as the tuner measured it, 0 before tuning)."""
+        nk = ctypes.c_int()
+        _lib.check(self._lib.qlles_batch_kinds(self.handle, ctypes.byref(nk), None, None, 0))
+        kind_of = (ctypes.c_int * max(1, len(self.members)))()
+        us = (ctypes.c_double * max(1, nk.value))()
+        _lib.check(self._lib.qlles_batch_kinds(self.handle, ctypes.byref(nk), kind_of, us, nk.value))
+        return list(kind_of)[:len(self.members)], list(us)[:nk.value]
+
     def run(self, nsteps):
         """Advance every member: ``nsteps`` an int (all the same) or a list (uneven lengths)."""
         if isinstance(nsteps, int):

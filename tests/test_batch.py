@@ -240,6 +240,10 @@ def test_batch_grouped_bit_identical_to_solo(gpu):
         ur = None if it is None else torch.empty((r.shape[0], it.dim), dtype=it.dtype, device=r.device)
         urs.append(ur)
         b.add(step, uh, it, r, ur)
+    # kinds: 3-D FP32 HIT 32^3 (0, 2, 4), 2-D FP32 Kolmogorov 64^2 (1, 5), 3-D FP64 TGV 16^3 (3, 6), 2-D FP64
+    # TGV 32^2 with DFT particles (7, 8), the member at another kernel variant (9), the ekin_tot member (10)
+    kind_of, us = b.kinds()
+    assert kind_of == [0, 1, 0, 2, 0, 1, 2, 3, 3, 4, 5] and us == [0.0] * 6
     runs = []
 
     def run(cfg, lengths, repeat=1):
@@ -256,6 +260,8 @@ def test_batch_grouped_bit_identical_to_solo(gpu):
     run(dict(lanes=16, chunk=4, single_graph="grouped"), [6] * M)
     cfg = b.tune(trial_steps=4)  # scratch copies: the states do not move
     assert cfg["us_per_traj_step"] > 0
+    kind_of, us = b.kinds()  # the tuner may have moved members to other kernel variants, hence other kinds
+    assert len(kind_of) == M and len(us) == max(kind_of) + 1 and all(v > 0 for v in us)
     b.configure(lanes=4, chunk=1, single_graph="grouped-whole")
     run(None, [3, 1, 2, 2, 1, 3, 1, 2, 2, 1, 1])
     torch.cuda.synchronize()
