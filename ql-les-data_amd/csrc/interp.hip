@@ -164,7 +164,7 @@ struct GatherArgs {
 
 template <typename T>
 __global__ void k_build_coef(Pack<CoefArgs<T>> pk, int dim, int n, int mode, int nf) {
-  const CoefArgs<T>& a = pk.m[blockIdx.z];
+  const CoefArgs<T> a = pk.m[blockIdx.z];
   build_coef_run<T>(a.uh, a.Ns, a.ld, a.qplane, dim, n, a.scale, a.corr, a.out, mode, nf);
 }
 
@@ -239,7 +239,7 @@ __device__ __forceinline__ void gather_run(const cx<T>* __restrict__ f, int nf, 
 template <typename T, int DIM>
 __global__ void __launch_bounds__(256)
 k_gather(Pack<GatherArgs<T>> pk, int nf, int w, T beta) {
-  const GatherArgs<T>& a = pk.m[blockIdx.z];
+  const GatherArgs<T> a = pk.m[blockIdx.z];
   gather_run<T, DIM>(reinterpret_cast<const cx<T>*>(a.f), nf, w, beta, a.r, a.np, a.ur, a.dt, a.r_out);
 }
 
@@ -348,7 +348,7 @@ __device__ __forceinline__ void gather3p_run(const T* __restrict__ f4, int nf, T
 template <typename T, int W>
 __global__ void __launch_bounds__(256)
 k_gather3p(Pack<GatherArgs<T>> pk, int nf, T beta) {
-  const GatherArgs<T>& a = pk.m[blockIdx.z];
+  const GatherArgs<T> a = pk.m[blockIdx.z];
   gather3p_run<T, W>(a.f, nf, beta, a.r, a.np, a.ur, a.dt, a.r_out);
 }
 

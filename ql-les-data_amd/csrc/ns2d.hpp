@@ -198,7 +198,8 @@ __device__ __forceinline__ void ns2_x_run(const NS2X<T>& a) {
 template <typename T, int N, int V = 0>
 __global__ void __launch_bounds__((XCfg<N, V>::THREADS))
 k_ns2_x(Pack<NS2X<T>> pk) {
-  ns2_x_run<T, N, V>(pk.m[blockIdx.z]);
+  const NS2X<T> a = pk.m[blockIdx.z];  // fields loaded once, at the top
+  ns2_x_run<T, N, V>(a);
 }
 
 template <typename T>
@@ -310,7 +311,8 @@ __device__ __forceinline__ void ns2_y_run(const NS2Y<T>& a) {
 template <typename T, int N, int V = 0>
 __global__ void __launch_bounds__((ZCfg<N, V>::THREADS))
 k_ns2_y(Pack<NS2Y<T>> pk) {
-  ns2_y_run<T, N, V>(pk.m[blockIdx.z]);
+  const NS2Y<T> a = pk.m[blockIdx.z];
+  ns2_y_run<T, N, V>(a);
 }
 
 }  // namespace qk

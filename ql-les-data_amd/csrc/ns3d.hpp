@@ -44,6 +44,10 @@
 #ifndef QL_ZPAD
 #define QL_ZPAD 3
 #endif
+// the same for the FP64 3-D z-pass (16-byte elements, so another bank pattern)
+#ifndef QL_ZPAD64
+#define QL_ZPAD64 3
+#endif
 #ifndef QL_ZDEPTH32
 #define QL_ZDEPTH32 2
 #endif
@@ -58,6 +62,13 @@
 #endif
 #ifndef QL_ZDEPTH64
 #define QL_ZDEPTH64 3
+#endif
+// FP64 z-pass: the c2r pre-processing reads each pair (X_n, X_{M-n}) from the
+// rows in HBM straight into the inverse FFT's first-pass layout, instead of
+// parking the rows in LDS and reading the pairs from there (LDS-bound in
+// FP64: that round trip is a third of its LDS traffic)
+#ifndef QL_ZDIRECT64
+#define QL_ZDIRECT64 1
 #endif
 // FP64 z-pass at N = 256: 2 waves per SIMD requested explicitly, which lets
 // it keep its registers (204-206 VGPRs) instead of spilling to private memory
@@ -114,7 +125,8 @@ namespace qk {
 // kernels, so a member's results are bit-identical to its solo run (two
 // kernels compiled from one body need not round alike: floating-point
 // contraction depends on the surrounding code).  K keeps the kernel
-// arguments near 1.3 KB.
+// arguments near 1.3 KB.  Kernels copy their entry into a local at entry, so
+// its fields load together, once.
 template <class A>
 struct Pack {
   static constexpr int K0 = 1280 / (int)sizeof(A);
@@ -329,10 +341,12 @@ template <int SG, bool VIF> struct AccOut {
 template <typename T, int N, int SG, bool VIF, int V = 0>
 __global__ void __launch_bounds__((XCfg<N, V>::THREADS)) QL_ATTR_X
 k_ns3_x(Pack<NS3X<T>> pk) {
-  const NS3X<T>& a = pk.m[blockIdx.z];
-  // the body is included (not called) so that it compiles as the kernel's own
-  // code, reading its arguments where they lie: as a called function taking
-  // `a`, it compiled to a schedule that spilled 60-80 more registers per lane
+  // the entry is copied once, so its fields load together at the top (read
+  // in place at a run-time index, they were reloaded near each use: 109
+  // scalar loads instead of 19 in the 256^3 stage-3 kernel); the body is
+  // included, not called, so that it compiles as the kernel's own code (as a
+  // called function it spilled 60-80 more registers per lane)
+  const NS3X<T> a = pk.m[blockIdx.z];
 #include "ns3x_body.inc"
 }
 
@@ -445,7 +459,8 @@ template <typename T, int N, int V = 0>
 __global__ void __launch_bounds__((XCfg<N, V>::THREADS)) QL_ATTR_YI
 k_ns3_yinv(Pack<NS3Y<T>> pk) {
   QL_YI_LDS;
-  ns3_yinv_run<T, N, V>(pk.m[blockIdx.z], s_tw, s_buf);
+  const NS3Y<T> a = pk.m[blockIdx.z];
+  ns3_yinv_run<T, N, V>(a, s_tw, s_buf);
 }
 
 // ---------------------------------------------------------------------------
@@ -488,7 +503,7 @@ template <int M> struct ZPlanV<M, 1> { using R = typename Plan<M>::R; };
 template <int M, int V, int SEL> struct ZPlanS { using R = typename ZPlanV<M, V>::R; };
 template <> struct ZPlanS<128, 0, 1> { using R = Radices<8, 4, 4>; };
 
-template <int N, int V = 0, int SEL = 0>
+template <int N, int V = 0, int SEL = 0, int PADV = QL_ZPAD>
 struct ZCfg {
   static constexpr int M = N / 2;
   using R = typename ZPlanS<M, V, SEL>::R;
@@ -496,7 +511,7 @@ struct ZCfg {
   static constexpr int P0 = QL_ZTHREADS / TP;
   static constexpr int P = (N <= 64) ? (P0 > 8 ? 8 : P0) : (P0 > 16 ? 16 : P0);
   static constexpr int THREADS = P * TP;
-  static constexpr int ZPAD = QL_ZPAD;  // one LDS padding slot per 2^ZPAD elements of a pencil
+  static constexpr int ZPAD = PADV;  // one LDS padding slot per 2^ZPAD elements of a pencil
   static constexpr int LSR = M + (M >> ZPAD) + 1 + LPAD;
 };
 
@@ -570,7 +585,8 @@ __device__ __forceinline__ void r2c_post_store(cx<T>* dst, const cx<T>* pb, cons
 // every index is a compile-time constant (no private-memory arrays)
 // plan selector of the 3-D z-pass (SEL of ZCfg): 1 in FP32
 template <typename T> struct ZSel { static constexpr int value = sizeof(T) == 4 ? 1 : 0; };
-template <typename T, int N, int V> using Z3Cfg = ZCfg<N, V, ZSel<T>::value>;
+template <typename T, int N, int V>
+using Z3Cfg = ZCfg<N, V, ZSel<T>::value, sizeof(T) == 8 ? QL_ZPAD64 : QL_ZPAD>;
 
 template <typename T, int N, int V = 0>
 struct ZState {
@@ -598,6 +614,7 @@ struct ZState {
   // at 357.5, 354.1 and 346.7 us per launch: FP64 waits on memory more), one
   // at the smaller grids, where more would spill
   static constexpr int DEPTH = sizeof(T) == 4 ? QL_ZDEPTH32 : (N >= 256 ? QL_ZDEPTH64 : 1);
+  static constexpr bool DIRECT = sizeof(T) == 8 && QL_ZDIRECT64 != 0;
   static constexpr bool NTL = QL_NTL && NTBig<T, N>::value;
   const NS3Z<T>& a;
   cx<T>* s_tw;   // flat N-point table (c2r / r2c pre- and post-processing)
@@ -651,17 +668,46 @@ struct ZState {
   __device__ __forceinline__ cx<T>* slot() {
     return pf[F % DEPTH];
   }
-  // c2r of field F from its parked rows; field F+DEPTH is requested meanwhile
+  // c2r pre-processing of field F read from HBM: lane j forms its first-pass
+  // inputs Z'[n] from X_n and X_{M-n} of its pencil's row (X_M from the
+  // Nyquist plane), the same arithmetic as c2r_prep on parked rows
+  template <int F>
+  __device__ __forceinline__ void prep_direct(cx<T>* v) {
+    const cx<T>* src = a.in + F * fs + (size_t)row * M;
+    const cx<T> nyq = a.in[a.q + (size_t)F * N * N + row];
+#pragma unroll
+    for (int s = 0; s < CI::NIN; ++s) {
+      cx<T> z = {0, 0};
+      if (CI::in_valid(j, s)) {
+        const int n = CI::in_index(j, s);
+        cx<T> xn = src[n], xr = n == 0 ? nyq : src[M - n];
+        if (n == 0) {
+          xn.im = 0;
+          xr.im = 0;
+        }
+        xr = cconj(xr);
+        z = cadd(cadd(xn, xr), cmuli(cmulc(csub(xn, xr), s_tw[n])));
+      }
+      v[s] = z;
+    }
+  }
+  // c2r of field F from its parked rows (field F+DEPTH is requested
+  // meanwhile), or with DIRECT from its rows in HBM
   template <int F>
   __device__ __forceinline__ void field() {
     cx<T>* pb = s_buf + p * C::LSR;
-    lds_sync<SYNC_WAVE>();  // previous field's FFT scratch is free
-    park(slot<F>());
-    if constexpr (F + DEPTH < 6) fetch(F + DEPTH, slot<F>(), row0);
-    lds_sync<SYNC_WAVE>();
     cx<T> v[CI::MAXE];
-    c2r_prep<T, M, CI, ZP, TWR>(v, pb, s_tw, j);
-    lds_sync<SYNC_WAVE>();
+    if constexpr (DIRECT) {
+      prep_direct<F>(v);
+      lds_sync<SYNC_WAVE>();  // previous field's FFT scratch is free
+    } else {
+      lds_sync<SYNC_WAVE>();  // previous field's FFT scratch is free
+      park(slot<F>());
+      if constexpr (F + DEPTH < 6) fetch(F + DEPTH, slot<F>(), row0);
+      lds_sync<SYNC_WAVE>();
+      c2r_prep<T, M, CI, ZP, TWR>(v, pb, s_tw, j);
+      lds_sync<SYNC_WAVE>();
+    }
     CI::run(v, pb, 1, j, s_twi, 0);
     // (re, im) hold the (even, odd) z samples, still without the inverse
     // transform's 1/N^3, which the r2c post-processing applies to u x omega
@@ -732,9 +778,11 @@ __device__ __forceinline__ void ns3_z_body(const NS3Z<T>& a, cx<T>* s_tw, cx<T>*
       for (int s = 0; s < NO; ++s) z.uu[f][s] = src[CI::out_index(z.j, s)];
     }
   }
-  z.fetch(f0, z.template slot<f0>(), z.row0);
-  if constexpr (St::DEPTH >= 2) z.fetch(f0 + 1, z.template slot<f0 + 1>(), z.row0);
-  if constexpr (St::DEPTH >= 3) z.fetch(f0 + 2, z.template slot<f0 + 2>(), z.row0);
+  if constexpr (!St::DIRECT) {
+    z.fetch(f0, z.template slot<f0>(), z.row0);
+    if constexpr (St::DEPTH >= 2) z.fetch(f0 + 1, z.template slot<f0 + 1>(), z.row0);
+    if constexpr (St::DEPTH >= 3) z.fetch(f0 + 2, z.template slot<f0 + 2>(), z.row0);
+  }
   z.template fields<f0>();
   if (a.u_out != nullptr) {
     const T su = UIN ? (T)1 : St::inv_n3();
@@ -785,7 +833,8 @@ template <typename T, int N, int V = 0>
 __global__ void __launch_bounds__((Z3Cfg<T, N, V>::THREADS)) QL_ATTR_Z
 k_ns3_z(Pack<NS3Z<T>> pk) {
   QL_Z_LDS;
-  ns3_z_run<T, N, V>(pk.m[blockIdx.z], s_tw, s_twi, s_twf, s_buf);
+  const NS3Z<T> a = pk.m[blockIdx.z];
+  ns3_z_run<T, N, V>(a, s_tw, s_twi, s_twf, s_buf);
 }
 
 // ---------------------------------------------------------------------------
@@ -847,7 +896,8 @@ template <typename T, int N, int V = 0>
 __global__ void __launch_bounds__((XCfg<N, V>::THREADS)) QL_ATTR_YF
 k_ns3_yfwd(Pack<NS3YF<T>> pk) {
   QL_YF_LDS;
-  ns3_yfwd_run<T, N, V>(pk.m[blockIdx.z], s_tw, s_buf);
+  const NS3YF<T> a = pk.m[blockIdx.z];
+  ns3_yfwd_run<T, N, V>(a, s_tw, s_buf);
 }
 
 }  // namespace qk
