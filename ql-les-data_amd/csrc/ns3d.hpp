@@ -63,13 +63,6 @@
 #ifndef QL_ZDEPTH64
 #define QL_ZDEPTH64 3
 #endif
-// FP64 z-pass: the c2r pre-processing reads each pair (X_n, X_{M-n}) from the
-// rows in HBM straight into the inverse FFT's first-pass layout, instead of
-// parking the rows in LDS and reading the pairs from there (LDS-bound in
-// FP64: that round trip is a third of its LDS traffic)
-#ifndef QL_ZDIRECT64
-#define QL_ZDIRECT64 1
-#endif
 // FP64 z-pass at N = 256: 2 waves per SIMD requested explicitly, which lets
 // it keep its registers (204-206 VGPRs) instead of spilling to private memory
 #ifndef QL_Z64_WPE
@@ -614,7 +607,6 @@ struct ZState {
   // at 357.5, 354.1 and 346.7 us per launch: FP64 waits on memory more), one
   // at the smaller grids, where more would spill
   static constexpr int DEPTH = sizeof(T) == 4 ? QL_ZDEPTH32 : (N >= 256 ? QL_ZDEPTH64 : 1);
-  static constexpr bool DIRECT = sizeof(T) == 8 && QL_ZDIRECT64 != 0;
   static constexpr bool NTL = QL_NTL && NTBig<T, N>::value;
   const NS3Z<T>& a;
   cx<T>* s_tw;   // flat N-point table (c2r / r2c pre- and post-processing)
@@ -668,46 +660,17 @@ struct ZState {
   __device__ __forceinline__ cx<T>* slot() {
     return pf[F % DEPTH];
   }
-  // c2r pre-processing of field F read from HBM: lane j forms its first-pass
-  // inputs Z'[n] from X_n and X_{M-n} of its pencil's row (X_M from the
-  // Nyquist plane), the same arithmetic as c2r_prep on parked rows
-  template <int F>
-  __device__ __forceinline__ void prep_direct(cx<T>* v) {
-    const cx<T>* src = a.in + F * fs + (size_t)row * M;
-    const cx<T> nyq = a.in[a.q + (size_t)F * N * N + row];
-#pragma unroll
-    for (int s = 0; s < CI::NIN; ++s) {
-      cx<T> z = {0, 0};
-      if (CI::in_valid(j, s)) {
-        const int n = CI::in_index(j, s);
-        cx<T> xn = src[n], xr = n == 0 ? nyq : src[M - n];
-        if (n == 0) {
-          xn.im = 0;
-          xr.im = 0;
-        }
-        xr = cconj(xr);
-        z = cadd(cadd(xn, xr), cmuli(cmulc(csub(xn, xr), s_tw[n])));
-      }
-      v[s] = z;
-    }
-  }
-  // c2r of field F from its parked rows (field F+DEPTH is requested
-  // meanwhile), or with DIRECT from its rows in HBM
+  // c2r of field F from its parked rows; field F+DEPTH is requested meanwhile
   template <int F>
   __device__ __forceinline__ void field() {
     cx<T>* pb = s_buf + p * C::LSR;
+    lds_sync<SYNC_WAVE>();  // previous field's FFT scratch is free
+    park(slot<F>());
+    if constexpr (F + DEPTH < 6) fetch(F + DEPTH, slot<F>(), row0);
+    lds_sync<SYNC_WAVE>();
     cx<T> v[CI::MAXE];
-    if constexpr (DIRECT) {
-      prep_direct<F>(v);
-      lds_sync<SYNC_WAVE>();  // previous field's FFT scratch is free
-    } else {
-      lds_sync<SYNC_WAVE>();  // previous field's FFT scratch is free
-      park(slot<F>());
-      if constexpr (F + DEPTH < 6) fetch(F + DEPTH, slot<F>(), row0);
-      lds_sync<SYNC_WAVE>();
-      c2r_prep<T, M, CI, ZP, TWR>(v, pb, s_tw, j);
-      lds_sync<SYNC_WAVE>();
-    }
+    c2r_prep<T, M, CI, ZP, TWR>(v, pb, s_tw, j);
+    lds_sync<SYNC_WAVE>();
     CI::run(v, pb, 1, j, s_twi, 0);
     // (re, im) hold the (even, odd) z samples, still without the inverse
     // transform's 1/N^3, which the r2c post-processing applies to u x omega
@@ -778,11 +741,9 @@ __device__ __forceinline__ void ns3_z_body(const NS3Z<T>& a, cx<T>* s_tw, cx<T>*
       for (int s = 0; s < NO; ++s) z.uu[f][s] = src[CI::out_index(z.j, s)];
     }
   }
-  if constexpr (!St::DIRECT) {
-    z.fetch(f0, z.template slot<f0>(), z.row0);
-    if constexpr (St::DEPTH >= 2) z.fetch(f0 + 1, z.template slot<f0 + 1>(), z.row0);
-    if constexpr (St::DEPTH >= 3) z.fetch(f0 + 2, z.template slot<f0 + 2>(), z.row0);
-  }
+  z.fetch(f0, z.template slot<f0>(), z.row0);
+  if constexpr (St::DEPTH >= 2) z.fetch(f0 + 1, z.template slot<f0 + 1>(), z.row0);
+  if constexpr (St::DEPTH >= 3) z.fetch(f0 + 2, z.template slot<f0 + 2>(), z.row0);
   z.template fields<f0>();
   if (a.u_out != nullptr) {
     const T su = UIN ? (T)1 : St::inv_n3();
