@@ -1,6 +1,6 @@
 """The batched configuration at its size: 24 trajectories of uneven lengths on one GPU, 2-D Kolmogorov
 at 64^2, 128^2, 256^2 and 512^2 (32^2 NUFFT particles each) and 3-D power-forced HIT at 64^3, 96^3 and
-128^3 (16^3 particles each), advanced by qlles_batch after its autotuner picked the launch shape.
+128^3 (16^3 particles each), advanced by qlles_batch in launch modes picked by its autotuner or by hand.
 
 No reference counterpart (the reference runs one trajectory per process, slurm_*.sh:5).  Pins:
   - every FP32 member is bit-identical to the same steps taken alone (qlles_step_advect);
@@ -108,6 +108,55 @@ def test_batch24_bit_identical_to_solo_and_energy_vs_oracle(gpu):
     print("[batch24] every member bit-identical to its solo run", flush=True)
     worst = 0.0
     total = [a + b_ + c for a, b_, c in zip(lengths, *more)]
+    for k in range(len(SIZES)):
+        i = min(range(k, M, len(SIZES)), key=lambda j: total[j])  # shortest member of this size
+        _, uho, _ = oracle_run(i, total[i])
+        dim, N = SIZES[k]
+        e_o = O.kinetic_energy_hat(uho, N, dim)
+        e_g = kinetic_energy_hat(mem[i][1])[0]
+        err = abs(e_g - e_o) / abs(e_o)
+        worst = max(worst, err)
+        print(f"[batch24] member {i} ({dim}-D {N}, {total[i]} steps): FP32 energy vs FP64 oracle {err:.2e}",
+              flush=True)
+    assert worst < 1e-5, worst
+
+
+def test_batch24_configured_modes_bit_identical_to_solo_and_energy_vs_oracle(gpu):
+    """The 24 members without the tuner: per-member graphs, then both grouped modes, configured by hand, with
+    uneven lengths; every member bitwise equal to its solo runs, FP32 energies within 1e-5 of the FP64 oracle."""
+    from l3es_amd import Batch
+    from l3es_amd.utils import kinetic_energy_hat
+
+    M = 24
+    mem = [make(i, "float32") for i in range(M)]
+    print("[batch24] 24 FP32 members set up", flush=True)
+    solo = [(uh.clone(), r.clone()) for (_, uh, _, r) in mem]
+    b = Batch()
+    urs = []
+    for step, uh, it, r in mem:
+        ur = torch.empty((r.shape[0], it.dim), dtype=it.dtype, device=r.device)
+        urs.append(ur)
+        b.add(step, uh, it, r, ur)
+    runs = [
+        (dict(lanes=4, chunk=2, single_graph=True), [4 + 2 * (i % 4) for i in range(M)]),
+        (dict(lanes=4, chunk=2, single_graph="grouped"), [3 + (i % 3) for i in range(M)]),
+        (dict(lanes=6, chunk=1, single_graph="grouped-whole"), [2 + ((i + 1) % 4) for i in range(M)]),
+    ]
+    for cfg_r, lengths in runs:
+        b.configure(**cfg_r)
+        b.run(lengths)
+        print(f"[batch24] ran {cfg_r}", flush=True)
+    torch.cuda.synchronize()
+    for i, ((step, uh_b, it, r_b), (uh_s, r_s)) in enumerate(zip(mem, solo)):
+        for _, lengths in runs:
+            ur_s = _advect_solo(step, it, uh_s, r_s, lengths[i])
+        torch.cuda.synchronize()
+        assert np.array_equal(host(uh_s), host(uh_b)), f"member {i}: spectrum differs from its solo run"
+        assert np.array_equal(host(r_s), host(r_b)), f"member {i}: positions differ"
+        assert np.array_equal(host(ur_s), host(urs[i])), f"member {i}: particle velocities differ"
+    print("[batch24] every member bit-identical to its solo run", flush=True)
+    worst = 0.0
+    total = [sum(lengths[i] for _, lengths in runs) for i in range(M)]
     for k in range(len(SIZES)):
         i = min(range(k, M, len(SIZES)), key=lambda j: total[j])  # shortest member of this size
         _, uho, _ = oracle_run(i, total[i])
