@@ -207,6 +207,15 @@ template <> struct XPlan<128> { using R = Radices<4, 4, 8>; static constexpr int
 #ifndef QL_NTL
 #define QL_NTL 1
 #endif
+// per stream: NY in the x-pass, NZ in the y-forward (each pairs with its
+// producer's QL_NT_YF / QL_NT_Z store: a field stored and read with cached
+// accesses can stay in the last-level cache between the two kernels)
+#ifndef QL_NTL_NY
+#define QL_NTL_NY 1
+#endif
+#ifndef QL_NTL_NZ
+#define QL_NTL_NZ 1
+#endif
 // smallest field component (N^3 reals, MiB) that takes the streaming stores and
 // loads; below it the last-level cache holds enough of the data between
 // kernels (with no gate: 24 batched trajectories of 64^3-128^3 9440 -> 8840
@@ -816,7 +825,7 @@ struct NS3YF {
 template <typename T, int N, int V = 0>
 __device__ __forceinline__ void ns3_yfwd_run(const NS3YF<T>& a, cx<T>* s_tw, cx<T>* s_buf) {
   using C = XCfg<N, V>;
-  constexpr bool NTL = QL_NTL && NTBig<T, N>::value;
+  constexpr bool NTL = QL_NTL && QL_NTL_NZ && NTBig<T, N>::value;
   using F = XFwd<T, N>;
   const int tid = threadIdx.x, p = tid % C::TIX, j = tid / C::TIX;
   const int kz = blockIdx.x * C::TIX + p;
