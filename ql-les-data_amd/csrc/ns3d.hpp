@@ -53,20 +53,24 @@
 #endif
 // z-pass twiddles formed from one table read (bit 0: FFT passes, bit 1: c2r /
 // r2c pre/post).  FP32 256^3 step: 455 -> 471 steps/s with both (206 -> 168
-// VGPRs); FP64 212.4 -> 211.1-211.6 with either, so off there.
+// VGPRs).  FP64: the FFT passes' half, with one field in flight and 3 waves
+// per SIMD (below)
 #ifndef QL_ZTWR32
 #define QL_ZTWR32 3
 #endif
 #ifndef QL_ZTWR64
-#define QL_ZTWR64 0
+#define QL_ZTWR64 1
 #endif
+// FP64 z-pass at N >= 256: one field in flight and 3 waves per SIMD (168
+// VGPRs, 60 B per lane spilled) with the FFT passes' twiddles formed: 349.7
+// -> 294.7 us per launch at 256^3 (210.8 -> 221.1 steps/s).  Three fields in
+// flight at 2 waves per SIMD (206 VGPRs, no spill) was the previous setting;
+// two fields at 3 waves spill 84 B and take 323 us
 #ifndef QL_ZDEPTH64
-#define QL_ZDEPTH64 3
+#define QL_ZDEPTH64 1
 #endif
-// FP64 z-pass at N = 256: 2 waves per SIMD requested explicitly, which lets
-// it keep its registers (204-206 VGPRs) instead of spilling to private memory
 #ifndef QL_Z64_WPE
-#define QL_Z64_WPE 2
+#define QL_Z64_WPE 3
 #endif
 #ifndef QL_WPE_YI
 #define QL_WPE_YI 0
@@ -76,6 +80,12 @@
 #endif
 #ifndef QL_WPE_YF
 #define QL_WPE_YF 0
+#endif
+// FP64 y-forward at N >= 256: 4 waves per SIMD (128 VGPRs, 40 B per lane
+// spilled) against 3 (140 VGPRs): 99.3 -> 94.0 us per launch at 256^3.  The
+// y-inverse at 3 waves (168 VGPRs, 148 B spilled) took 490 against 304 us
+#ifndef QL_YF64_WPE
+#define QL_YF64_WPE 4
 #endif
 #define QL_WPE_ATTR_(n) __attribute__((amdgpu_waves_per_eu(n)))
 // the x-pass target applies to FP32 at N >= 256, where it fits without
@@ -105,6 +115,12 @@ template <typename T, int N> struct ZWpe {
 #endif
 #if QL_WPE_YF > 0
 #define QL_ATTR_YF QL_WPE_ATTR_(QL_WPE_YF)
+#elif QL_YF64_WPE > 0
+template <typename T, int N> struct YfWpe {
+  static constexpr bool on = sizeof(T) == 8 && N >= 256;
+  static constexpr int lo = on ? QL_YF64_WPE : 1, hi = on ? QL_YF64_WPE : 8;
+};
+#define QL_ATTR_YF __attribute__((amdgpu_waves_per_eu(YfWpe<T, N>::lo, YfWpe<T, N>::hi)))
 #else
 #define QL_ATTR_YF
 #endif
@@ -612,9 +628,8 @@ struct ZState {
   // lanes 0..PWV-1 from the contiguous Nyquist plane)
   static constexpr int NEM = (PWV * M + WL - 1) / WL;
   static constexpr int NE = NEM + 1;
-  // fields in flight: two in FP32; three in FP64 at N = 256 (1/2/3 measured
-  // at 357.5, 354.1 and 346.7 us per launch: FP64 waits on memory more), one
-  // at the smaller grids, where more would spill
+  // fields in flight: two in FP32; QL_ZDEPTH64 in FP64 at N >= 256, one at
+  // the smaller grids, where more would spill
   static constexpr int DEPTH = sizeof(T) == 4 ? QL_ZDEPTH32 : (N >= 256 ? QL_ZDEPTH64 : 1);
   static constexpr bool NTL = QL_NTL && NTBig<T, N>::value;
   const NS3Z<T>& a;
