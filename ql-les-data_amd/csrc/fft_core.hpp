@@ -274,9 +274,20 @@ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 // LDS element address of pencil element n: n*ls, plus one padding slot every
 // 2^PAD elements when PAD > 0 (breaks the stride-R bank pattern of the
 // first Stockham pass when a pencil is contiguous in LDS).
+//
+// PAD == LDS_PAIR16: pencils of 16 lanes stored in pairs, element n of a
+// pencil in row n / 16 (32 elements per row: 16 of each pencil of the pair,
+// the pencil base pointer selecting its half) at column n % 16.  On gfx950 an
+// 8-byte LDS read serves 32 lanes (two such pencils) per cycle over 64 banks
+// and an 8-byte write 16 lanes over 32 banks, so any 16 consecutive elements
+// of the two pencils land on distinct banks.  Inside an FFT the pair layout is
+// swizzled per exchange (PFFT::xat), which keeps the strided Stockham writes
+// on distinct banks as well.
+constexpr int LDS_PAIR16 = -1;
 template <int PAD>
 __device__ __forceinline__ int lds_at(int n) {
-  if constexpr (PAD > 0) return n + (n >> PAD);
+  if constexpr (PAD == LDS_PAIR16) return (n >> 4) * 32 + (n & 15);
+  else if constexpr (PAD > 0) return n + (n >> PAD);
   else return n;
 }
 
@@ -399,6 +410,29 @@ struct PFFT {
     }
   }
 
+  // LDS position of element n in the exchange read by pass e.  With
+  // LDS_PAIR16 the column is XORed with (row & mask) << shift, chosen so that
+  // for every (t, r) the 16 lanes of a pencil write distinct columns: the
+  // inverse radix-(4, 4, 8) plan's writes after pass 0 (4b + r) and pass 1
+  // ((b / 4) 16 + b % 4 + 4r) and the forward radix-(8, 4, 4) plan's after
+  // pass 0 (8b + r) and pass 1 ((b / 8) 32 + b % 8 + 8r); the reads
+  // (b + B r) are consecutive runs of 16 and stay conflict-free under any
+  // row swizzle.
+  static constexpr int xmask(int e) { return FWD ? (e == 1 ? 7 : 2) : 3; }
+  static constexpr int xshift(int e) { return FWD ? (e == 1 ? 0 : 2) : (e == 1 ? 0 : 2); }
+  template <int e>
+  __device__ __forceinline__ static int xat(int n) {
+    if constexpr (PAD == LDS_PAIR16) {
+      static_assert(TP == 16 && M == 128 && NP == 3 &&
+                        (FWD ? (R(0) == 8 && R(1) == 4 && R(2) == 4) : (R(0) == 4 && R(1) == 4 && R(2) == 8)),
+                    "the paired LDS layout's swizzles are derived for the 16-lane radix-(8, 4, 4) plan");
+      const int row = n >> 4;
+      return row * 32 + ((n & 15) ^ ((row & xmask(e)) << xshift(e)));
+    } else {
+      return lds_at<PAD>(n);
+    }
+  }
+
   template <int p>
   __device__ __forceinline__ static void write_lds(const cx<T>* v, cx<T>* lds, int ls, int j) {
     constexpr int Rp = R(p);
@@ -409,7 +443,7 @@ struct PFFT {
       if ((B(p) % TP == 0) || b < B(p)) {
         const int base = (b / Ns) * Ns * Rp + (b % Ns);
 #pragma unroll
-        for (int r = 0; r < Rp; ++r) lds[lds_at<PAD>(base + Ns * r) * ls] = v[t * Rp + r];
+        for (int r = 0; r < Rp; ++r) lds[xat<p + 1>(base + Ns * r) * ls] = v[t * Rp + r];
       }
     }
   }
@@ -422,7 +456,7 @@ struct PFFT {
       const int b = j + TP * t;
       if ((B(p) % TP == 0) || b < B(p)) {
 #pragma unroll
-        for (int r = 0; r < Rp; ++r) v[t * Rp + r] = lds[lds_at<PAD>(b + B(p) * r) * ls];
+        for (int r = 0; r < Rp; ++r) v[t * Rp + r] = lds[xat<p>(b + B(p) * r) * ls];
       }
     }
   }

@@ -529,8 +529,15 @@ struct ZCfg {
   static constexpr int P0 = QL_ZTHREADS / TP;
   static constexpr int P = (N <= 64) ? (P0 > 8 ? 8 : P0) : (P0 > 16 ? 16 : P0);
   static constexpr int THREADS = P * TP;
-  static constexpr int ZPAD = PADV;  // one LDS padding slot per 2^ZPAD elements of a pencil
-  static constexpr int LSR = M + (M >> ZPAD) + 1 + LPAD;
+  static constexpr int ZPAD = PADV;  // one LDS padding slot per 2^ZPAD elements of a pencil, or LDS_PAIR16
+  static constexpr bool PAIRED = ZPAD == LDS_PAIR16;
+  static constexpr int LSR = M + (PAIRED ? 0 : (M >> ZPAD)) + 1 + LPAD;
+  // paired layout: rows of 32 elements per pencil pair, element M in the last
+  static constexpr int PAIR = (M / 16 + 1) * 32;
+  static_assert(!PAIRED || (TP == 16 && P % 2 == 0), "paired LDS layout: pencils of 16 lanes, in pairs");
+  static constexpr int BUF = PAIRED ? (P / 2) * PAIR : P * LSR;  // LDS elements of the block's pencils
+  // first element of pencil p of the block
+  static __device__ __forceinline__ int pencil_off(int p) { return PAIRED ? (p >> 1) * PAIR + (p & 1) * 16 : p * LSR; }
 };
 
 // W_N^{B0 r} a with N = 2M, r < R: a compile-time constant for each r (r is a
@@ -603,8 +610,19 @@ __device__ __forceinline__ void r2c_post_store(cx<T>* dst, const cx<T>* pb, cons
 // every index is a compile-time constant (no private-memory arrays)
 // plan selector of the 3-D z-pass (SEL of ZCfg): 1 in FP32
 template <typename T> struct ZSel { static constexpr int value = sizeof(T) == 4 ? 1 : 0; };
+// FP32 at N = 256 (16-lane radix-(8, 4, 4) plan): the paired, per-exchange
+// swizzled LDS layout (fft_core.hpp LDS_PAIR16), no bank conflicts in the
+// pencil exchanges
+#ifndef QL_ZPAIR32
+#define QL_ZPAIR32 1
+#endif
 template <typename T, int N, int V>
-using Z3Cfg = ZCfg<N, V, ZSel<T>::value, sizeof(T) == 8 ? QL_ZPAD64 : QL_ZPAD>;
+struct ZPadSel {
+  static constexpr int value = sizeof(T) == 8 ? QL_ZPAD64
+                                              : ((QL_ZPAIR32 && N == 256 && V == 0) ? LDS_PAIR16 : QL_ZPAD);
+};
+template <typename T, int N, int V>
+using Z3Cfg = ZCfg<N, V, ZSel<T>::value, ZPadSel<T, N, V>::value>;
 
 template <typename T, int N, int V = 0>
 struct ZState {
@@ -661,23 +679,36 @@ struct ZState {
   // request field f of the pencils whose block starts at row r0
   __device__ __forceinline__ void fetch(int f, cx<T>* dst, int r0) {
     const cx<T>* src = a.in + f * fs + (size_t)(r0 + wrow) * M;
+    if constexpr (C::PAIRED) {
+      // 16 consecutive entries of each pencil per load: lanes 16q .. 16q+15
+      // take entries 16e .. 16e+15 of pencil q (128-byte runs)
+      static_assert(NEM * TP == M, "paired fetch covers each row");
 #pragma unroll
-    for (int e = 0; e < NEM; ++e) {
-      const int idx = ln + e * WL;
-      if (idx < PWV * M) dst[e] = ld_stream<NTL>(&src[idx]);
+      for (int e = 0; e < NEM; ++e) dst[e] = ld_stream<NTL>(&src[(ln / TP) * M + TP * e + ln % TP]);
+    } else {
+#pragma unroll
+      for (int e = 0; e < NEM; ++e) {
+        const int idx = ln + e * WL;
+        if (idx < PWV * M) dst[e] = ld_stream<NTL>(&src[idx]);
+      }
     }
     if (ln < PWV) dst[NEM] = a.in[a.q + (size_t)f * N * N + r0 + wrow + ln];
   }
   __device__ __forceinline__ void park(const cx<T>* src) {
+    if constexpr (C::PAIRED) {
 #pragma unroll
-    for (int e = 0; e < NEM; ++e) {
-      const int idx = ln + e * WL;
-      if (idx < PWV * M) {
-        const int q = idx / M, k = idx - q * M;
-        s_buf[(wrow + q) * C::LSR + lds_at<ZP>(k)] = src[e];
+      for (int e = 0; e < NEM; ++e) s_buf[C::pencil_off(wrow + ln / TP) + lds_at<ZP>(TP * e + ln % TP)] = src[e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < NEM; ++e) {
+        const int idx = ln + e * WL;
+        if (idx < PWV * M) {
+          const int q = idx / M, k = idx - q * M;
+          s_buf[C::pencil_off(wrow + q) + lds_at<ZP>(k)] = src[e];
+        }
       }
     }
-    if (ln < PWV) s_buf[(wrow + ln) * C::LSR + lds_at<ZP>(M)] = src[NEM];
+    if (ln < PWV) s_buf[C::pencil_off(wrow + ln) + lds_at<ZP>(M)] = src[NEM];
   }
   static __device__ __forceinline__ T inv_n3() { return (T)(1.0 / ((double)N * N * N)); }
   template <int F>
@@ -687,7 +718,7 @@ struct ZState {
   // c2r of field F from its parked rows; field F+DEPTH is requested meanwhile
   template <int F>
   __device__ __forceinline__ void field() {
-    cx<T>* pb = s_buf + p * C::LSR;
+    cx<T>* pb = s_buf + C::pencil_off(p);
     lds_sync<SYNC_WAVE>();  // previous field's FFT scratch is free
     park(slot<F>());
     if constexpr (F + DEPTH < 6) fetch(F + DEPTH, slot<F>(), row0);
@@ -782,7 +813,7 @@ __device__ __forceinline__ void ns3_z_body(const NS3Z<T>& a, cx<T>* s_tw, cx<T>*
   // factor of u x omega (u given in physical space carries its own)
   const double n3 = (double)N * N * N;
   const T h = (T)(UIN ? 0.5 / n3 : 0.5 / (n3 * n3));
-  cx<T>* pb = s_buf + z.p * Z3Cfg<T, N, V>::LSR;
+  cx<T>* pb = s_buf + Z3Cfg<T, N, V>::pencil_off(z.p);
 #pragma unroll
   for (int cc = 0; cc < 3; ++cc) {
     cx<T> v[CF::MAXE];
@@ -804,7 +835,7 @@ __device__ __forceinline__ void ns3_z_body(const NS3Z<T>& a, cx<T>* s_tw, cx<T>*
   __shared__ cx<T> s_tw[N];                                                                 \
   __shared__ cx<T> s_twi[St::CI::TWL > 0 ? St::CI::TWL : 1];                                \
   __shared__ cx<T> s_twf[St::CF::TWL > 0 ? St::CF::TWL : 1];                                \
-  __shared__ cx<T> s_buf[Z3Cfg<T, N, V>::P * Z3Cfg<T, N, V>::LSR]
+  __shared__ cx<T> s_buf[Z3Cfg<T, N, V>::BUF]
 
 template <typename T, int N, int V = 0>
 __device__ __forceinline__ void ns3_z_run(const NS3Z<T>& a, cx<T>* s_tw, cx<T>* s_twi, cx<T>* s_twf,
