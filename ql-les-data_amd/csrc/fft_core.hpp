@@ -283,10 +283,18 @@ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 // of the two pencils land on distinct banks.  Inside an FFT the pair layout is
 // swizzled per exchange (PFFT::xat), which keeps the strided Stockham writes
 // on distinct banks as well.
+//
+// PAD == LDS_XOR32: pencils of 32 lanes with 16-byte elements, unpadded.  A
+// 16-byte read serves 16 lanes per cycle over 64 banks, the lanes {0-3, 12-15,
+// 20-27} and {4-11, 16-19, 28-31} of each half wave together, and a 16-byte
+// write 8 consecutive lanes over 32 banks, so consecutive runs need no
+// padding; the FFT exchanges XOR low index bits with higher ones (PFFT::xat).
 constexpr int LDS_PAIR16 = -1;
+constexpr int LDS_XOR32 = -2;
 template <int PAD>
 __device__ __forceinline__ int lds_at(int n) {
   if constexpr (PAD == LDS_PAIR16) return (n >> 4) * 32 + (n & 15);
+  else if constexpr (PAD == LDS_XOR32) return n;
   else if constexpr (PAD > 0) return n + (n >> PAD);
   else return n;
 }
@@ -428,6 +436,16 @@ struct PFFT {
                     "the paired LDS layout's swizzles are derived for the 16-lane radix-(8, 4, 4) plan");
       const int row = n >> 4;
       return row * 32 + ((n & 15) ^ ((row & xmask(e)) << xshift(e)));
+    } else if constexpr (PAD == LDS_XOR32) {
+      // inverse radix-(8, 4, 4) plan: writes 8b + r after pass 0, so XOR the
+      // low 3 bits with b (bits 3-5); forward radix-(4, 4, 8): writes 4b + r
+      // after pass 0 (XOR bits 0-1 with bits 3-4) and (b / 4) 16 + b % 4 + 4r
+      // after pass 1 (XOR bit 2 with bit 4)
+      static_assert(TP == 32 && M == 128 && NP == 3 &&
+                        (FWD ? (R(0) == 4 && R(1) == 4 && R(2) == 8) : (R(0) == 8 && R(1) == 4 && R(2) == 4)),
+                    "the XOR LDS layout's swizzles are derived for the 32-lane radix-(4, 4, 8) plan");
+      if constexpr (!FWD) return e == 1 ? (n ^ ((n >> 3) & 7)) : n;
+      else return e == 1 ? (n ^ ((n >> 3) & 3)) : (n ^ (((n >> 4) & 1) << 2));
     } else {
       return lds_at<PAD>(n);
     }

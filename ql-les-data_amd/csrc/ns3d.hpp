@@ -531,7 +531,7 @@ struct ZCfg {
   static constexpr int THREADS = P * TP;
   static constexpr int ZPAD = PADV;  // one LDS padding slot per 2^ZPAD elements of a pencil, or LDS_PAIR16
   static constexpr bool PAIRED = ZPAD == LDS_PAIR16;
-  static constexpr int LSR = M + (PAIRED ? 0 : (M >> ZPAD)) + 1 + LPAD;
+  static constexpr int LSR = M + (ZPAD > 0 ? (M >> ZPAD) : 0) + 1 + LPAD;
   // paired layout: rows of 32 elements per pencil pair, element M in the last
   static constexpr int PAIR = (M / 16 + 1) * 32;
   static_assert(!PAIRED || (TP == 16 && P % 2 == 0), "paired LDS layout: pencils of 16 lanes, in pairs");
@@ -616,9 +616,14 @@ template <typename T> struct ZSel { static constexpr int value = sizeof(T) == 4 
 #ifndef QL_ZPAIR32
 #define QL_ZPAIR32 1
 #endif
+// FP64 at N = 256 (32-lane radix-(4, 4, 8) plan): unpadded pencils with XOR
+// swizzles in the FFT exchanges (fft_core.hpp LDS_XOR32)
+#ifndef QL_ZXOR64
+#define QL_ZXOR64 1
+#endif
 template <typename T, int N, int V>
 struct ZPadSel {
-  static constexpr int value = sizeof(T) == 8 ? QL_ZPAD64
+  static constexpr int value = sizeof(T) == 8 ? ((QL_ZXOR64 && N == 256 && V == 0) ? LDS_XOR32 : QL_ZPAD64)
                                               : ((QL_ZPAIR32 && N == 256 && V == 0) ? LDS_PAIR16 : QL_ZPAD);
 };
 template <typename T, int N, int V>
