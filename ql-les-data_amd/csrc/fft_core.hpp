@@ -437,15 +437,24 @@ struct PFFT {
       const int row = n >> 4;
       return row * 32 + ((n & 15) ^ ((row & xmask(e)) << xshift(e)));
     } else if constexpr (PAD == LDS_XOR32) {
-      // inverse radix-(8, 4, 4) plan: writes 8b + r after pass 0, so XOR the
-      // low 3 bits with b (bits 3-5); forward radix-(4, 4, 8): writes 4b + r
-      // after pass 0 (XOR bits 0-1 with bits 3-4) and (b / 4) 16 + b % 4 + 4r
-      // after pass 1 (XOR bit 2 with bit 4)
-      static_assert(TP == 32 && M == 128 && NP == 3 &&
-                        (FWD ? (R(0) == 4 && R(1) == 4 && R(2) == 8) : (R(0) == 8 && R(1) == 4 && R(2) == 4)),
-                    "the XOR LDS layout's swizzles are derived for the 32-lane radix-(4, 4, 8) plan");
-      if constexpr (!FWD) return e == 1 ? (n ^ ((n >> 3) & 7)) : n;
-      else return e == 1 ? (n ^ ((n >> 3) & 3)) : (n ^ (((n >> 4) & 1) << 2));
+      // n ^ ((n >> a1) & m1) << s1 ^ ((n >> a2) & m2) << s2 per exchange,
+      // searched over with the 16-byte bank rule: every write (8 consecutive
+      // lanes) and read (16 lanes of a group) of the radix-(4, 4, 4, 2) plan
+      // and its inverse (2, 4, 4, 4) on distinct banks.  Each is a bijection
+      // on 0 .. 127.
+      static_assert(TP == 32 && M == 128 && NP == 4 &&
+                        (FWD ? (R(0) == 4 && R(1) == 4 && R(2) == 4 && R(3) == 2)
+                             : (R(0) == 2 && R(1) == 4 && R(2) == 4 && R(3) == 4)),
+                    "the XOR LDS layout's swizzles are derived for the 32-lane radix-(4, 4, 4, 2) plan");
+      if constexpr (!FWD) {
+        if constexpr (e == 1) return n ^ ((n >> 3) & 1);
+        else if constexpr (e == 2) return n ^ (((n >> 3) & 1) << 2) ^ ((n >> 3) & 3);
+        else return n;
+      } else {
+        if constexpr (e == 1) return n ^ ((n >> 3) & 3);
+        else if constexpr (e == 2) return n ^ (((n >> 3) & 3) << 1) ^ (((n >> 4) & 1) << 3);
+        else return n;
+      }
     } else {
       return lds_at<PAD>(n);
     }

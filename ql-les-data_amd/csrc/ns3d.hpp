@@ -61,11 +61,12 @@
 #ifndef QL_ZTWR64
 #define QL_ZTWR64 1
 #endif
-// FP64 z-pass at N >= 256: one field in flight and 3 waves per SIMD (168
-// VGPRs, 60 B per lane spilled) with the FFT passes' twiddles formed: 349.7
-// -> 294.7 us per launch at 256^3 (210.8 -> 221.1 steps/s).  Three fields in
-// flight at 2 waves per SIMD (206 VGPRs, no spill) was the previous setting;
-// two fields at 3 waves spill 84 B and take 323 us
+// FP64 z-pass at N >= 256: one field in flight and 3 waves per SIMD with the
+// FFT passes' twiddles formed (168 VGPRs; no spill with the radix-(4, 4, 4, 2)
+// plan at N = 256, below).  Three fields in flight at 2 waves per SIMD (206
+// VGPRs) took 349.7 us per launch at 256^3; this setting with the radix-(4,
+// 4, 8) plan 294.7 us (60 B per lane spilled), with the (4, 4, 4, 2) plan
+// 265-266 us (two fields in flight the same)
 #ifndef QL_ZDEPTH64
 #define QL_ZDEPTH64 1
 #endif
@@ -518,8 +519,13 @@ template <int M> struct ZPlanV<M, 1> { using R = typename Plan<M>::R; };
 // pass is bound by vector-ALU issue, so the busier lanes pay for the lower
 // occupancy (2 waves per SIMD at 206 registers against 4): 146.6 -> 137.8 us
 // per launch.  FP64 (256 registers, 1 wave per SIMD) keeps the 32-lane plan.
+// FP64 (SEL 2) at N = 256: radices 4, 4, 4, 2 on 32 lanes.  Its four-element
+// register tile (against eight for 4, 4, 8) keeps the 3-wave build free of
+// spills, whose scratch reloads had waited on the in-flight row prefetch:
+// 292.9 -> 265.3-266.2 us per launch.
 template <int M, int V, int SEL> struct ZPlanS { using R = typename ZPlanV<M, V>::R; };
 template <> struct ZPlanS<128, 0, 1> { using R = Radices<8, 4, 4>; };
+template <> struct ZPlanS<128, 0, 2> { using R = Radices<4, 4, 4, 2>; };
 
 template <int N, int V = 0, int SEL = 0, int PADV = QL_ZPAD>
 struct ZCfg {
@@ -608,15 +614,15 @@ __device__ __forceinline__ void r2c_post_store(cx<T>* dst, const cx<T>* pb, cons
 // the three products stay live.
 // state of one z-pass workgroup; the prefetch ring is two named buffers so
 // every index is a compile-time constant (no private-memory arrays)
-// plan selector of the 3-D z-pass (SEL of ZCfg): 1 in FP32
-template <typename T> struct ZSel { static constexpr int value = sizeof(T) == 4 ? 1 : 0; };
+// plan selector of the 3-D z-pass (SEL of ZCfg): 1 in FP32, 2 in FP64
+template <typename T> struct ZSel { static constexpr int value = sizeof(T) == 4 ? 1 : 2; };
 // FP32 at N = 256 (16-lane radix-(8, 4, 4) plan): the paired, per-exchange
 // swizzled LDS layout (fft_core.hpp LDS_PAIR16), no bank conflicts in the
 // pencil exchanges
 #ifndef QL_ZPAIR32
 #define QL_ZPAIR32 1
 #endif
-// FP64 at N = 256 (32-lane radix-(4, 4, 8) plan): unpadded pencils with XOR
+// FP64 at N = 256 (32-lane radix-(4, 4, 4, 2) plan): unpadded pencils with XOR
 // swizzles in the FFT exchanges (fft_core.hpp LDS_XOR32)
 #ifndef QL_ZXOR64
 #define QL_ZXOR64 1
