@@ -422,18 +422,22 @@ struct PFFT {
   // LDS_PAIR16 the column is XORed with (row & mask) << shift, chosen so that
   // for every (t, r) the 16 lanes of a pencil write distinct columns: the
   // inverse radix-(4, 4, 8) plan's writes after pass 0 (4b + r) and pass 1
-  // ((b / 4) 16 + b % 4 + 4r) and the forward radix-(8, 4, 4) plan's after
-  // pass 0 (8b + r) and pass 1 ((b / 8) 32 + b % 8 + 8r); the reads
-  // (b + B r) are consecutive runs of 16 and stay conflict-free under any
-  // row swizzle.
-  static constexpr int xmask(int e) { return FWD ? (e == 1 ? 7 : 2) : 3; }
-  static constexpr int xshift(int e) { return FWD ? (e == 1 ? 0 : 2) : (e == 1 ? 0 : 2); }
+  // ((b / 4) 16 + b % 4 + 4r), the forward radix-(8, 4, 4) plan's after
+  // pass 0 (8b + r) and pass 1 ((b / 8) 32 + b % 8 + 8r), and the M = 64
+  // radix-(4, 4, 4) plan's (4b + r, (b / 4) 16 + b % 4 + 4r) both ways; the
+  // reads (b + B r) are consecutive runs of 16 and stay conflict-free under
+  // any row swizzle.
+  static constexpr int xmask(int e) { return (FWD && M == 128) ? (e == 1 ? 7 : 2) : 3; }
+  static constexpr int xshift(int e) { return e == 1 ? 0 : 2; }
   template <int e>
   __device__ __forceinline__ static int xat(int n) {
     if constexpr (PAD == LDS_PAIR16) {
-      static_assert(TP == 16 && M == 128 && NP == 3 &&
-                        (FWD ? (R(0) == 8 && R(1) == 4 && R(2) == 4) : (R(0) == 4 && R(1) == 4 && R(2) == 8)),
-                    "the paired LDS layout's swizzles are derived for the 16-lane radix-(8, 4, 4) plan");
+      static_assert(TP == 16 && NP == 3 &&
+                        ((M == 128 && (FWD ? (R(0) == 8 && R(1) == 4 && R(2) == 4)
+                                           : (R(0) == 4 && R(1) == 4 && R(2) == 8))) ||
+                         (M == 64 && R(0) == 4 && R(1) == 4 && R(2) == 4)),
+                    "the paired LDS layout's swizzles are derived for the 16-lane radix-(8, 4, 4) and "
+                    "radix-(4, 4, 4) plans");
       const int row = n >> 4;
       return row * 32 + ((n & 15) ^ ((row & xmask(e)) << xshift(e)));
     } else if constexpr (PAD == LDS_XOR32) {

@@ -616,9 +616,9 @@ __device__ __forceinline__ void r2c_post_store(cx<T>* dst, const cx<T>* pb, cons
 // every index is a compile-time constant (no private-memory arrays)
 // plan selector of the 3-D z-pass (SEL of ZCfg): 1 in FP32, 2 in FP64
 template <typename T> struct ZSel { static constexpr int value = sizeof(T) == 4 ? 1 : 2; };
-// FP32 at N = 256 (16-lane radix-(8, 4, 4) plan): the paired, per-exchange
-// swizzled LDS layout (fft_core.hpp LDS_PAIR16), no bank conflicts in the
-// pencil exchanges
+// FP32 at N = 256 and 128 (16-lane radix-(8, 4, 4) and (4, 4, 4) plans): the
+// paired, per-exchange swizzled LDS layout (fft_core.hpp LDS_PAIR16), no bank
+// conflicts in the pencil exchanges
 #ifndef QL_ZPAIR32
 #define QL_ZPAIR32 1
 #endif
@@ -630,7 +630,7 @@ template <typename T> struct ZSel { static constexpr int value = sizeof(T) == 4 
 template <typename T, int N, int V>
 struct ZPadSel {
   static constexpr int value = sizeof(T) == 8 ? ((QL_ZXOR64 && N == 256 && V == 0) ? LDS_XOR32 : QL_ZPAD64)
-                                              : ((QL_ZPAIR32 && N == 256 && V == 0) ? LDS_PAIR16 : QL_ZPAD);
+                                              : ((QL_ZPAIR32 && (N == 256 || N == 128) && V == 0) ? LDS_PAIR16 : QL_ZPAD);
 };
 template <typename T, int N, int V>
 using Z3Cfg = ZCfg<N, V, ZSel<T>::value, ZPadSel<T, N, V>::value>;
