@@ -147,30 +147,44 @@ struct AdjZCfg {
   static constexpr int THREADS = P * TP;
   static constexpr int LSR = M + 1 + LPAD;
 };
+// LDS layout of the pencils: FP32 at N = 128 (8-lane radix-(8, 8) plan) in
+// groups of four pencils with the exchange swizzled (fft_core.hpp LDS_QUAD8,
+// no bank conflicts); otherwise one contiguous row of LSR entries per pencil
+template <typename T, int N>
+struct AdjZLay {
+  using C = AdjZCfg<N>;
+  static constexpr int LAY = (QL_ZPAIR32 && sizeof(T) == 4 && N == 128) ? LDS_QUAD8 : 0;
+  static constexpr bool GROUPED = LAY == LDS_QUAD8;
+  static constexpr int GS = (C::M / 8 + 1) * 32;  // elements of a group of four pencils
+  static_assert(!GROUPED || (C::TP == 8 && C::P % 4 == 0), "grouped layout: four pencils of 8 lanes");
+  static constexpr int BUF = GROUPED ? (C::P / 4) * GS : C::P * C::LSR;
+  static __device__ __forceinline__ int pencil_off(int p) { return GROUPED ? (p >> 2) * GS + (p & 3) * 8 : p * C::LSR; }
+};
 
 template <typename T, int N>
 __global__ void __launch_bounds__(AdjZCfg<N>::THREADS)
 k_adj_z(const cx<T>* __restrict__ in, const T* __restrict__ w, const T* __restrict__ u, cx<T>* __restrict__ out,
         const cx<T>* __restrict__ tw, T h) {
   using C = AdjZCfg<N>;
-  constexpr int M = C::M, Nh = M + 1, TP = C::TP;
-  using CI = PFFT<T, false, M, TP, Reverse<typename C::R>>;
-  using CF = PFFT<T, true, M, TP, typename C::R>;
+  using L = AdjZLay<T, N>;
+  constexpr int M = C::M, Nh = M + 1, TP = C::TP, LAY = L::LAY;
+  using CI = PFFT<T, false, M, TP, Reverse<typename C::R>, LAY>;
+  using CF = PFFT<T, true, M, TP, typename C::R, LAY>;
   static_assert(CI::NOUT == CF::NIN, "inverse output order must be the forward input order");
   constexpr int NO = CI::NOUT;
   constexpr size_t rows = (size_t)N * N;
   __shared__ cx<T> s_tw[N];
-  __shared__ cx<T> s_buf[C::P * C::LSR];
+  __shared__ cx<T> s_buf[L::BUF];
   const int tid = threadIdx.x, p = tid / TP, j = tid % TP;
   stage_table(s_tw, tw, N);
   const size_t row = (size_t)blockIdx.x * C::P + p;  // rows is a multiple of P
-  cx<T>* pb = s_buf + p * C::LSR;
+  cx<T>* pb = s_buf + L::pencil_off(p);
   cx<T> psi[3][NO];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     __syncthreads();  // the previous transform's LDS scratch is free (and the table staged)
     const cx<T>* src = in + (c * rows + row) * Nh;
-    for (int k = j; k <= M; k += TP) pb[k] = src[k];
+    for (int k = j; k <= M; k += TP) pb[lds_at<LAY>(k)] = src[k];
     __syncthreads();
     cx<T> v[CI::MAXE];
 #pragma unroll
@@ -178,7 +192,7 @@ k_adj_z(const cx<T>* __restrict__ in, const T* __restrict__ w, const T* __restri
       cx<T> z = {0, 0};
       if (CI::in_valid(j, s)) {
         const int n = CI::in_index(j, s);
-        cx<T> xn = pb[n], xr = pb[M - n];
+        cx<T> xn = pb[lds_at<LAY>(n)], xr = pb[lds_at<LAY>(M - n)];
         if (n == 0) {
           xn.im = 0;
           xr.im = 0;
@@ -215,12 +229,12 @@ k_adj_z(const cx<T>* __restrict__ in, const T* __restrict__ w, const T* __restri
     CF::run(v, pb, 1, j, s_tw, 2);
 #pragma unroll
     for (int s = 0; s < CF::NOUT; ++s)
-      if (CF::out_valid(j, s)) pb[CF::out_index(j, s)] = v[s];
+      if (CF::out_valid(j, s)) pb[lds_at<LAY>(CF::out_index(j, s))] = v[s];
     __syncthreads();
     cx<T>* dst = out + (o * rows + row) * Nh;
     for (int k = j; k <= M; k += TP) {
-      const cx<T> zk = pb[k == M ? 0 : k];
-      const cx<T> zr = cconj(pb[k == 0 ? 0 : M - k]);
+      const cx<T> zk = pb[lds_at<LAY>(k == M ? 0 : k)];
+      const cx<T> zr = cconj(pb[lds_at<LAY>(k == 0 ? 0 : M - k)]);
       const cx<T> e = cscale(cadd(zk, zr), h);
       const cx<T> od = crot<T, true>(cscale(csub(zk, zr), h));  // (zk - zr) / (2i), times 2h
       const cx<T> wk = k == M ? cx<T>{-1, 0} : s_tw[k];

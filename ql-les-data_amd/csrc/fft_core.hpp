@@ -289,11 +289,18 @@ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 // 20-27} and {4-11, 16-19, 28-31} of each half wave together, and a 16-byte
 // write 8 consecutive lanes over 32 banks, so consecutive runs need no
 // padding; the FFT exchanges XOR low index bits with higher ones (PFFT::xat).
+//
+// PAD == LDS_QUAD8: pencils of 8 lanes with 8-byte elements stored in groups
+// of four, element n of a pencil in row n / 8 (32 elements per row, 8 per
+// pencil of the group) at column n % 8; the radix-(8, 8) plan's one exchange
+// XORs the column with the row.
 constexpr int LDS_PAIR16 = -1;
 constexpr int LDS_XOR32 = -2;
+constexpr int LDS_QUAD8 = -3;
 template <int PAD>
 __device__ __forceinline__ int lds_at(int n) {
   if constexpr (PAD == LDS_PAIR16) return (n >> 4) * 32 + (n & 15);
+  else if constexpr (PAD == LDS_QUAD8) return (n >> 3) * 32 + (n & 7);
   else if constexpr (PAD == LDS_XOR32) return n;
   else if constexpr (PAD > 0) return n + (n >> PAD);
   else return n;
@@ -440,6 +447,11 @@ struct PFFT {
                     "radix-(4, 4, 4) plans");
       const int row = n >> 4;
       return row * 32 + ((n & 15) ^ ((row & xmask(e)) << xshift(e)));
+    } else if constexpr (PAD == LDS_QUAD8) {
+      static_assert(TP == 8 && M == 64 && NP == 2 && R(0) == 8 && R(1) == 8,
+                    "the grouped 8-lane LDS layout's swizzle is derived for the radix-(8, 8) plan");
+      const int row = n >> 3;
+      return row * 32 + ((n & 7) ^ (row & 7));
     } else if constexpr (PAD == LDS_XOR32) {
       // n ^ ((n >> a1) & m1) << s1 ^ ((n >> a2) & m2) << s2 per exchange,
       // searched over with the 16-byte bank rule: every write (8 consecutive

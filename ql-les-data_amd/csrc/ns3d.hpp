@@ -535,15 +535,22 @@ struct ZCfg {
   static constexpr int P0 = QL_ZTHREADS / TP;
   static constexpr int P = (N <= 64) ? (P0 > 8 ? 8 : P0) : (P0 > 16 ? 16 : P0);
   static constexpr int THREADS = P * TP;
-  static constexpr int ZPAD = PADV;  // one LDS padding slot per 2^ZPAD elements of a pencil, or LDS_PAIR16
-  static constexpr bool PAIRED = ZPAD == LDS_PAIR16;
+  // one LDS padding slot per 2^ZPAD elements of a pencil, or a grouped /
+  // swizzled layout (fft_core.hpp LDS_PAIR16, LDS_QUAD8, LDS_XOR32)
+  static constexpr int ZPAD = PADV;
+  // grouped layouts: G pencils of TP lanes side by side in rows of 32
+  // elements (TP of each pencil), element M in the last row
+  static constexpr bool PAIRED = ZPAD == LDS_PAIR16 || ZPAD == LDS_QUAD8;
+  static constexpr int G = ZPAD == LDS_PAIR16 ? 2 : 4;
   static constexpr int LSR = M + (ZPAD > 0 ? (M >> ZPAD) : 0) + 1 + LPAD;
-  // paired layout: rows of 32 elements per pencil pair, element M in the last
-  static constexpr int PAIR = (M / 16 + 1) * 32;
-  static_assert(!PAIRED || (TP == 16 && P % 2 == 0), "paired LDS layout: pencils of 16 lanes, in pairs");
-  static constexpr int BUF = PAIRED ? (P / 2) * PAIR : P * LSR;  // LDS elements of the block's pencils
+  static constexpr int PAIR = (M / (32 / G) + 1) * 32;
+  static_assert(!PAIRED || (TP * G == 32 && P % G == 0), "grouped LDS layout: G pencils of 32 / G lanes");
+  static constexpr int BUF = PAIRED ? (P / G) * PAIR : P * LSR;  // LDS elements of the block's pencils
   // first element of pencil p of the block
-  static __device__ __forceinline__ int pencil_off(int p) { return PAIRED ? (p >> 1) * PAIR + (p & 1) * 16 : p * LSR; }
+  static constexpr int GL = G == 2 ? 1 : 2;  // log2 G
+  static __device__ __forceinline__ int pencil_off(int p) {
+    return PAIRED ? (p >> GL) * PAIR + (p & (G - 1)) * TP : p * LSR;
+  }
 };
 
 // W_N^{B0 r} a with N = 2M, r < R: a compile-time constant for each r (r is a
@@ -617,8 +624,9 @@ __device__ __forceinline__ void r2c_post_store(cx<T>* dst, const cx<T>* pb, cons
 // plan selector of the 3-D z-pass (SEL of ZCfg): 1 in FP32, 2 in FP64
 template <typename T> struct ZSel { static constexpr int value = sizeof(T) == 4 ? 1 : 2; };
 // FP32 at N = 256 and 128 (16-lane radix-(8, 4, 4) and (4, 4, 4) plans): the
-// paired, per-exchange swizzled LDS layout (fft_core.hpp LDS_PAIR16), no bank
-// conflicts in the pencil exchanges
+// paired, per-exchange swizzled LDS layout (fft_core.hpp LDS_PAIR16); the
+// 8-lane radix-(8, 8) plan of N = 128's second variant: groups of four
+// pencils (LDS_QUAD8).  No bank conflicts in the pencil exchanges
 #ifndef QL_ZPAIR32
 #define QL_ZPAIR32 1
 #endif
@@ -630,7 +638,9 @@ template <typename T> struct ZSel { static constexpr int value = sizeof(T) == 4 
 template <typename T, int N, int V>
 struct ZPadSel {
   static constexpr int value = sizeof(T) == 8 ? ((QL_ZXOR64 && N == 256 && V == 0) ? LDS_XOR32 : QL_ZPAD64)
-                                              : ((QL_ZPAIR32 && (N == 256 || N == 128) && V == 0) ? LDS_PAIR16 : QL_ZPAD);
+                                              : !QL_ZPAIR32 ? QL_ZPAD
+                                              : ((N == 256 || N == 128) && V == 0) ? LDS_PAIR16
+                                              : (N == 128 && V == 1) ? LDS_QUAD8 : QL_ZPAD;
 };
 template <typename T, int N, int V>
 using Z3Cfg = ZCfg<N, V, ZSel<T>::value, ZPadSel<T, N, V>::value>;
@@ -691,8 +701,8 @@ struct ZState {
   __device__ __forceinline__ void fetch(int f, cx<T>* dst, int r0) {
     const cx<T>* src = a.in + f * fs + (size_t)(r0 + wrow) * M;
     if constexpr (C::PAIRED) {
-      // 16 consecutive entries of each pencil per load: lanes 16q .. 16q+15
-      // take entries 16e .. 16e+15 of pencil q (128-byte runs)
+      // TP consecutive entries of each pencil per load: lanes TP q .. TP q +
+      // TP - 1 take entries TP e .. TP e + TP - 1 of pencil q
       static_assert(NEM * TP == M, "paired fetch covers each row");
 #pragma unroll
       for (int e = 0; e < NEM; ++e) dst[e] = ld_stream<NTL>(&src[(ln / TP) * M + TP * e + ln % TP]);
