@@ -540,16 +540,16 @@ struct ZCfg {
   static constexpr int ZPAD = PADV;
   // grouped layouts: G pencils of TP lanes side by side in rows of 32
   // elements (TP of each pencil), element M in the last row
-  static constexpr bool PAIRED = ZPAD == LDS_PAIR16 || ZPAD == LDS_QUAD8;
+  static constexpr bool GROUPED = ZPAD == LDS_PAIR16 || ZPAD == LDS_QUAD8;
   static constexpr int G = ZPAD == LDS_PAIR16 ? 2 : 4;
   static constexpr int LSR = M + (ZPAD > 0 ? (M >> ZPAD) : 0) + 1 + LPAD;
-  static constexpr int PAIR = (M / (32 / G) + 1) * 32;
-  static_assert(!PAIRED || (TP * G == 32 && P % G == 0), "grouped LDS layout: G pencils of 32 / G lanes");
-  static constexpr int BUF = PAIRED ? (P / G) * PAIR : P * LSR;  // LDS elements of the block's pencils
+  static constexpr int GSTRIDE = (M / (32 / G) + 1) * 32;  // elements of a group of G pencils
+  static_assert(!GROUPED || (TP * G == 32 && P % G == 0), "grouped LDS layout: G pencils of 32 / G lanes");
+  static constexpr int BUF = GROUPED ? (P / G) * GSTRIDE : P * LSR;  // LDS elements of the block's pencils
   // first element of pencil p of the block
   static constexpr int GL = G == 2 ? 1 : 2;  // log2 G
   static __device__ __forceinline__ int pencil_off(int p) {
-    return PAIRED ? (p >> GL) * PAIR + (p & (G - 1)) * TP : p * LSR;
+    return GROUPED ? (p >> GL) * GSTRIDE + (p & (G - 1)) * TP : p * LSR;
   }
 };
 
@@ -700,10 +700,10 @@ struct ZState {
   // request field f of the pencils whose block starts at row r0
   __device__ __forceinline__ void fetch(int f, cx<T>* dst, int r0) {
     const cx<T>* src = a.in + f * fs + (size_t)(r0 + wrow) * M;
-    if constexpr (C::PAIRED) {
+    if constexpr (C::GROUPED) {
       // TP consecutive entries of each pencil per load: lanes TP q .. TP q +
       // TP - 1 take entries TP e .. TP e + TP - 1 of pencil q
-      static_assert(NEM * TP == M, "paired fetch covers each row");
+      static_assert(NEM * TP == M, "grouped fetch covers each row");
 #pragma unroll
       for (int e = 0; e < NEM; ++e) dst[e] = ld_stream<NTL>(&src[(ln / TP) * M + TP * e + ln % TP]);
     } else {
@@ -716,7 +716,7 @@ struct ZState {
     if (ln < PWV) dst[NEM] = a.in[a.q + (size_t)f * N * N + r0 + wrow + ln];
   }
   __device__ __forceinline__ void park(const cx<T>* src) {
-    if constexpr (C::PAIRED) {
+    if constexpr (C::GROUPED) {
 #pragma unroll
       for (int e = 0; e < NEM; ++e) s_buf[C::pencil_off(wrow + ln / TP) + lds_at<ZP>(TP * e + ln % TP)] = src[e];
     } else {
