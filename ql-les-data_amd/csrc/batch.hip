@@ -79,6 +79,12 @@ void drop_groups(qlles_batch* b) {
 }
 
 void drop_graphs(qlles_batch* b) {
+  // qlles_batch_run returns with its launches queued: a later configure, add
+  // or destroy must not free graphs those launches are still executing
+  bool live = b->all != nullptr;
+  for (const auto& x : b->m) live = live || x.g != nullptr;
+  if (live)
+    for (hipStream_t s : b->streams) (void)hipStreamSynchronize(s);
   for (auto& x : b->m)
     if (x.g) {
       (void)hipGraphExecDestroy(x.g);
@@ -94,6 +100,7 @@ void drop_graphs(qlles_batch* b) {
 }
 
 void drop_streams(qlles_batch* b) {
+  for (hipStream_t s : b->streams) (void)hipStreamSynchronize(s);  // work queued by an earlier run
   for (hipStream_t s : b->streams) (void)hipStreamDestroy(s);
   for (hipEvent_t e : b->join) (void)hipEventDestroy(e);
   b->streams.clear();
