@@ -87,6 +87,28 @@ def run_gpu(args, precision, steps, warmup, profile, repeats=1):
         _lib.check(lib.qlles_profile_read(step.plan.handle, ms_a, cnt_a, by_a))
         _lib.check(lib.qlles_profile(step.plan.handle, 0))
         prof = {_lib.KINDS[k]: dict(ms=ms_a[k], count=cnt_a[k], bytes=by_a[k]) for k in range(5) if cnt_a[k]}
+    u_each = None
+    if profile:
+        # the reference's step_fn contract (spectral_solver.py:114-118): the physical u returned every step,
+        # as a caller gets it from this API (one call per step with u_out); on copies, so the trajectory
+        # whose energy is compared does not move
+        uh2, r2, ur2 = uh.clone(), r.clone(), torch.empty_like(ur)
+        u_phys = torch.empty((3, N, N, N), dtype=step.dtype, device=uh.device)
+
+        def advance_u(k):
+            for _ in range(k):
+                _lib.check(lib.qlles_step_advect(step.plan.handle, it.it.handle, _lib.ptr(u_phys), _lib.ptr(uh2), 1,
+                                                 _lib.ptr(r2), r2.shape[0], _lib.ptr(ur2), None, _lib.stream_ptr()))
+
+        advance_u(3)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        advance_u(steps)
+        torch.cuda.synchronize()
+        dt_u = time.perf_counter() - t0
+        u_each = dict(steps_per_s=steps / dt_u, ms_per_step=1e3 * dt_u / steps, calls=steps,
+                      how="one qlles_step_advect call per step with u_out (the physical u written every step)")
+        del uh2, r2, ur2, u_phys
     sps = float(np.median(rates))
     u_fin = torch.empty((3, N, N, N), dtype=step.dtype, device=uh.device)
     _lib.check(lib.qlles_irfftn(step.plan.handle, _lib.ptr(uh), _lib.ptr(u_fin), _lib.stream_ptr()))
@@ -97,7 +119,7 @@ def run_gpu(args, precision, steps, warmup, profile, repeats=1):
     del uh, u_fin, step, it
     torch.cuda.empty_cache()
     reps = dict(n=len(rates), median=sps, min=min(rates), max=max(rates), steps_each=steps)
-    return sps, e_fin, prof, finite, reps, nufft
+    return sps, e_fin, prof, finite, reps, nufft, u_each
 
 
 def kernel_table(prof, traffic):
@@ -930,7 +952,7 @@ def main():
     import torch
 
     torch.cuda.set_device(0)
-    sps32, e32, prof, ok32, reps32, nufft = run_gpu(args, "float32", args.steps, args.warmup, profile=True,
+    sps32, e32, prof, ok32, reps32, nufft, u_each32 = run_gpu(args, "float32", args.steps, args.warmup, profile=True,
                                                     repeats=args.repeats)
     # the FP32 run advanced warmup + repeats x steps (timed) + steps (profiled)
     horizon = args.warmup + (args.repeats + 1) * args.steps
@@ -940,12 +962,13 @@ def main():
         k64 = max(10, args.steps // 2)
         # the FP64 run covers the same horizon for the energy comparison: its warm-up, 3 timed repeats
         # and one profiled region of k64 steps each
-        sps64, e64, prof64, ok64, reps64, nufft64 = run_gpu(args, "float64", k64, horizon - 4 * k64,
+        sps64, e64, prof64, ok64, reps64, nufft64, u_each64 = run_gpu(args, "float64", k64, horizon - 4 * k64,
                                                             profile=True, repeats=3)
         ke_err = abs(e32 - e64) / abs(e64)
         k64s, roof64 = kernel_table(prof64, None)
         fp64 = dict(steps_per_s=sps64, ms_per_step=1000.0 / sps64, speedup_fp32_over_fp64=sps32 / sps64,
                     e_kin=e64, finite=ok64, repeats=reps64, roofline=roof64, kernels=k64s, nufft=nufft64,
+                    u_every_step=u_each64,
                     alg_bytes_per_step=sum(v["bytes"] for v in prof64.values()) / k64)
         fp64["alg_frac"] = fp64["alg_bytes_per_step"] * sps64 / (HBM_PEAK_GBS * 1e9)
         t32, f32 = energy_trajectory(args, "float32")
@@ -990,6 +1013,7 @@ def main():
                    "reductions": "f64", "positions": "f64"},
         "roofline": roof,
         "repeats": reps32,
+        "u_every_step": u_each32,
         "step_hbm": {"ledger_bytes_per_step": ledger, "ledger_frac": ledger * sps32 / (HBM_PEAK_GBS * 1e9),
                      "alg_bytes_per_step": alg_step,
                      "alg_frac": (alg_step * sps32 / (HBM_PEAK_GBS * 1e9)) if alg_step else None,

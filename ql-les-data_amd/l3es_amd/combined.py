@@ -49,14 +49,21 @@ def combined(case, N=256, dim=3, nu=0.000625, t_final=0.1, t_burnin=0, dt=0.01, 
     t_sim, t_int, t0 = 0.0, 0.0, time()
     e_acc = 0.0
     rows = []
+    u_fresh = True  # u holds irfftn(u_hat)
     for i in range(tstep_max + 1):
         if i == burnin_steps and dst_path:  # combined.py:89-90: the full-resolution field before step i
             write_u(u_hat, i, dst_path, integrate_fn.plan, N, suffix="_burnin", u=u)
         if i == burnin_steps and case == "HIT":
             integrate_fn = rk4_wrapper(dt, rhs_wrapper(N, nu, fft_axes, precision), None, e_kin_init, "none",
                                        viscous_mode=viscous_mode)
+        # combined.py:97.  The step's physical output u is formed only where it is read: the logged
+        # diagnostics, the burn-in snapshot before the next step and the returned state (the particles
+        # read the spectrum).  Stage 0 takes u when it is current (spectral_solver.py:51) and
+        # transforms u_hat itself otherwise.
+        want_u = i % log_freq == 0 or i == tstep_max or (dst_path is not None and i + 1 == burnin_steps)
         t_temp = time()
-        integrate_fn.advance(u, u_hat, 1, u, e_buf)  # combined.py:97
+        integrate_fn.advance(u if u_fresh else None, u_hat, 1, u if want_u else None, e_buf)
+        u_fresh = want_u
         torch.cuda.synchronize()
         t_sim += time() - t_temp
         e_acc += float(e_buf[0])
