@@ -206,6 +206,14 @@ template <> struct XPlan<128> { using R = Radices<4, 4, 8>; static constexpr int
 #ifndef QL_NT_XS
 #define QL_NT_XS 1
 #endif
+// The last step of a call writes u_{n+1} into the caller's spectrum, whose
+// rows hold N/2+1 complex: a 16-column tile's 128-byte run then straddles two
+// cache lines, and the neighbouring tiles that write the rest of those lines
+// run on other XCDs.  Non-temporal stores send each half line to HBM on its
+// own; cached stores (0) let the last-level cache merge them.
+#ifndef QL_NT_CALLER
+#define QL_NT_CALLER 0
+#endif
 #ifndef QL_NT_X5
 #define QL_NT_X5 0
 #endif
