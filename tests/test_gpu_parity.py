@@ -509,6 +509,35 @@ def test_simulate_driver_tgv(gpu, tmp_path):
     assert np.load(os.path.join(tmp_path, "u_32_00000_burnin.npy")).shape == (3, 32, 32, 32)
 
 
+def test_simulate_full_resolution_checkpoints(gpu, tmp_path):
+    """The driver forms the physical u only on the steps that read it. Full-resolution checkpoints
+    (ckp_N = N, written from u itself, utils.py:161) every 2 steps, diagnostics every 3 and the
+    returned field must each be the state after their own step: TGV 32^3 FP64 against the oracle."""
+    import csv as _csv
+    import os
+
+    from l3es_amd import simulate
+
+    N, nu, dt = 32, 0.000625, 0.01
+    out = simulate("TGV3D", N=N, nu=nu, t_final=0.06, dt=dt, ckp_N=N, log_freq=3, ckp_freq=2,
+                   dst_path=str(tmp_path), precision="float64", verbose=False)
+    u, uh, step, _, _ = O.set_up_solver(N, nu, 3, "TGV3D", dt, seed=42, warmup=True)
+    states = []
+    for _ in range(7):  # loop steps 0..6
+        u, uh, _ = step(u, uh)
+        states.append(u)
+    for i in (0, 2, 4, 6):
+        ck = np.load(os.path.join(tmp_path, "sim", f"u_{N}_{i:05d}.npy"))
+        assert ck.shape == (3, N, N, N) and rel(ck, states[i]) < 1e-12, f"checkpoint of step {i}"
+    assert rel(np.load(os.path.join(tmp_path, f"u_{N}_00000_burnin.npy")), states[0]) < 1e-12
+    rows = list(_csv.reader(open(os.path.join(tmp_path, "diagnostics.csv"))))
+    assert [int(r[0]) for r in rows[1:]] == [0, 3, 6]
+    for r in rows[1:]:
+        e = 0.5 * np.mean(np.sum(states[int(r[0])] ** 2, axis=0))
+        assert abs(float(r[3]) - e) < 1e-12 and abs(float(r[2]) - np.abs(states[int(r[0])]).max()) < 1e-12
+    assert rel(host(out["u"]), states[-1]) < 1e-12
+
+
 def _oracle_hit_burnin(N, nu, dt, kf, e0, forced_steps, total_steps, log_freq):
     """The reference's forced-HIT driver loop (spectral_solver.py:300-320): the set-up's warm-up step,
     forced steps until the burn-in step, then steps without forcing; the injection rates summed, the
