@@ -12,6 +12,7 @@ part of this library; particle frames are written as .npz (no h5py).
 
 import csv
 import os
+from concurrent.futures import ThreadPoolExecutor
 from time import time
 
 import numpy as np
@@ -20,6 +21,11 @@ import torch
 from .integrator import set_up_integrator
 from .relax import relax_advect
 from .spectral_solver import diagnostics, rk4_wrapper, rhs_wrapper, set_up_solver, write_u
+
+
+def _save_frame(path, r, u_r):
+    """One particle frame: device copies of the positions and velocities taken at that step."""
+    np.savez(path, r=r.cpu().numpy(), u=u_r.cpu().numpy())
 
 
 def combined(case, N=256, dim=3, nu=0.000625, t_final=0.1, t_burnin=0, dt=0.01, u_ref=4.0, ckp_N=64,
@@ -50,52 +56,59 @@ def combined(case, N=256, dim=3, nu=0.000625, t_final=0.1, t_burnin=0, dt=0.01, 
     e_acc = 0.0
     rows = []
     u_fresh = True  # u holds irfftn(u_hat)
-    for i in range(tstep_max + 1):
-        if i == burnin_steps and dst_path:  # combined.py:89-90: the full-resolution field before step i
-            write_u(u_hat, i, dst_path, integrate_fn.plan, N, suffix="_burnin", u=u)
-        if i == burnin_steps and case == "HIT":
-            integrate_fn = rk4_wrapper(dt, rhs_wrapper(N, nu, fft_axes, precision), None, e_kin_init, "none",
-                                       viscous_mode=viscous_mode)
-        # combined.py:97.  The step's physical output u is formed only where it is read: the logged
-        # diagnostics, the burn-in snapshot before the next step and the returned state (the particles
-        # read the spectrum).  Stage 0 takes u when it is current (spectral_solver.py:51) and
-        # transforms u_hat itself otherwise.
-        want_u = i % log_freq == 0 or i == tstep_max or (dst_path is not None and i + 1 == burnin_steps)
-        t_temp = time()
-        integrate_fn.advance(u if u_fresh else None, u_hat, 1, u if want_u else None, e_buf)
-        u_fresh = want_u
-        torch.cuda.synchronize()
-        t_sim += time() - t_temp
-        e_acc += float(e_buf[0])
-        t_temp = time()
-        is_ckp = (i % ckp_freq == 0) and (i >= burnin_steps) and int_path is not None and write_frames
-        r0 = r.clone() if is_ckp else None
-        need_rho = comp_rho is not None and (i % log_freq == 0 or is_ckp)
-        rho = None
-        if relax_dt_factor > 0.0:
-            interpolator.from_hat(u_hat, r, 0.0, u_r)  # combined.py:127,130
-            u_r0 = u_r.clone() if is_ckp else None  # combined.py:131
-            rho = comp_rho(r) if need_rho else None  # combined.py:147
-            relax_advect(relax_fn, r, u_r, dt, relax_dt_factor)  # combined.py:132-142, :199
-        else:
-            rho = comp_rho(r) if need_rho else None
-            interpolator.from_hat(u_hat, r, dt, u_r)  # combined.py:127,130,199
-            u_r0 = u_r
-        torch.cuda.synchronize()
-        t_int += time() - t_temp
-        rho_max = float(rho.max()) if rho is not None else None
-        if i % log_freq == 0:
-            d = diagnostics(u, integrate_fn.plan).cpu().numpy()
-            dt_est = float(1.0 / (d[2:5] / dx + nu / dx ** 2).min())
-            rows.append([i, float(i * dt), float(d[1]), float(d[0]), dt_est, float(e_buf[0]), rho_max])
-            if diag_path:
-                with open(diag_path, "a", newline="") as fh:
-                    csv.writer(fh).writerow(rows[-1])
-            if verbose:
-                rho_txt = f", rho_max = {rho_max:.3f}" if rho_max is not None else ""
-                print(f"Step {i}/{tstep_max}, u_max = {d[1]:.3f}, E_kin = {d[0]:.3f}, dt_est = {dt_est:.5f}{rho_txt}")
-        if is_ckp:
-            np.savez(os.path.join(int_path, f"step_{i:05d}.npz"), r=r0.cpu().numpy(), u=u_r0.cpu().numpy())
+    # Frames and the burn-in snapshot are written by one thread while the loop steps on.  It gets
+    # device copies taken at their step and makes no library call; the loop waits for every write
+    # (and re-raises its errors) before it stops the clock.
+    pending = []
+    with ThreadPoolExecutor(max_workers=1) as writer:
+        for i in range(tstep_max + 1):
+            if i == burnin_steps and dst_path:  # combined.py:89-90: the full-resolution field before step i
+                pending.append(writer.submit(write_u, None, i, dst_path, integrate_fn.plan, N, "_burnin", u.clone()))
+            if i == burnin_steps and case == "HIT":
+                integrate_fn = rk4_wrapper(dt, rhs_wrapper(N, nu, fft_axes, precision), None, e_kin_init, "none",
+                                           viscous_mode=viscous_mode)
+            # combined.py:97.  The step's physical output u is formed only where it is read: the logged
+            # diagnostics, the burn-in snapshot before the next step and the returned state (the particles
+            # read the spectrum).  Stage 0 takes u when it is current (spectral_solver.py:51) and
+            # transforms u_hat itself otherwise.
+            want_u = i % log_freq == 0 or i == tstep_max or (dst_path is not None and i + 1 == burnin_steps)
+            t_temp = time()
+            integrate_fn.advance(u if u_fresh else None, u_hat, 1, u if want_u else None, e_buf)
+            u_fresh = want_u
+            torch.cuda.synchronize()
+            t_sim += time() - t_temp
+            e_acc += float(e_buf[0])
+            t_temp = time()
+            is_ckp = (i % ckp_freq == 0) and (i >= burnin_steps) and int_path is not None and write_frames
+            r0 = r.clone() if is_ckp else None
+            need_rho = comp_rho is not None and (i % log_freq == 0 or is_ckp)
+            rho = None
+            if relax_dt_factor > 0.0:
+                interpolator.from_hat(u_hat, r, 0.0, u_r)  # combined.py:127,130
+                u_r0 = u_r.clone() if is_ckp else None  # combined.py:131
+                rho = comp_rho(r) if need_rho else None  # combined.py:147
+                relax_advect(relax_fn, r, u_r, dt, relax_dt_factor)  # combined.py:132-142, :199
+            else:
+                rho = comp_rho(r) if need_rho else None
+                interpolator.from_hat(u_hat, r, dt, u_r)  # combined.py:127,130,199
+                u_r0 = u_r.clone() if is_ckp else None
+            torch.cuda.synchronize()
+            t_int += time() - t_temp
+            rho_max = float(rho.max()) if rho is not None else None
+            if i % log_freq == 0:
+                d = diagnostics(u, integrate_fn.plan).cpu().numpy()
+                dt_est = float(1.0 / (d[2:5] / dx + nu / dx ** 2).min())
+                rows.append([i, float(i * dt), float(d[1]), float(d[0]), dt_est, float(e_buf[0]), rho_max])
+                if diag_path:
+                    with open(diag_path, "a", newline="") as fh:
+                        csv.writer(fh).writerow(rows[-1])
+                if verbose:
+                    rho_txt = f", rho_max = {rho_max:.3f}" if rho_max is not None else ""
+                    print(f"Step {i}/{tstep_max}, u_max = {d[1]:.3f}, E_kin = {d[0]:.3f}, dt_est = {dt_est:.5f}{rho_txt}")
+            if is_ckp:
+                pending.append(writer.submit(_save_frame, os.path.join(int_path, f"step_{i:05d}.npz"), r0, u_r0))
+        for f in pending:
+            f.result()
     t_tot = time() - t0
     if verbose:
         print(f"t_tot = {t_tot:.3f}, t_sim = {t_sim}, t_int = {t_int:.3f}")
