@@ -23,9 +23,22 @@ from .relax import relax_advect
 from .spectral_solver import diagnostics, rk4_wrapper, rhs_wrapper, set_up_solver, write_u
 
 
+def _to_host(t, dtype=None):
+    """Queue a copy of device tensor t into pinned host memory on the current stream."""
+    h = torch.empty(t.shape, dtype=dtype or t.dtype, pin_memory=True)
+    h.copy_(t if dtype is None else t.to(dtype), non_blocking=True)
+    return h
+
+
+def _when_copied(done, fn, *args):
+    """fn(*args) once the copies queued before event `done` have landed (runs on the writer thread)."""
+    done.synchronize()
+    fn(*args)
+
+
 def _save_frame(path, r, u_r):
-    """One particle frame: device copies of the positions and velocities taken at that step."""
-    np.savez(path, r=r.cpu().numpy(), u=u_r.cpu().numpy())
+    """One particle frame from host copies of the positions and velocities taken at that step."""
+    np.savez(path, r=r.numpy(), u=u_r.numpy())
 
 
 def combined(case, N=256, dim=3, nu=0.000625, t_final=0.1, t_burnin=0, dt=0.01, u_ref=4.0, ckp_N=64,
@@ -51,19 +64,26 @@ def combined(case, N=256, dim=3, nu=0.000625, t_final=0.1, t_burnin=0, dt=0.01, 
     tstep_max = round(t_final / dt)
     burnin_steps = round(t_burnin / dt) if t_burnin > 0 else 0
     e_buf = torch.zeros(1, dtype=torch.float64, device=u.device)
+    e_sum = torch.zeros((), dtype=torch.float64, device=u.device)  # sum of the injection rates, kept on the GPU
+    marks = []  # per step: events before the DNS step, after it, and after the particles
     u_r = torch.empty((r.shape[0], dim), dtype=interpolator.dtype, device=r.device)
-    t_sim, t_int, t0 = 0.0, 0.0, time()
-    e_acc = 0.0
+    t0 = time()
     rows = []
     u_fresh = True  # u holds irfftn(u_hat)
-    # Frames and the burn-in snapshot are written by one thread while the loop steps on.  It gets
-    # device copies taken at their step and makes no library call; the loop waits for every write
-    # (and re-raises its errors) before it stops the clock.
+    # Frames and the burn-in snapshot are copied to pinned host memory in stream order at their step
+    # and written by one thread while the loop steps on; the writer makes no library call, and the
+    # loop waits for every write (and re-raises its errors) before it stops the clock.  The loop
+    # itself reads the GPU only on logged steps (and, when verbose, on checkpoints), so the host
+    # queues steps ahead of the GPU; CUDA events time the DNS and particle phases.
     pending = []
     with ThreadPoolExecutor(max_workers=1) as writer:
         for i in range(tstep_max + 1):
             if i == burnin_steps and dst_path:  # combined.py:89-90: the full-resolution field before step i
-                pending.append(writer.submit(write_u, None, i, dst_path, integrate_fn.plan, N, "_burnin", u.clone()))
+                u_h = _to_host(u, torch.float64)
+                done = torch.cuda.Event()
+                done.record()
+                pending.append(writer.submit(_when_copied, done, write_u, None, i, dst_path, integrate_fn.plan, N,
+                                             "_burnin", u_h))
             if i == burnin_steps and case == "HIT":
                 integrate_fn = rk4_wrapper(dt, rhs_wrapper(N, nu, fft_axes, precision), None, e_kin_init, "none",
                                            viscous_mode=viscous_mode)
@@ -72,28 +92,29 @@ def combined(case, N=256, dim=3, nu=0.000625, t_final=0.1, t_burnin=0, dt=0.01, 
             # read the spectrum).  Stage 0 takes u when it is current (spectral_solver.py:51) and
             # transforms u_hat itself otherwise.
             want_u = i % log_freq == 0 or i == tstep_max or (dst_path is not None and i + 1 == burnin_steps)
-            t_temp = time()
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+            ev[0].record()
             integrate_fn.advance(u if u_fresh else None, u_hat, 1, u if want_u else None, e_buf)
             u_fresh = want_u
-            torch.cuda.synchronize()
-            t_sim += time() - t_temp
-            e_acc += float(e_buf[0])
-            t_temp = time()
+            ev[1].record()
+            e_sum += e_buf[0]
             is_ckp = (i % ckp_freq == 0) and (i >= burnin_steps) and int_path is not None and write_frames
-            r0 = r.clone() if is_ckp else None
-            need_rho = comp_rho is not None and (i % log_freq == 0 or is_ckp)
+            r0 = _to_host(r) if is_ckp else None
+            # combined.py:145-149 forms the density on logged steps and checkpoints; on checkpoints it
+            # only prints its maximum, so a silent run forms it on logged steps alone
+            need_rho = comp_rho is not None and (i % log_freq == 0 or (is_ckp and verbose))
             rho = None
             if relax_dt_factor > 0.0:
                 interpolator.from_hat(u_hat, r, 0.0, u_r)  # combined.py:127,130
-                u_r0 = u_r.clone() if is_ckp else None  # combined.py:131
+                u_r0 = _to_host(u_r) if is_ckp else None  # combined.py:131
                 rho = comp_rho(r) if need_rho else None  # combined.py:147
                 relax_advect(relax_fn, r, u_r, dt, relax_dt_factor)  # combined.py:132-142, :199
             else:
                 rho = comp_rho(r) if need_rho else None
                 interpolator.from_hat(u_hat, r, dt, u_r)  # combined.py:127,130,199
-                u_r0 = u_r.clone() if is_ckp else None
-            torch.cuda.synchronize()
-            t_int += time() - t_temp
+                u_r0 = _to_host(u_r) if is_ckp else None
+            ev[2].record()
+            marks.append(ev)
             rho_max = float(rho.max()) if rho is not None else None
             if i % log_freq == 0:
                 d = diagnostics(u, integrate_fn.plan).cpu().numpy()
@@ -106,10 +127,17 @@ def combined(case, N=256, dim=3, nu=0.000625, t_final=0.1, t_burnin=0, dt=0.01, 
                     rho_txt = f", rho_max = {rho_max:.3f}" if rho_max is not None else ""
                     print(f"Step {i}/{tstep_max}, u_max = {d[1]:.3f}, E_kin = {d[0]:.3f}, dt_est = {dt_est:.5f}{rho_txt}")
             if is_ckp:
-                pending.append(writer.submit(_save_frame, os.path.join(int_path, f"step_{i:05d}.npz"), r0, u_r0))
+                pending.append(writer.submit(_when_copied, ev[2], _save_frame,
+                                             os.path.join(int_path, f"step_{i:05d}.npz"), r0, u_r0))
+                if verbose and rho_max is not None:
+                    print(f"Step {i}/{tstep_max}, rho_max={rho_max:.3f}.")
         for f in pending:
             f.result()
+        torch.cuda.synchronize()
     t_tot = time() - t0
+    t_sim = sum(a.elapsed_time(b) for a, b, _ in marks) / 1e3
+    t_int = sum(b.elapsed_time(c) for _, b, c in marks) / 1e3
+    e_acc = float(e_sum)
     if verbose:
         print(f"t_tot = {t_tot:.3f}, t_sim = {t_sim}, t_int = {t_int:.3f}")
     return dict(u=u, u_hat=u_hat, r=r, u_r=u_r, rows=rows, t_sim=t_sim, t_int=t_int, t_tot=t_tot, e_inj_acc=e_acc,

@@ -165,15 +165,22 @@ def test_combined_with_relaxation_matches_stepwise_oracle(gpu, tmp_path):
     r = osph.lattice(ckp_N, 3)
     e = torch.zeros(1, dtype=torch.float64, device=gpu)
     rho0 = None
+    frames = []
     for i in range(steps):
         integrate_fn.advance(u, u_hat, 1, u, e)
         u_r = host(interp.from_hat(u_hat, dev(r, gpu), 0.0))
+        frames.append((r.copy(), u_r))
         if i == 0:
             rho0 = osph.density(r, ckp_N, 3).max()
         r, _, _ = osph.relax_advect(r, u_r, dt, f, ckp_N, 3, L, u_ref)
     d = np.mod(host(out["r"]) - r + L / 2, L) - L / 2
     assert np.max(np.abs(d)) < 1e-10
     assert abs(out["rows"][0][6] - rho0) < 1e-12
+    # every frame holds the positions and the interpolated velocities of its own step (combined.py:167-169)
+    for i, (r_i, u_i) in enumerate(frames):
+        fr = np.load(tmp_path / "com" / f"step_{i:05d}.npz")
+        assert np.max(np.abs(np.mod(fr["r"] - r_i + L / 2, L) - L / 2)) < 1e-10, f"frame {i}: positions"
+        assert np.max(np.abs(fr["u"] - u_i)) < 1e-10, f"frame {i}: velocities"
 
 
 def test_combined_2d_dft_with_relaxation_matches_stepwise_oracle(gpu, tmp_path):
@@ -183,9 +190,11 @@ def test_combined_2d_dft_with_relaxation_matches_stepwise_oracle(gpu, tmp_path):
     from l3es_amd.integrator import spectral_interpolator_wrapper
     from l3es_amd.spectral_solver import set_up_solver
 
-    N, ckp_N, dt, steps, f, u_ref = 64, 16, 0.005, 3, 1.4142135624, 6.0
+    # log_freq past the run: only step 0 reads the GPU, so the loop queues the other steps (and their
+    # frame copies) ahead of the GPU
+    N, ckp_N, dt, steps, f, u_ref = 64, 16, 0.005, 6, 1.4142135624, 6.0
     out = combined("Kolm", N=N, dim=2, nu=0.001, t_final=dt * (steps - 1), dt=dt, u_ref=u_ref, ckp_N=ckp_N,
-                   interp_backend="dft", dft_splits=2, relax_dt_factor=f, seed=3, log_freq=1, ckp_freq=1,
+                   interp_backend="dft", dft_splits=2, relax_dt_factor=f, seed=3, log_freq=100, ckp_freq=1,
                    dst_path=str(tmp_path), precision="float64", verbose=False)
     assert out["steps"] == steps and all(np.isfinite(row[6]) for row in out["rows"])
     u, u_hat, *_rest = set_up_solver(N, 0.001, 2, "Kolm", dt, 3, ckp_N, None, None, "none", True, "float64")
@@ -193,9 +202,15 @@ def test_combined_2d_dft_with_relaxation_matches_stepwise_oracle(gpu, tmp_path):
     interp = spectral_interpolator_wrapper(ckp_N, (1, 2), backend="dft", precision="float64")
     r = osph.lattice(ckp_N, 2)
     e = torch.zeros(1, dtype=torch.float64, device=gpu)
+    frames = []
     for _ in range(steps):
         integrate_fn.advance(u, u_hat, 1, u, e)
         u_r = host(interp.from_hat(u_hat, dev(r, gpu), 0.0))
+        frames.append((r.copy(), u_r))
         r, _, _ = osph.relax_advect(r, u_r, dt, f, ckp_N, 2, L, u_ref)
     d = np.mod(host(out["r"]) - r + L / 2, L) - L / 2
     assert np.max(np.abs(d)) < 1e-10
+    for i, (r_i, u_i) in enumerate(frames):
+        fr = np.load(tmp_path / "com" / f"step_{i:05d}.npz")
+        assert np.max(np.abs(np.mod(fr["r"] - r_i + L / 2, L) - L / 2)) < 1e-10, f"frame {i}: positions"
+        assert np.max(np.abs(fr["u"] - u_i)) < 1e-10, f"frame {i}: velocities"
