@@ -66,6 +66,17 @@ def combined(case, N=256, dim=3, nu=0.000625, t_final=0.1, t_burnin=0, dt=0.01, 
     e_buf = torch.zeros(1, dtype=torch.float64, device=u.device)
     e_sum = torch.zeros((), dtype=torch.float64, device=u.device)  # sum of the injection rates, kept on the GPU
     marks = []  # per step: events before the DNS step, after it, and after the particles
+    t_sim = t_int = 0.0
+    lead = 64  # steps the host may queue ahead; also the frame writes in flight
+
+    def fold(keep):
+        # add the phase times of all but the last `keep` steps (waiting for them) and drop their events
+        nonlocal t_sim, t_int
+        while len(marks) > keep:
+            a, b, c = marks.pop(0)
+            c.synchronize()
+            t_sim += a.elapsed_time(b) / 1e3
+            t_int += b.elapsed_time(c) / 1e3
     u_r = torch.empty((r.shape[0], dim), dtype=interpolator.dtype, device=r.device)
     t0 = time()
     rows = []
@@ -115,6 +126,7 @@ def combined(case, N=256, dim=3, nu=0.000625, t_final=0.1, t_burnin=0, dt=0.01, 
                 u_r0 = _to_host(u_r) if is_ckp else None
             ev[2].record()
             marks.append(ev)
+            fold(lead)
             rho_max = float(rho.max()) if rho is not None else None
             if i % log_freq == 0:
                 d = diagnostics(u, integrate_fn.plan).cpu().numpy()
@@ -129,14 +141,15 @@ def combined(case, N=256, dim=3, nu=0.000625, t_final=0.1, t_burnin=0, dt=0.01, 
             if is_ckp:
                 pending.append(writer.submit(_when_copied, ev[2], _save_frame,
                                              os.path.join(int_path, f"step_{i:05d}.npz"), r0, u_r0))
+                while len(pending) > lead:  # bounds the pinned frames held (and raises a failed write)
+                    pending.pop(0).result()
                 if verbose and rho_max is not None:
                     print(f"Step {i}/{tstep_max}, rho_max={rho_max:.3f}.")
         for f in pending:
             f.result()
         torch.cuda.synchronize()
     t_tot = time() - t0
-    t_sim = sum(a.elapsed_time(b) for a, b, _ in marks) / 1e3
-    t_int = sum(b.elapsed_time(c) for _, b, c in marks) / 1e3
+    fold(0)
     e_acc = float(e_sum)
     if verbose:
         print(f"t_tot = {t_tot:.3f}, t_sim = {t_sim}, t_int = {t_int:.3f}")
