@@ -188,8 +188,19 @@ struct NS3X {
 // spread over more lanes (fewer elements, hence fewer registers, per thread)
 // at the price of one more LDS round trip per transform.
 template <int N> struct XPlan { using R = typename Plan<N>::R; static constexpr int TP = Plan<N>::TP; };
+// 1: radix (4, 4, 4, 4) on 64 lanes per column at N = 256 (1024-thread
+// workgroups, 4 elements per lane).  Slower at 256^3: FP32 471.8 -> 418.8
+// steps/s (x-pass 218 -> 258 us); FP64 229.7 -> 217.0 (x-pass 424 -> 492 us,
+// spilling at its 128-VGPR cap; y-inverse 302 -> 290 us), so off
+#ifndef QL_XPLAN256_TP64
+#define QL_XPLAN256_TP64 0
+#endif
 #if QL_XPLAN_WIDE
+#if QL_XPLAN256_TP64
+template <> struct XPlan<256> { using R = Radices<4, 4, 4, 4>; static constexpr int TP = 64; };
+#else
 template <> struct XPlan<256> { using R = Radices<8, 4, 8>; static constexpr int TP = 32; };
+#endif
 template <> struct XPlan<128> { using R = Radices<4, 4, 8>; static constexpr int TP = 16; };
 #endif
 
