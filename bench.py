@@ -644,6 +644,58 @@ def run_dataset(args):
     print(json.dumps(out), flush=True)
 
 
+def run_onestep(args):
+    """The headline step by call pattern: one call of K steps, K one-step calls, and K one-step calls that
+    also return the physical u (the reference's step_fn contract, spectral_solver.py:114-118, as its loop
+    drives it: combined.py:97), at 256^3 FP32 and FP64 with the 32^3-particle advection."""
+    import torch
+
+    from l3es_amd import _lib, set_up_solver
+    from l3es_amd.integrator import pos_init_cartesian, spectral_interpolator_wrapper
+
+    torch.cuda.set_device(0)
+    N, n, K = args.N, args.ckp_N, min(args.steps, 50)
+    lib = _lib.load()
+    res = {}
+    for prec in ("float32", "float64"):
+        _, uh, _, _, step, _, _, _, _ = set_up_solver(
+            N, args.nu, 3, "HIT", args.dt, args.seed, n, args.kf, args.e_kin, "power", precision=prec,
+            spectrum="k4exp", power=args.power, warmup=False)
+        it = spectral_interpolator_wrapper(n, (1, 2, 3), backend="nufft", precision=prec, tol=args.tol)
+        r = torch.from_numpy(pos_init_cartesian(n, 3)).cuda()
+        ur = torch.empty((r.shape[0], 3), dtype=step.dtype, device=r.device)
+        u_phys = torch.empty((3, N, N, N), dtype=step.dtype, device=uh.device)
+
+        def call(k, u_out):
+            _lib.check(lib.qlles_step_advect(step.plan.handle, it.it.handle, _lib.ptr(u_out), _lib.ptr(uh), k,
+                                             _lib.ptr(r), r.shape[0], _lib.ptr(ur), None, _lib.stream_ptr()))
+
+        patterns = {"one_call_of_K_steps": lambda: call(K, None),
+                    "K_one_step_calls": lambda: [call(1, None) for _ in range(K)],
+                    "K_one_step_calls_with_u": lambda: [call(1, u_phys) for _ in range(K)]}
+        row = {}
+        for name, go in patterns.items():
+            call(3, None)  # warm-up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            go()
+            torch.cuda.synchronize()
+            dt_s = time.perf_counter() - t0
+            row[name] = dict(ms_per_step=1e3 * dt_s / K, steps_per_s=K / dt_s)
+            log(f"[bench] onestep {prec} {name}: {1e3 * dt_s / K:.3f} ms/step")
+        res[prec] = row
+        del uh, step, it, r, ur, u_phys
+        torch.cuda.empty_cache()
+    print(json.dumps({
+        "metric": "RK4 steps/s at 256^3 HIT with 32^3-particle NUFFT advection, by call pattern",
+        "value": res["float32"]["K_one_step_calls_with_u"]["steps_per_s"], "unit": "steps/s", "n_gpus": 1,
+        "steps": K, "warmup": 3, "ms_per_step": res["float32"]["K_one_step_calls_with_u"]["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"3-D forced HIT {N}^3 + {n}^3 particles; value: FP32, one qlles_step_advect "
+                               "call per step with u_out"},
+        "patterns": res}), flush=True)
+
+
 def run_horizon(args):
     """The FP32-vs-FP64 kinetic-energy trajectory of the headline configuration over a long horizon:
     both precisions advance the same initial field (256^3 power-forced HIT) and record the FP64 Parseval
@@ -880,13 +932,14 @@ def main():
                     help="only run a plain device-to-device copy of this many MiB (counter calibration)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH/WRITE passes")
     ap.add_argument("--workload", choices=("hit256", "batch", "adjoint", "relax", "dataset", "stats", "horizon",
-                                           "configs"),
+                                           "configs", "onestep"),
                     default="hit256",
                     help="hit256: the headline 256^3 step (default); batch: 24 uneven batched trajectories; "
                          "adjoint: gradient through K checkpointed steps; relax: SPH particle relaxation; "
                          "dataset: the dataset-generation loops end to end; stats: dataset statistics moments; "
                          "horizon: FP32-vs-FP64 energy trajectory over --samples x --every steps; "
-                         "configs: TGV 64^3 and 2-D Kolmogorov 256^2")
+                         "configs: TGV 64^3 and 2-D Kolmogorov 256^2; onestep: the 256^3 step by call pattern "
+                         "(K steps per call, one step per call, one step per call returning u)")
     ap.add_argument("--samples", type=int, default=20)
     ap.add_argument("--every", type=int, default=100)
     ap.add_argument("--adj-N", dest="adj_N", type=int, default=128)
@@ -947,6 +1000,9 @@ def main():
         return
     if args.workload == "configs":
         run_configs(args)
+        return
+    if args.workload == "onestep":
+        run_onestep(args)
         return
 
     import torch
