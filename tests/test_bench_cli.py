@@ -22,5 +22,5 @@ def test_more_than_one_gpu_is_refused():
 def test_workloads_are_listed():
     p = run("--help")
     assert p.returncode == 0
-    for w in ("hit256", "batch", "adjoint", "relax", "dataset", "stats", "horizon", "configs"):
+    for w in ("hit256", "batch", "adjoint", "relax", "dataset", "stats", "horizon", "configs", "onestep"):
         assert w in p.stdout
