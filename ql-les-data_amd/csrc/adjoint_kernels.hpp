@@ -82,6 +82,121 @@ __global__ void k_vjp_combine(const cx<T>* __restrict__ a, const cx<T>* __restri
   }
 }
 
+// k_pd (mode 1) and the inverse transform along x in one pass over
+// [3][N][N*(N/2+1)]: each workgroup loads all three components of TI columns,
+// forms P D l / c per mode, then transforms the components one after another
+// through one LDS tile.  Same arithmetic as k_pd followed by k_c2c_strided.
+template <typename T, int N>
+__global__ void __launch_bounds__((StridedCfg<T, N>::THREADS))
+k_adj_pd_x(const cx<T>* __restrict__ in, cx<T>* __restrict__ out, int KZD, const cx<T>* __restrict__ tw) {
+  using C = StridedCfg<T, N>;
+  using F = PFFT<T, false, N, C::TP, typename Plan<N>::R>;
+  constexpr int Nh = N / 2 + 1, inner = N * Nh;
+  constexpr size_t cs = (size_t)N * inner;
+  __shared__ cx<T> s_tw[N];
+  __shared__ cx<T> s_buf[N * C::LS];
+  const int tid = threadIdx.x, p = tid % TI, j = tid / TI;
+  stage_table(s_tw, tw, N);
+  __syncthreads();
+  const int col = blockIdx.x * TI + p;
+  const bool ok = col < inner;
+  cx<T> v[3][F::MAXE];
+#pragma unroll
+  for (int s = 0; s < F::NIN; ++s) {
+    cx<T> a = {0, 0}, b = {0, 0}, c = {0, 0};
+    if (ok && F::in_valid(j, s)) {
+      const size_t m = (size_t)F::in_index(j, s) * inner + col;
+      a = in[m];
+      b = in[cs + m];
+      c = in[2 * cs + m];
+      const Mode3 q(m, N, KZD);
+      if (!q.keep) {
+        a = b = c = {0, 0};
+      } else if (q.k2 > 0) {
+        const T inv = (T)1 / (T)q.k2;
+        const cx<T> pr = {(a.re * q.kx + b.re * q.ky + c.re * q.kz) * inv,
+                          (a.im * q.kx + b.im * q.ky + c.im * q.kz) * inv};
+        a = csub(a, cscale(pr, (T)q.kx));
+        b = csub(b, cscale(pr, (T)q.ky));
+        c = csub(c, cscale(pr, (T)q.kz));
+      }
+      const T w = (q.kz == 0 || 2 * q.kz == N) ? (T)1 : (T)0.5;
+      a = cscale(a, w);
+      b = cscale(b, w);
+      c = cscale(c, w);
+    }
+    v[0][s] = a;
+    v[1][s] = b;
+    v[2][s] = c;
+  }
+#pragma unroll
+  for (int cc = 0; cc < 3; ++cc) {
+    F::run(v[cc], s_buf + p, C::LS, j, s_tw, 1);  // each pass ends on a barrier: the tile is free again
+#pragma unroll
+    for (int s = 0; s < F::NOUT; ++s)
+      if (ok && F::out_valid(j, s)) out[cc * cs + (size_t)F::out_index(j, s) * inner + col] = v[cc][s];
+  }
+}
+
+// the forward transform along x of the six products [6][N][N*(N/2+1)] (a, then
+// b) and k_vjp_combine in one pass: out = c (a + i k x b) - nu k^2 lam.  The
+// b components are transformed first (i k x b needs all three per mode), then
+// each a component is transformed and combined.  The transforms take the
+// generic plan's radices in reverse order, so the last pass keeps every lane
+// busy with the fewest outputs per lane (radix (16, 8) at N = 128: 8 outputs
+// per lane instead of 16 on half the lanes), which keeps the three transformed
+// b components in registers.
+template <typename T, int N>
+__global__ void __launch_bounds__((StridedCfg<T, N>::THREADS))
+k_adj_x_combine(const cx<T>* __restrict__ ab, const cx<T>* __restrict__ lam, cx<T>* __restrict__ out, int KZD, T nu,
+                const cx<T>* __restrict__ tw) {
+  using C = StridedCfg<T, N>;
+  using F = PFFT<T, true, N, C::TP, Reverse<typename Plan<N>::R>>;
+  constexpr int Nh = N / 2 + 1, inner = N * Nh;
+  constexpr size_t cs = (size_t)N * inner;
+  __shared__ cx<T> s_tw[N];
+  __shared__ cx<T> s_buf[N * C::LS];
+  const int tid = threadIdx.x, p = tid % TI, j = tid / TI;
+  stage_table(s_tw, tw, N);
+  __syncthreads();
+  const int col = blockIdx.x * TI + p;
+  const bool ok = col < inner;
+  cx<T> kb[3][F::MAXE];
+#pragma unroll
+  for (int cc = 0; cc < 3; ++cc) {
+#pragma unroll
+    for (int s = 0; s < F::NIN; ++s)
+      kb[cc][s] = (ok && F::in_valid(j, s)) ? ab[(3 + cc) * cs + (size_t)F::in_index(j, s) * inner + col]
+                                            : cx<T>{0, 0};
+    F::run(kb[cc], s_buf + p, C::LS, j, s_tw, 1);
+  }
+#pragma unroll
+  for (int s = 0; s < F::NOUT; ++s) {  // i k x b per mode
+    const Mode3 q((size_t)F::out_index(j, s) * inner + (ok ? col : 0), N, KZD);
+    const cx<T> b0 = kb[0][s], b1 = kb[1][s], b2 = kb[2][s];
+    kb[0][s] = cmuli(csub(cscale(b2, (T)q.ky), cscale(b1, (T)q.kz)));
+    kb[1][s] = cmuli(csub(cscale(b0, (T)q.kz), cscale(b2, (T)q.kx)));
+    kb[2][s] = cmuli(csub(cscale(b1, (T)q.kx), cscale(b0, (T)q.ky)));
+  }
+#pragma unroll
+  for (int cc = 0; cc < 3; ++cc) {
+    cx<T> v[F::MAXE];
+#pragma unroll
+    for (int s = 0; s < F::NIN; ++s)
+      v[s] = (ok && F::in_valid(j, s)) ? ab[cc * cs + (size_t)F::in_index(j, s) * inner + col] : cx<T>{0, 0};
+    F::run(v, s_buf + p, C::LS, j, s_tw, 1);
+#pragma unroll
+    for (int s = 0; s < F::NOUT; ++s) {
+      if (ok && F::out_valid(j, s)) {
+        const size_t m = (size_t)F::out_index(j, s) * inner + col;
+        const Mode3 q(m, N, KZD);
+        const T w = (q.kz == 0 || 2 * q.kz == N) ? (T)1 : (T)2;
+        out[cc * cs + m] = csub(cscale(cadd(v[s], kb[cc][s]), w), cscale(lam[cc * cs + m], nu * (T)q.k2));
+      }
+    }
+  }
+}
+
 // out = P D g - nu k^2 d
 template <typename T>
 __global__ void k_jvp_combine(const cx<T>* __restrict__ g, const cx<T>* __restrict__ d, cx<T>* __restrict__ out, int N,

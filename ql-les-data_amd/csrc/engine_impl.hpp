@@ -388,11 +388,12 @@ class EngineT final : public Engine {
       up = AP_[0];
       wp = AP_[1];
     }
-    hipLaunchKernelGGL((k_pd<T>), dim3(nblk(rows_ * Nh)), dim3(256), 0, s, l, AS_[8], N, KZD, 1);
+    // psi = irfftn(P D l / c): P D / c fused with the x transform (k_adj_pd_x),
+    // then y; along z, fused with both products (w x psi, psi x u) and their
+    // forward transforms along z (k_adj_z)
+    hipLaunchKernelGGL((k_adj_pd_x<T, N>), dim3((N * Nh + TI - 1) / TI), dim3(StridedCfg<T, N>::THREADS), 0, s, l,
+                       AS_[9], KZD, twN_);
     QL_LAUNCHED();
-    // psi = irfftn(P D l / c) along x and y; along z, fused with both products
-    // (w x psi, psi x u) and their forward transforms along z (k_adj_z)
-    c2c_strided<T>(N, AS_[8], AS_[9], 3, N * Nh, false, (T)1, twN_, s);
     c2c_strided<T>(N, AS_[9], AS_[9], 3 * N, Nh, false, (T)1, twN_, s);
     {
       using ZC = AdjZCfg<N>;
@@ -402,11 +403,11 @@ class EngineT final : public Engine {
                          twN_, h);
       QL_LAUNCHED();
     }
-    // the six products' y and x forward transforms (AS_[8] follows AS_[7])
+    // the six products' y forward transforms (AS_[8] follows AS_[7]), then
+    // their x transforms fused with the combination (k_adj_x_combine)
     c2c_strided<T>(N, AS_[7], AS_[7], 6 * N, Nh, true, (T)1, twN_, s);
-    c2c_strided<T>(N, AS_[7], AS_[7], 6, N * Nh, true, (T)1, twN_, s);
-    hipLaunchKernelGGL((k_vjp_combine<T>), dim3(nblk(rows_ * Nh)), dim3(256), 0, s, AS_[7], AS_[8], l, out, N, KZD,
-                       (T)d_.nu);
+    hipLaunchKernelGGL((k_adj_x_combine<T, N>), dim3((N * Nh + TI - 1) / TI), dim3(StridedCfg<T, N>::THREADS), 0, s,
+                       AS_[7], l, out, KZD, (T)d_.nu, twN_);
     QL_LAUNCHED();
     if (d_.forcing == QLLES_FORCING_POWER) {
       hipLaunchKernelGGL((k_force_tangent<T>), dim3(1), dim3(256), 0, s, y, l, out, N, d_.kf * d_.kf, d_.power, 1,
