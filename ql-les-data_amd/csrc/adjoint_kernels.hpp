@@ -14,6 +14,11 @@
 #pragma once
 #include "ns3d.hpp"
 
+// FP32 occupancy target of k_adj_x_combine (waves per SIMD; 1 leaves it to the compiler)
+#ifndef QL_ADJ_COMBINE_WPE32
+#define QL_ADJ_COMBINE_WPE32 3
+#endif
+
 namespace qk {
 
 // the per-mode wavenumbers and masks used by the fused RHS kernels
@@ -146,8 +151,10 @@ k_adj_pd_x(const cx<T>* __restrict__ in, cx<T>* __restrict__ out, int KZD, const
 // busy with the fewest outputs per lane (radix (16, 8) at N = 128: 8 outputs
 // per lane instead of 16 on half the lanes), which keeps the three transformed
 // b components in registers.
+template <typename T> struct AdjCombineWpe { static constexpr int value = sizeof(T) == 4 ? QL_ADJ_COMBINE_WPE32 : 1; };
 template <typename T, int N>
 __global__ void __launch_bounds__((StridedCfg<T, N>::THREADS))
+__attribute__((amdgpu_waves_per_eu(AdjCombineWpe<T>::value)))
 k_adj_x_combine(const cx<T>* __restrict__ ab, const cx<T>* __restrict__ lam, cx<T>* __restrict__ out, int KZD, T nu,
                 const cx<T>* __restrict__ tw) {
   using C = StridedCfg<T, N>;
